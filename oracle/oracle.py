@@ -1,0 +1,293 @@
+"""ORACLE — test infrastructure only, never part of the product.
+
+ctypes front end for the plain-C restatement in ``oracle/sw_oracle.c`` (FP32
+and FP64 builds, ``oracle/libswo.so``) and, where it was built, for the
+reference's own CPU kernels in ``oracle/_ref/``.  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import it.
+
+The composites below restate which kernel passes each reference primitive
+runs and how it contracts them:
+
+* ``forward``            -> src/sw*/torch_cpu.cpp ``soft_sw*_cpu_with_grads``
+                            (forward + backward: value, map, parameter grads)
+* ``hvp``                -> ``soft_sw*_hvp_cpu`` (forward + HVP)
+* ``param_jacobian``     -> ``soft_sw*_param_jacobian_cpu`` (forward + backward +
+                            U/W sensitivity pass)
+* ``backward_full``      -> ``soft_sw*_backward_full_cpu`` (forward + backward +
+                            HVP + one sensitivity pass per parameter, each
+                            contracted with the cotangent and summed)
+
+All arrays are numpy; ``lengths`` defaults to the full shape.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = {}
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build(ref: bool = False) -> None:
+    """Compile the restatement (and the reference kernels when ``ref``)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    if ref:
+        subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libswo.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = ctypes.CDLL(path)
+    return _LIB
+
+
+def ref_available() -> bool:
+    return all(
+        os.path.exists(os.path.join(_HERE, "_ref", f"lib{n}_ref.so"))
+        for n in ("sw", "sw_affine")
+    )
+
+
+def _ref_lib(name: str):
+    if name not in _REF:
+        _REF[name] = ctypes.CDLL(os.path.join(_HERE, "_ref", f"lib{name}_ref.so"))
+    return _REF[name]
+
+
+class _Kernels:
+    """Resolve the eight kernels of one implementation and precision."""
+
+    def __init__(self, impl: str, prec: str):
+        if impl == "ref":
+            if prec != "f32":
+                raise ValueError("the reference kernels are FP32 only")
+            la, ll = _ref_lib("sw_affine"), _ref_lib("sw")
+            self.aff_forward = la.sw_affine_forward_cpu
+            self.aff_backward = la.sw_affine_backward_cpu
+            self.aff_hvp = la.sw_affine_hvp_cpu
+            self.aff_param_grad = la.sw_affine_param_grad_cpu
+            self.lin_forward = ll.sw_regular_forward_cpu
+            self.lin_backward = ll.sw_regular_backward_cpu
+            self.lin_hvp = ll.sw_regular_hvp_cpu
+            self.lin_param_grad = ll.sw_regular_param_grad_cpu
+        else:
+            lib = _lib()
+            for name in (
+                "aff_forward", "aff_backward", "aff_hvp", "aff_param_grad",
+                "lin_forward", "lin_backward", "lin_hvp", "lin_param_grad",
+            ):
+                setattr(self, name, getattr(lib, f"oracle_{prec}_{name}"))
+        self.dtype = np.float32 if prec == "f32" else np.float64
+        self.ctype = ctypes.c_float if prec == "f32" else ctypes.c_double
+
+
+def _kernels(impl: str, prec: str) -> _Kernels:
+    return _Kernels(impl, prec)
+
+
+def _arr(x, dtype):
+    return np.ascontiguousarray(x, dtype=dtype)
+
+
+def _ptr(a):
+    if a.dtype == np.float32:
+        return a.ctypes.data_as(_fp)
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    return a.ctypes.data_as(_ip)
+
+
+def _lens(scores, lengths):
+    B, L1, L2 = scores.shape
+    if lengths is None:
+        lengths = np.tile(np.array([L1, L2], dtype=np.int32), (B, 1))
+    return np.ascontiguousarray(lengths, dtype=np.int32)
+
+
+def _setup(scores, lengths, impl, prec):
+    k = _kernels(impl, prec)
+    s = _arr(scores, k.dtype)
+    ln = _lens(s, lengths)
+    B, L1, L2 = s.shape
+    return k, s, ln, B, L1, L2
+
+
+# --------------------------------------------------------------------------
+# Affine gap (src/sw_affine/torch_cpu.cpp)
+# --------------------------------------------------------------------------
+
+def _aff_fwd_bwd(k, s, ln, B, L1, L2, go, ge, T):
+    C = k.ctype
+    asz = 3 * (L1 + 1) * (L2 + 1)
+    alpha = np.zeros((B, asz), k.dtype)
+    part = np.zeros(B, k.dtype)
+    k.aff_forward(_ptr(s), _ptr(alpha), _ptr(part), _ptr(ln), B, L1, L2, C(go), C(ge), C(T))
+    beta = np.zeros((B, asz), k.dtype)
+    post = np.zeros((B, L1, L2), k.dtype)
+    g = [np.zeros(B, k.dtype) for _ in range(3)]
+    k.aff_backward(_ptr(alpha), _ptr(s), _ptr(part), _ptr(beta), _ptr(post),
+                   _ptr(g[0]), _ptr(g[1]), _ptr(g[2]), _ptr(ln), B, L1, L2,
+                   C(go), C(ge), C(T))
+    return alpha, part, post, g
+
+
+def sw_affine_forward(scores, gap_open, gap_ext, temp, lengths=None, impl="oracle", prec="f32"):
+    """(value[B], map[B,L1,L2], grad_open[B], grad_ext[B], grad_temp[B])."""
+    k, s, ln, B, L1, L2 = _setup(scores, lengths, impl, prec)
+    _, part, post, g = _aff_fwd_bwd(k, s, ln, B, L1, L2, gap_open, gap_ext, temp)
+    return part, post, g[0], g[1], g[2]
+
+
+def _aff_hvp_from(k, s, ln, B, L1, L2, alpha, part, V, go, ge, T):
+    C = k.ctype
+    asz = 3 * (L1 + 1) * (L2 + 1)
+    v = _arr(V, k.dtype)
+    da = np.zeros((B, asz), k.dtype)
+    dp = np.zeros(B, k.dtype)
+    db = np.zeros((B, asz), k.dtype)
+    H = np.zeros((B, L1, L2), k.dtype)
+    k.aff_hvp(_ptr(alpha), _ptr(s), _ptr(part), _ptr(v), _ptr(da), _ptr(dp), _ptr(db),
+              _ptr(H), _ptr(ln), B, L1, L2, C(go), C(ge), C(T))
+    return H
+
+
+def sw_affine_hvp(scores, V, gap_open, gap_ext, temp, lengths=None, impl="oracle", prec="f32"):
+    k, s, ln, B, L1, L2 = _setup(scores, lengths, impl, prec)
+    C = k.ctype
+    asz = 3 * (L1 + 1) * (L2 + 1)
+    alpha = np.zeros((B, asz), k.dtype)
+    part = np.zeros(B, k.dtype)
+    k.aff_forward(_ptr(s), _ptr(alpha), _ptr(part), _ptr(ln), B, L1, L2,
+                  C(gap_open), C(gap_ext), C(temp))
+    return _aff_hvp_from(k, s, ln, B, L1, L2, alpha, part, V, gap_open, gap_ext, temp)
+
+
+def _aff_pj_from(k, s, ln, B, L1, L2, alpha, part, dS, ptype, go, ge, T):
+    C = k.ctype
+    asz = 3 * (L1 + 1) * (L2 + 1)
+    U = np.zeros((B, asz), k.dtype)
+    be = np.zeros((B, asz), k.dtype)
+    W = np.zeros((B, asz), k.dtype)
+    dP = np.zeros((B, L1, L2), k.dtype)
+    dS = _arr(dS, k.dtype)
+    k.aff_param_grad(_ptr(alpha), _ptr(s), _ptr(part), _ptr(dS), _ptr(U), _ptr(be), _ptr(W),
+                     _ptr(dP), _ptr(ln), B, L1, L2, C(go), C(ge), C(T), int(ptype))
+    return dP
+
+
+def sw_affine_param_jacobian(scores, ptype, gap_open, gap_ext, temp, lengths=None,
+                             impl="oracle", prec="f32"):
+    """d(map)/d(theta), theta = gap_open (0), gap_ext (1), temperature (2)."""
+    k, s, ln, B, L1, L2 = _setup(scores, lengths, impl, prec)
+    alpha, part, _, g = _aff_fwd_bwd(k, s, ln, B, L1, L2, gap_open, gap_ext, temp)
+    return _aff_pj_from(k, s, ln, B, L1, L2, alpha, part, g[ptype], ptype,
+                        gap_open, gap_ext, temp)
+
+
+def sw_affine_backward_full(scores, G, gap_open, gap_ext, temp, lengths=None,
+                            impl="oracle", prec="f32"):
+    """(grad_scores[B,L1,L2], grad_open, grad_ext, grad_temp) for cotangent G."""
+    k, s, ln, B, L1, L2 = _setup(scores, lengths, impl, prec)
+    alpha, part, _, g = _aff_fwd_bwd(k, s, ln, B, L1, L2, gap_open, gap_ext, temp)
+    Gc = _arr(G, k.dtype)
+    H = _aff_hvp_from(k, s, ln, B, L1, L2, alpha, part, Gc, gap_open, gap_ext, temp)
+    out = [H]
+    for p in range(3):
+        dP = _aff_pj_from(k, s, ln, B, L1, L2, alpha, part, g[p], p, gap_open, gap_ext, temp)
+        out.append(np.sum(Gc.astype(np.float64) * dP.astype(np.float64)))
+    return tuple(out)
+
+
+# --------------------------------------------------------------------------
+# Linear gap (src/sw/torch_cpu.cpp)
+# --------------------------------------------------------------------------
+
+def _lin_fwd_bwd(k, s, ln, B, L1, L2, gap, T):
+    C = k.ctype
+    asz = (L1 + 1) * (L2 + 1)
+    alpha = np.zeros((B, asz), k.dtype)
+    part = np.zeros(B, k.dtype)
+    k.lin_forward(_ptr(s), _ptr(alpha), _ptr(part), _ptr(ln), B, L1, L2, C(gap), C(T))
+    beta = np.zeros((B, asz), k.dtype)
+    post = np.zeros((B, L1, L2), k.dtype)
+    gg = np.zeros(B, k.dtype)
+    gT = np.zeros(B, k.dtype)
+    k.lin_backward(_ptr(alpha), _ptr(s), _ptr(part), _ptr(beta), _ptr(post), _ptr(gg),
+                   _ptr(gT), _ptr(ln), B, L1, L2, C(gap), C(T))
+    return alpha, part, post, [gg, gT]
+
+
+def sw_forward(scores, gap, temp, lengths=None, impl="oracle", prec="f32"):
+    """(value[B], map[B,L1,L2], grad_gap[B], grad_temp[B])."""
+    k, s, ln, B, L1, L2 = _setup(scores, lengths, impl, prec)
+    _, part, post, g = _lin_fwd_bwd(k, s, ln, B, L1, L2, gap, temp)
+    return part, post, g[0], g[1]
+
+
+def _lin_hvp_from(k, s, ln, B, L1, L2, alpha, part, V, gap, T):
+    C = k.ctype
+    asz = (L1 + 1) * (L2 + 1)
+    v = _arr(V, k.dtype)
+    da = np.zeros((B, asz), k.dtype)
+    dp = np.zeros(B, k.dtype)
+    be = np.zeros((B, asz), k.dtype)
+    db = np.zeros((B, asz), k.dtype)
+    H = np.zeros((B, L1, L2), k.dtype)
+    k.lin_hvp(_ptr(alpha), _ptr(s), _ptr(part), _ptr(v), _ptr(da), _ptr(dp), _ptr(be), _ptr(db),
+              _ptr(H), _ptr(ln), B, L1, L2, C(gap), C(T))
+    return H
+
+
+def sw_hvp(scores, V, gap, temp, lengths=None, impl="oracle", prec="f32"):
+    k, s, ln, B, L1, L2 = _setup(scores, lengths, impl, prec)
+    C = k.ctype
+    alpha = np.zeros((B, (L1 + 1) * (L2 + 1)), k.dtype)
+    part = np.zeros(B, k.dtype)
+    k.lin_forward(_ptr(s), _ptr(alpha), _ptr(part), _ptr(ln), B, L1, L2, C(gap), C(temp))
+    return _lin_hvp_from(k, s, ln, B, L1, L2, alpha, part, V, gap, temp)
+
+
+def _lin_pj_from(k, s, ln, B, L1, L2, alpha, part, dS, ptype, gap, T):
+    C = k.ctype
+    asz = (L1 + 1) * (L2 + 1)
+    U = np.zeros((B, asz), k.dtype)
+    be = np.zeros((B, asz), k.dtype)
+    W = np.zeros((B, asz), k.dtype)
+    dP = np.zeros((B, L1, L2), k.dtype)
+    dS = _arr(dS, k.dtype)
+    k.lin_param_grad(_ptr(alpha), _ptr(s), _ptr(part), _ptr(dS), _ptr(U), _ptr(be), _ptr(W),
+                     _ptr(dP), _ptr(ln), B, L1, L2, C(gap), C(T), int(ptype))
+    return dP
+
+
+def sw_param_jacobian(scores, ptype, gap, temp, lengths=None, impl="oracle", prec="f32"):
+    """d(map)/d(theta), theta = gap (0), temperature (1)."""
+    k, s, ln, B, L1, L2 = _setup(scores, lengths, impl, prec)
+    alpha, part, _, g = _lin_fwd_bwd(k, s, ln, B, L1, L2, gap, temp)
+    return _lin_pj_from(k, s, ln, B, L1, L2, alpha, part, g[ptype], ptype, gap, temp)
+
+
+def sw_backward_full(scores, G, gap, temp, lengths=None, impl="oracle", prec="f32"):
+    """(grad_scores[B,L1,L2], grad_gap, grad_temp) for cotangent G."""
+    k, s, ln, B, L1, L2 = _setup(scores, lengths, impl, prec)
+    alpha, part, _, g = _lin_fwd_bwd(k, s, ln, B, L1, L2, gap, temp)
+    Gc = _arr(G, k.dtype)
+    H = _lin_hvp_from(k, s, ln, B, L1, L2, alpha, part, Gc, gap, temp)
+    out = [H]
+    for p in range(2):
+        dP = _lin_pj_from(k, s, ln, B, L1, L2, alpha, part, g[p], p, gap, temp)
+        out.append(np.sum(Gc.astype(np.float64) * dP.astype(np.float64)))
+    return tuple(out)

@@ -1,0 +1,138 @@
+"""Write the golden fixtures in tests/golden/*.npz.
+
+Every expected output comes from the reference's own CPU kernels
+(src/sw/kernels_cpu.cpp, src/sw_affine/kernels_cpu.cpp) compiled from their
+sources into oracle/_ref/ by ``make -C oracle ref``; the composites follow
+src/sw*/torch_cpu.cpp (which passes each primitive runs).  The fixtures hold
+inputs and expected outputs only.  Regenerate with
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Inputs use one RNG convention: ``torch.Generator().manual_seed(seed)`` and
+``torch.randn(shape, generator=g, dtype=torch.float32)``, drawn in the order
+scores, then V (HVP direction), then G (map cotangent).
+"""
+
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import oracle as O  # noqa: E402
+
+
+def _draw(seed, shape, scale=1.0, shift=0.0, extra=2):
+    g = torch.Generator().manual_seed(seed)
+    s = torch.randn(shape, generator=g, dtype=torch.float32) * scale + shift
+    rest = [torch.randn(shape, generator=g, dtype=torch.float32) for _ in range(extra)]
+    return [s.numpy()] + [r.numpy() for r in rest]
+
+
+def _sentinel(scores, lengths, T):
+    """Padded-and-masked input: cells beyond each pair's lengths get the
+    finite exclusion sentinel amax - ((L1+L2)*80 + 200)*T of
+    orihime/operator.py:334-386 (amax over the retained cells)."""
+    B, L1, L2 = scores.shape
+    out = scores.copy()
+    for b in range(B):
+        mask = np.ones((L1, L2), bool)
+        mask[: lengths[b, 0], : lengths[b, 1]] = False
+        kept = scores[b][~mask]
+        amax = kept.max() if kept.size else 0.0
+        out[b][mask] = np.float32(amax - ((L1 + L2) * 80.0 + 200.0) * T)
+    return out
+
+
+def affine_case(name, s, V, G, go, ge, T, lengths=None, full=True):
+    B, L1, L2 = s.shape
+    if lengths is None:
+        lengths = np.tile(np.array([L1, L2], np.int32), (B, 1))
+    d = dict(kind="affine", scores=s, lengths=lengths, gap_open=np.float32(go),
+             gap_ext=np.float32(ge), temp=np.float32(T))
+    v, m, go_, ge_, gt = O.sw_affine_forward(s, go, ge, T, lengths, impl="ref")
+    d.update(value=v, map=m, grad_open=go_, grad_ext=ge_, grad_temp=gt)
+    if full:
+        d["V"] = V
+        d["hvp"] = O.sw_affine_hvp(s, V, go, ge, T, lengths, impl="ref")
+        for p, n in enumerate(("gap_open", "gap_ext", "temp")):
+            d[f"sens_{n}"] = O.sw_affine_param_jacobian(s, p, go, ge, T, lengths, impl="ref")
+        d["G"] = G
+        gs, a, b, c = O.sw_affine_backward_full(s, G, go, ge, T, lengths, impl="ref")
+        d.update(mb_scores=gs, mb_gap_open=np.float64(a), mb_gap_ext=np.float64(b),
+                 mb_temp=np.float64(c))
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **d)
+    return d
+
+
+def linear_case(name, s, V, G, gap, T, lengths=None, full=True):
+    B, L1, L2 = s.shape
+    if lengths is None:
+        lengths = np.tile(np.array([L1, L2], np.int32), (B, 1))
+    d = dict(kind="linear", scores=s, lengths=lengths, gap=np.float32(gap), temp=np.float32(T))
+    v, m, gg, gt = O.sw_forward(s, gap, T, lengths, impl="ref")
+    d.update(value=v, map=m, grad_gap=gg, grad_temp=gt)
+    if full:
+        d["V"] = V
+        d["hvp"] = O.sw_hvp(s, V, gap, T, lengths, impl="ref")
+        for p, n in enumerate(("gap", "temp")):
+            d[f"sens_{n}"] = O.sw_param_jacobian(s, p, gap, T, lengths, impl="ref")
+        d["G"] = G
+        gs, a, b = O.sw_backward_full(s, G, gap, T, lengths, impl="ref")
+        d.update(mb_scores=gs, mb_gap=np.float64(a), mb_temp=np.float64(b))
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **d)
+    return d
+
+
+def main():
+    if not O.ref_available():
+        raise SystemExit("build the reference kernels first: make -C oracle ref")
+    # Linear gap, B=8, 128x128, N(0,1) seed 0, gap=-1, T=1: value, map and
+    # the parameter gradients at full size; second-order outputs on a crop.
+    s, V, G = _draw(0, (8, 128, 128))
+    linear_case("linear_b8_128x128_seed0", s, V, G, -1.0, 1.0, full=False)
+    linear_case("linear_b2_32x32_seed0_crop", s[:2, :32, :32].copy(), V[:2, :32, :32].copy(),
+                G[:2, :32, :32].copy(), -1.0, 1.0)
+    # Affine, N(0,1)*2-0.5 seed 1, go=-11, ge=-1, T=1 (crop of B=256 512x512).
+    s, V, G = _draw(1, (2, 48, 40), 2.0, -0.5)
+    affine_case("affine_b2_48x40_seed1", s, V, G, -11.0, -1.0, 1.0)
+    # Ragged affine, go=-8, ge=-0.5, T=0.5: lengths form and padded+sentinel form.
+    s, V, G = _draw(2, (4, 40, 36))
+    ln = np.array([[40, 36], [17, 29], [33, 6], [1, 23]], np.int32)
+    affine_case("affine_ragged_b4_40x36_seed2", s, V, G, -8.0, -0.5, 0.5, ln)
+    affine_case("affine_padmask_b4_40x36_seed2", _sentinel(s, ln, 0.5), V, G, -8.0, -0.5, 0.5,
+                full=False)
+    # Long-sequence crop, N(0,1) seed 3, go=-11, ge=-1, T=1.
+    s, V, G = _draw(3, (1, 64, 64))
+    affine_case("affine_b1_64x64_seed3", s, V, G, -11.0, -1.0, 1.0)
+    # Reference test-suite style edge cases (tests/test_soft_sw_affine.py:295-348).
+    edge = [
+        ("1x1", (1, 1, 1), -1.0, -0.3, 1.0),
+        ("1x3", (1, 1, 3), -1.0, -0.3, 1.0),
+        ("3x1", (1, 3, 1), -1.0, -0.3, 1.0),
+        ("lowT", (2, 8, 10), -1.0, -0.3, 0.01),
+        ("highT", (2, 8, 10), -1.0, -0.3, 10.0),
+        ("gaps_m1m1", (2, 5, 20), -1.0, -1.0, 1.0),
+        ("gaps_m2_0", (2, 16, 16), -2.0, 0.0, 1.0),
+        ("gaps_pos", (2, 8, 10), 0.5, 0.2, 1.0),
+    ]
+    for i, (tag, shape, go, ge, T) in enumerate(edge):
+        s, V, G = _draw(40 + i, shape)
+        affine_case(f"affine_edge_{tag}", s, V, G, go, ge, T)
+        linear_case(f"linear_edge_{tag}", s, V, G, go, T)
+    # Zero lengths and boundary-heavy scores (test_soft_sw_affine.py:220-264).
+    s, V, G = _draw(60, (3, 9, 11))
+    s[:, 0, :] -= 2.5
+    ln = np.array([[0, 11], [9, 0], [4, 7]], np.int32)
+    affine_case("affine_zero_lengths", s, V, G, -1.0, -0.3, 0.9, ln)
+    linear_case("linear_zero_lengths", s, V, G, -0.7, 0.9, ln)
+    print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
+
+
+if __name__ == "__main__":
+    main()
