@@ -1,0 +1,180 @@
+/*
+ * ohm_sw.h — C ABI of the MI355X (gfx950) soft Smith-Waterman hot path.
+ *
+ * One shared library, orihime_amd/libohm_sw.so, exports everything declared
+ * here.  Signatures use plain pointers, sizes and scalars only (no torch
+ * types), so the library can be bound from ctypes, cgo, JNI or N-API.
+ *
+ * The operations replace the reference's native layer for soft
+ * Smith-Waterman (linear gap, src/sw; affine gap, src/sw_affine):
+ *   - the dispatcher schemas src/sw_affine/registry.cpp:27-42 and
+ *     src/sw/registry.cpp:30-44 (what orihime/_ops.py:434-449 calls), and
+ *   - the extern "C" kernel launchers below them,
+ *     src/sw_affine/kernels_gpu.hiph:165-265, src/sw/kernels_gpu.hiph,
+ *     src/sw_affine/kernels_cpu.h:80-163, src/sw/kernels_cpu.h.
+ * Each declaration names the reference primitive(s) it serves.
+ *
+ * Problem description, shared by every entry point:
+ *   scores    FP32 pair scores.  offsets == NULL: padded [B, L1max, L2max]
+ *             row-major (row stride L2max).  offsets != NULL: packed — pair b
+ *             occupies lengths[b,0] x lengths[b,1] floats starting at element
+ *             offsets[b] (row stride lengths[b,1]).
+ *   lengths   int32 [B, 2] active lengths (L1, L2) of each pair, 0 <= L <= max.
+ *             NULL means every pair uses the full (L1max, L2max) (padded only).
+ *   offsets   int64 [B] or NULL, see above.
+ *   Map-shaped outputs and directions use the same layout as scores.  In the
+ *   padded layout, cells outside a pair's lengths are written as 0.
+ *   All GPU pointers are device pointers; *_cpu entry points take host
+ *   pointers.
+ *
+ * GPU contract: the call only enqueues work on `stream` (a hipStream_t, NULL =
+ * default stream).  It allocates nothing, never synchronises the host, and
+ * reads parameters by value, so it can be captured in a HIP graph.  The
+ * caller provides `workspace` of at least the size returned by the matching
+ * *_workspace query.  Results are bitwise repeatable: every reduction runs in
+ * a fixed order and no floating-point atomics are used.
+ *
+ * Errors are returned as status codes (never thrown); ohm_last_error() gives
+ * a message for the last failure on the calling thread.
+ */
+#ifndef OHM_SW_H
+#define OHM_SW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ohm_status {
+    OHM_OK = 0,
+    OHM_EINVAL = 1,      /* invalid argument (shape, length, temperature, pointer) */
+    OHM_EWORKSPACE = 2,  /* workspace missing or smaller than required */
+    OHM_ELAUNCH = 3,     /* HIP launch or runtime failure */
+} ohm_status;
+
+/* Message describing the last non-OK status returned on this thread. */
+const char* ohm_last_error(void);
+
+/* Library version, "major.minor.patch". */
+const char* ohm_version(void);
+
+/* Operation kinds for the workspace queries. */
+#define OHM_OP_VALUE 0    /* value only */
+#define OHM_OP_FORWARD 1  /* value + map + parameter gradients */
+#define OHM_OP_TANGENT 2  /* forward-over-reverse: HVP, sensitivity, map cotangent */
+
+/* Workspace bytes for one call.  total_cells is sum_b L1_b*L2_b for the packed
+ * layout, or -1 for the padded layout (bounded by B*L1max*L2max). */
+size_t ohm_sw_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
+                               int64_t total_cells);
+size_t ohm_sw_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
+                        int64_t total_cells);
+
+/* ------------------------------------------------------------------------ */
+/* Affine gap (three states M, I, D)                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Value, marginal map and scalar-parameter gradients.
+ *   value[B]                 partition value S_b (always written)
+ *   map (nullable)           dS/dscores, padded-layout zeros outside lengths
+ *   grad_open/ext/temp[B]    (nullable) dS/d(gap_open), dS/d(gap_ext),
+ *                            dS/d(temperature) = entropy
+ * When map and all three gradients are NULL only the forward sweep runs.
+ * Replaces: sw_affine_forward, sw_affine_forward_t (forward part),
+ * sw_affine_value_grad_params, soft_sw_affine_with_grads
+ * (src/sw_affine/torch_hip.cpp:128-208, :446, :820) and the launchers
+ * sw_affine_forward_hip / sw_affine_backward_hip (kernels_gpu.hiph). */
+ohm_status ohm_sw_affine_forward(const float* scores, const int32_t* lengths,
+                                 const int64_t* offsets, int32_t B, int32_t L1max,
+                                 int32_t L2max, float gap_open, float gap_ext, float temp,
+                                 float* value, float* map, float* grad_open, float* grad_ext,
+                                 float* grad_temp, void* workspace, size_t workspace_bytes,
+                                 void* stream);
+
+/* Forward-over-reverse pass along one direction: a score-space direction
+ * `direction` (nullable, same layout as scores) plus scalar parameter
+ * directions (d_gap_open, d_gap_ext, d_temp).
+ *   map_tangent (nullable)   directional derivative of the map:
+ *                            direction only            -> HVP  (H . v)
+ *                            d_gap_open = 1 only       -> dmap/dgap_open, etc.
+ *   contract[B*3] (nullable) per pair, the derivatives along `direction` of
+ *                            (dS/dgap_open, dS/dgap_ext, dS/dtemp); summed over
+ *                            pairs these are the parameter gradients of
+ *                            <cotangent, map> with cotangent = direction.
+ *   value, map (nullable)    primal value and map, computed on the way.
+ * Replaces: sw_affine_marginals_hvp, sw_affine_marginals_backward,
+ * sw_affine_marginals_grad_{gap_open,gap_ext,temp}, soft_sw_affine_hvp,
+ * soft_sw_affine_param_jacobian, soft_sw_affine_backward_full
+ * (src/sw_affine/torch_hip.cpp:508-678) and the launchers sw_affine_hvp_hip /
+ * sw_affine_param_grad_hip (kernels_gpu.hiph). */
+ohm_status ohm_sw_affine_tangent(const float* scores, const int32_t* lengths,
+                                 const int64_t* offsets, int32_t B, int32_t L1max,
+                                 int32_t L2max, float gap_open, float gap_ext, float temp,
+                                 const float* direction, float d_gap_open, float d_gap_ext,
+                                 float d_temp, float* map_tangent, float* contract,
+                                 float* value, float* map, void* workspace,
+                                 size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Linear gap (one state)                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* As ohm_sw_affine_forward with one gap score; gradients dS/dgap and
+ * dS/dtemperature.  Replaces: sw_forward, sw_forward_t (forward part),
+ * sw_value_grad_params, soft_sw_with_grads (src/sw/torch_hip.cpp) and the
+ * launchers sw_regular_forward_hip / sw_regular_backward_hip
+ * (src/sw/kernels_gpu.hip:1145-1299). */
+ohm_status ohm_sw_forward(const float* scores, const int32_t* lengths, const int64_t* offsets,
+                          int32_t B, int32_t L1max, int32_t L2max, float gap, float temp,
+                          float* value, float* map, float* grad_gap, float* grad_temp,
+                          void* workspace, size_t workspace_bytes, void* stream);
+
+/* As ohm_sw_affine_tangent; contract is [B*2] (gap, temperature).
+ * Replaces: sw_marginals_hvp, sw_marginals_backward, sw_marginals_grad_gap,
+ * sw_marginals_grad_temp and the launchers sw_regular_hvp_hip /
+ * sw_regular_param_grad_hip (src/sw/kernels_gpu.hip:1301-1422). */
+ohm_status ohm_sw_tangent(const float* scores, const int32_t* lengths, const int64_t* offsets,
+                          int32_t B, int32_t L1max, int32_t L2max, float gap, float temp,
+                          const float* direction, float d_gap, float d_temp,
+                          float* map_tangent, float* contract, float* value, float* map,
+                          void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Host (CPU) twins: same arguments on host memory, no workspace or stream. */
+/* `threads` <= 0 uses every hardware thread.  Replace the reference CPU     */
+/* dispatch (src/sw_affine/torch_cpu.cpp:593-707, src/sw/torch_cpu.cpp:522-591) */
+/* and kernels (src/sw/kernels_cpu.cpp, src/sw_affine/kernels_cpu.cpp).       */
+/* ------------------------------------------------------------------------ */
+
+ohm_status ohm_sw_affine_forward_cpu(const float* scores, const int32_t* lengths,
+                                     const int64_t* offsets, int32_t B, int32_t L1max,
+                                     int32_t L2max, float gap_open, float gap_ext, float temp,
+                                     float* value, float* map, float* grad_open,
+                                     float* grad_ext, float* grad_temp, int32_t threads);
+
+ohm_status ohm_sw_affine_tangent_cpu(const float* scores, const int32_t* lengths,
+                                     const int64_t* offsets, int32_t B, int32_t L1max,
+                                     int32_t L2max, float gap_open, float gap_ext, float temp,
+                                     const float* direction, float d_gap_open,
+                                     float d_gap_ext, float d_temp, float* map_tangent,
+                                     float* contract, float* value, float* map,
+                                     int32_t threads);
+
+ohm_status ohm_sw_forward_cpu(const float* scores, const int32_t* lengths,
+                              const int64_t* offsets, int32_t B, int32_t L1max, int32_t L2max,
+                              float gap, float temp, float* value, float* map, float* grad_gap,
+                              float* grad_temp, int32_t threads);
+
+ohm_status ohm_sw_tangent_cpu(const float* scores, const int32_t* lengths,
+                              const int64_t* offsets, int32_t B, int32_t L1max, int32_t L2max,
+                              float gap, float temp, const float* direction, float d_gap,
+                              float d_temp, float* map_tangent, float* contract, float* value,
+                              float* map, int32_t threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OHM_SW_H */

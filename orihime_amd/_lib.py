@@ -1,0 +1,211 @@
+"""ctypes binding of libohm_sw.so (C ABI in include/ohm_sw.h).
+
+This is the only module that touches the native library.  It raises
+ImportError when the library is missing — there is no Python fallback — and
+passes raw data pointers, sizes and the current HIP stream to the C entry
+points.  Tensors on a GPU go to the HIP kernels, CPU tensors to the host
+implementation in the same library.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libohm_sw.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"orihime_amd native library not found at {LIB_PATH}; build it with "
+        "`python -m orihime_amd.build_native` (requires ROCm hipcc)"
+    )
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+_f = ctypes.c_float
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+
+EXPORTS = (
+    "ohm_last_error",
+    "ohm_version",
+    "ohm_sw_affine_workspace",
+    "ohm_sw_workspace",
+    "ohm_sw_affine_forward",
+    "ohm_sw_affine_tangent",
+    "ohm_sw_forward",
+    "ohm_sw_tangent",
+    "ohm_sw_affine_forward_cpu",
+    "ohm_sw_affine_tangent_cpu",
+    "ohm_sw_forward_cpu",
+    "ohm_sw_tangent_cpu",
+)
+
+
+def _sig(name, restype, *argtypes):
+    fn = getattr(_lib, name)
+    fn.restype = restype
+    fn.argtypes = list(argtypes)
+    return fn
+
+
+_last_error = _sig("ohm_last_error", ctypes.c_char_p)
+_version = _sig("ohm_version", ctypes.c_char_p)
+_ws_affine = _sig("ohm_sw_affine_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
+_ws_linear = _sig("ohm_sw_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
+
+_PROBLEM = (_vp, _vp, _vp, _i32, _i32, _i32)  # scores, lengths, offsets, B, L1max, L2max
+_aff_fwd = _sig("ohm_sw_affine_forward", _i32, *_PROBLEM, _f, _f, _f,
+                _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp)
+_aff_tan = _sig("ohm_sw_affine_tangent", _i32, *_PROBLEM, _f, _f, _f,
+                _vp, _f, _f, _f, _vp, _vp, _vp, _vp, _vp, _sz, _vp)
+_lin_fwd = _sig("ohm_sw_forward", _i32, *_PROBLEM, _f, _f,
+                _vp, _vp, _vp, _vp, _vp, _sz, _vp)
+_lin_tan = _sig("ohm_sw_tangent", _i32, *_PROBLEM, _f, _f,
+                _vp, _f, _f, _vp, _vp, _vp, _vp, _vp, _sz, _vp)
+_aff_fwd_cpu = _sig("ohm_sw_affine_forward_cpu", _i32, *_PROBLEM, _f, _f, _f,
+                    _vp, _vp, _vp, _vp, _vp, _i32)
+_aff_tan_cpu = _sig("ohm_sw_affine_tangent_cpu", _i32, *_PROBLEM, _f, _f, _f,
+                    _vp, _f, _f, _f, _vp, _vp, _vp, _vp, _i32)
+_lin_fwd_cpu = _sig("ohm_sw_forward_cpu", _i32, *_PROBLEM, _f, _f,
+                    _vp, _vp, _vp, _vp, _i32)
+_lin_tan_cpu = _sig("ohm_sw_tangent_cpu", _i32, *_PROBLEM, _f, _f,
+                    _vp, _f, _f, _vp, _vp, _vp, _vp, _i32)
+
+OP_VALUE, OP_FORWARD, OP_TANGENT = 0, 1, 2
+
+
+def version() -> str:
+    return _version().decode()
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        msg = _last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed (status {status}): {msg}")
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+# --------------------------------------------------------------------------
+# Device workspace: one growing byte buffer per (device, stream).
+# --------------------------------------------------------------------------
+
+_ws_lock = threading.Lock()
+_ws_cache: dict[tuple[int, int], torch.Tensor] = {}
+
+
+def workspace_bytes(affine: bool, op: int, B: int, L1: int, L2: int, total_cells: int = -1) -> int:
+    fn = _ws_affine if affine else _ws_linear
+    return int(fn(op, B, L1, L2, total_cells))
+
+
+def _workspace(device: torch.device, nbytes: int, stream: int) -> torch.Tensor:
+    key = (device.index if device.index is not None else torch.cuda.current_device(), stream)
+    with _ws_lock:
+        buf = _ws_cache.get(key)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, device=device)
+            _ws_cache[key] = buf
+        return buf
+
+
+def release_workspaces() -> None:
+    with _ws_lock:
+        _ws_cache.clear()
+
+
+def _cpu_threads() -> int:
+    return int(torch.get_num_threads())
+
+
+# --------------------------------------------------------------------------
+# Entry points.  All tensors must already be validated, contiguous FP32 (int32
+# lengths, int64 offsets) on one device; outputs are allocated by the caller.
+# --------------------------------------------------------------------------
+
+
+def _dims(scores: torch.Tensor, lengths, offsets, L1max, L2max):
+    if offsets is None:
+        B, L1, L2 = scores.shape
+        return B, L1, L2
+    return int(offsets.shape[0]), int(L1max), int(L2max)
+
+
+def affine_forward(scores, lengths, offsets, gap_open, gap_ext, temp, *, value, map=None,
+                   grad_open=None, grad_ext=None, grad_temp=None, L1max=None, L2max=None,
+                   workspace=None):
+    B, L1, L2 = _dims(scores, lengths, offsets, L1max, L2max)
+    args = (_ptr(scores), _ptr(lengths), _ptr(offsets), B, L1, L2,
+            float(gap_open), float(gap_ext), float(temp),
+            _ptr(value), _ptr(map), _ptr(grad_open), _ptr(grad_ext), _ptr(grad_temp))
+    if scores.is_cuda:
+        op = OP_FORWARD if (map is not None or grad_open is not None or grad_ext is not None
+                            or grad_temp is not None) else OP_VALUE
+        total = -1 if offsets is None else scores.numel()
+        need = workspace_bytes(True, op, B, L1, L2, total)
+        stream = torch.cuda.current_stream(scores.device).cuda_stream
+        ws = workspace if workspace is not None else _workspace(scores.device, need, stream)
+        _check(_aff_fwd(*args, ws.data_ptr(), ws.numel(), stream), "ohm_sw_affine_forward")
+    else:
+        _check(_aff_fwd_cpu(*args, _cpu_threads()), "ohm_sw_affine_forward_cpu")
+
+
+def affine_tangent(scores, lengths, offsets, gap_open, gap_ext, temp, *, direction=None,
+                   d_gap_open=0.0, d_gap_ext=0.0, d_temp=0.0, map_tangent=None, contract=None,
+                   value=None, map=None, L1max=None, L2max=None, workspace=None):
+    B, L1, L2 = _dims(scores, lengths, offsets, L1max, L2max)
+    args = (_ptr(scores), _ptr(lengths), _ptr(offsets), B, L1, L2,
+            float(gap_open), float(gap_ext), float(temp), _ptr(direction),
+            float(d_gap_open), float(d_gap_ext), float(d_temp),
+            _ptr(map_tangent), _ptr(contract), _ptr(value), _ptr(map))
+    if scores.is_cuda:
+        total = -1 if offsets is None else scores.numel()
+        need = workspace_bytes(True, OP_TANGENT, B, L1, L2, total)
+        stream = torch.cuda.current_stream(scores.device).cuda_stream
+        ws = workspace if workspace is not None else _workspace(scores.device, need, stream)
+        _check(_aff_tan(*args, ws.data_ptr(), ws.numel(), stream), "ohm_sw_affine_tangent")
+    else:
+        _check(_aff_tan_cpu(*args, _cpu_threads()), "ohm_sw_affine_tangent_cpu")
+
+
+def linear_forward(scores, lengths, offsets, gap, temp, *, value, map=None, grad_gap=None,
+                   grad_temp=None, L1max=None, L2max=None, workspace=None):
+    B, L1, L2 = _dims(scores, lengths, offsets, L1max, L2max)
+    args = (_ptr(scores), _ptr(lengths), _ptr(offsets), B, L1, L2, float(gap), float(temp),
+            _ptr(value), _ptr(map), _ptr(grad_gap), _ptr(grad_temp))
+    if scores.is_cuda:
+        op = OP_FORWARD if (map is not None or grad_gap is not None or grad_temp is not None) \
+            else OP_VALUE
+        total = -1 if offsets is None else scores.numel()
+        need = workspace_bytes(False, op, B, L1, L2, total)
+        stream = torch.cuda.current_stream(scores.device).cuda_stream
+        ws = workspace if workspace is not None else _workspace(scores.device, need, stream)
+        _check(_lin_fwd(*args, ws.data_ptr(), ws.numel(), stream), "ohm_sw_forward")
+    else:
+        _check(_lin_fwd_cpu(*args, _cpu_threads()), "ohm_sw_forward_cpu")
+
+
+def linear_tangent(scores, lengths, offsets, gap, temp, *, direction=None, d_gap=0.0,
+                   d_temp=0.0, map_tangent=None, contract=None, value=None, map=None,
+                   L1max=None, L2max=None, workspace=None):
+    B, L1, L2 = _dims(scores, lengths, offsets, L1max, L2max)
+    args = (_ptr(scores), _ptr(lengths), _ptr(offsets), B, L1, L2, float(gap), float(temp),
+            _ptr(direction), float(d_gap), float(d_temp), _ptr(map_tangent), _ptr(contract),
+            _ptr(value), _ptr(map))
+    if scores.is_cuda:
+        total = -1 if offsets is None else scores.numel()
+        need = workspace_bytes(False, OP_TANGENT, B, L1, L2, total)
+        stream = torch.cuda.current_stream(scores.device).cuda_stream
+        ws = workspace if workspace is not None else _workspace(scores.device, need, stream)
+        _check(_lin_tan(*args, ws.data_ptr(), ws.numel(), stream), "ohm_sw_tangent")
+    else:
+        _check(_lin_tan_cpu(*args, _cpu_threads()), "ohm_sw_tangent_cpu")

