@@ -1,0 +1,530 @@
+"""Named primitives of the soft Smith-Waterman path (drop-in boundary).
+
+Registers the reference's dispatcher schemas under the same ``orihime::``
+namespace and names (src/sw_affine/registry.cpp:27-42,
+src/sw/registry.cpp:30-44) with CPU and CUDA (HIP) kernels that call the C
+ABI of libohm_sw.so through ``_lib``; fake (meta) kernels for torch.compile
+(orihime/_compile.py:25-134); autograd for the forward primitives
+(src/sw_affine/torch_hip.cpp:126-400); and batch-folding vmap rules keyed on
+the schema argument names (orihime/_ops.py:237-406).
+
+Module-level wrappers (``sw_affine_forward`` ...) have the call signatures
+orihime/_ops.py:434-449 exposes to the kernel adapters.
+
+Two primitives are additions with no reference counterpart:
+``*_value_only`` (value without the map, for value calls that need no
+gradient) and ``*_marginals_backward_pairs`` (the map VJP with per-pair,
+unsummed parameter contractions, which the vmap rule of
+``*_marginals_backward`` folds per mapped element).
+"""
+
+from __future__ import annotations
+
+from typing import Optional
+
+import torch
+from torch import Tensor
+
+from . import _lib
+
+_LIB = torch.library.Library("orihime", "FRAGMENT")
+
+# --------------------------------------------------------------------------
+# Validation at the native boundary (src/sw_affine/torch_hip.cpp:35-99,
+# src/sw_affine/torch_cpu.cpp:40-107)
+# --------------------------------------------------------------------------
+
+
+def _check_scores(scores: Tensor, short: bool = False) -> None:
+    if scores.dim() != 3:
+        raise RuntimeError("scores must be 3D" if short else "scores must be 3D (B, L1, L2)")
+    if scores.dtype != torch.float32:
+        raise RuntimeError("scores must be float32")
+    if not scores.is_contiguous():
+        raise RuntimeError("scores must be contiguous")
+
+
+def _check_vector(scores: Tensor, vec: Tensor, name: str) -> Tensor:
+    if vec.shape != scores.shape:
+        raise RuntimeError(f"scores and {name} must have same shape")
+    if vec.device != scores.device:
+        raise RuntimeError(f"{name} must be on same device as scores")
+    if vec.dtype != torch.float32:
+        raise RuntimeError(f"{name} must have dtype torch.float32")
+    if not vec.is_contiguous():
+        raise RuntimeError(f"{name} must be contiguous")
+    return vec
+
+
+def _check_lengths(lengths: Optional[Tensor], scores: Tensor) -> Optional[Tensor]:
+    if lengths is None:
+        return None
+    B, L1, L2 = scores.shape
+    if lengths.device != scores.device:
+        raise RuntimeError("lengths must be on same device as scores")
+    if not lengths.is_contiguous():
+        raise RuntimeError("lengths must be contiguous")
+    if lengths.dim() != 2 or lengths.shape[0] != B or lengths.shape[1] != 2:
+        raise RuntimeError("lengths must have shape [B, 2]")
+    if lengths.dtype != torch.int32:
+        raise RuntimeError("lengths must be int32")
+    if lengths.numel():
+        lo = lengths.amin(dim=0)
+        hi = lengths.amax(dim=0)
+        bounds = torch.stack([lo, hi]).cpu()
+        if bounds.min() < 0 or bounds[1, 0] > L1 or bounds[1, 1] > L2:
+            host = lengths.cpu()
+            for b in range(B):
+                l1, l2 = int(host[b, 0]), int(host[b, 1])
+                if not 0 <= l1 <= L1:
+                    raise RuntimeError(f"lengths[{b},0] must be between 0 and {L1}, got {l1}")
+                if not 0 <= l2 <= L2:
+                    raise RuntimeError(f"lengths[{b},1] must be between 0 and {L2}, got {l2}")
+    return lengths
+
+
+def _scalar(t: Tensor, name: str) -> float:
+    if t.numel() != 1:
+        raise RuntimeError(f"{name} must be a scalar tensor")
+    return float(t.detach().reshape(()).item())
+
+
+def _vec(scores: Tensor, n: int) -> Tensor:
+    return torch.empty(n, dtype=torch.float32, device=scores.device)
+
+
+# --------------------------------------------------------------------------
+# Affine gap implementations
+# --------------------------------------------------------------------------
+
+
+def _aff_forward(scores, go, ge, temp, lengths, want_map=True, want_grads=False):
+    _check_scores(scores)
+    ln = _check_lengths(lengths, scores)
+    B = scores.shape[0]
+    value = _vec(scores, B)
+    mp = torch.empty_like(scores) if want_map else None
+    g = [_vec(scores, B) for _ in range(3)] if want_grads else [None] * 3
+    _lib.affine_forward(scores, ln, None, go, ge, temp, value=value, map=mp,
+                        grad_open=g[0], grad_ext=g[1], grad_temp=g[2])
+    return value, mp, g
+
+
+def _aff_tangent(scores, go, ge, temp, lengths, direction=None, dgo=0.0, dge=0.0, dT=0.0,
+                 contract=False):
+    ln = _check_lengths(lengths, scores)
+    B = scores.shape[0]
+    mt = torch.empty_like(scores)
+    c = torch.empty(B, 3, dtype=torch.float32, device=scores.device) if contract else None
+    _lib.affine_tangent(scores, ln, None, go, ge, temp, direction=direction, d_gap_open=dgo,
+                        d_gap_ext=dge, d_temp=dT, map_tangent=mt, contract=c)
+    return mt, c
+
+
+def sw_affine_forward_impl(scores, gap_open: float, gap_ext: float, temp: float, lengths=None):
+    value, mp, _ = _aff_forward(scores, gap_open, gap_ext, temp, lengths)
+    return [value, mp]
+
+
+def sw_affine_forward_t_impl(scores, gap_open, gap_ext, temp, lengths):
+    go = _scalar(gap_open, "gap_open")
+    ge = _scalar(gap_ext, "gap_ext")
+    T = _scalar(temp, "temperature")
+    value, mp, _ = _aff_forward(scores, go, ge, T, lengths)
+    return [value, mp]
+
+
+def sw_affine_value_grad_params_impl(scores, gap_open: float, gap_ext: float, temp: float,
+                                     lengths=None):
+    _, _, g = _aff_forward(scores, gap_open, gap_ext, temp, lengths, want_map=False,
+                           want_grads=True)
+    return g[0], g[1], g[2]
+
+
+def sw_affine_value_only_impl(scores, gap_open: float, gap_ext: float, temp: float, lengths=None):
+    value, _, _ = _aff_forward(scores, gap_open, gap_ext, temp, lengths, want_map=False)
+    return value
+
+
+def sw_affine_marginals_hvp_impl(scores, v, gap_open: float, gap_ext: float, temp: float,
+                                 lengths=None):
+    _check_scores(scores, short=True)
+    _check_vector(scores, v, "tangent")
+    mt, _ = _aff_tangent(scores, gap_open, gap_ext, temp, lengths, direction=v)
+    return mt
+
+
+def _aff_sens(param: int):
+    def impl(scores, gap_open: float, gap_ext: float, temp: float, lengths=None):
+        _check_scores(scores, short=True)
+        mt, _ = _aff_tangent(scores, gap_open, gap_ext, temp, lengths,
+                             dgo=float(param == 0), dge=float(param == 1), dT=float(param == 2))
+        return mt
+
+    return impl
+
+
+def sw_affine_marginals_backward_pairs_impl(scores, grad_marginals, gap_open: float,
+                                            gap_ext: float, temp: float, lengths=None):
+    _check_scores(scores)
+    _check_vector(scores, grad_marginals, "grad_marginals")
+    return _aff_tangent(scores, gap_open, gap_ext, temp, lengths, direction=grad_marginals,
+                        contract=True)
+
+
+def sw_affine_marginals_backward_impl(scores, grad_marginals, gap_open: float, gap_ext: float,
+                                      temp: float, lengths=None):
+    gs, c = sw_affine_marginals_backward_pairs_impl(scores, grad_marginals, gap_open, gap_ext,
+                                                    temp, lengths)
+    tot = c.sum(dim=0)
+    return gs, tot[0:1].clone(), tot[1:2].clone(), tot[2:3].clone()
+
+
+# --------------------------------------------------------------------------
+# Linear gap implementations
+# --------------------------------------------------------------------------
+
+
+def _lin_forward(scores, gap, temp, lengths, want_map=True, want_grads=False):
+    _check_scores(scores)
+    ln = _check_lengths(lengths, scores)
+    B = scores.shape[0]
+    value = _vec(scores, B)
+    mp = torch.empty_like(scores) if want_map else None
+    g = [_vec(scores, B) for _ in range(2)] if want_grads else [None] * 2
+    _lib.linear_forward(scores, ln, None, gap, temp, value=value, map=mp, grad_gap=g[0],
+                        grad_temp=g[1])
+    return value, mp, g
+
+
+def _lin_tangent(scores, gap, temp, lengths, direction=None, dg=0.0, dT=0.0, contract=False):
+    ln = _check_lengths(lengths, scores)
+    B = scores.shape[0]
+    mt = torch.empty_like(scores)
+    c = torch.empty(B, 2, dtype=torch.float32, device=scores.device) if contract else None
+    _lib.linear_tangent(scores, ln, None, gap, temp, direction=direction, d_gap=dg, d_temp=dT,
+                        map_tangent=mt, contract=c)
+    return mt, c
+
+
+def sw_forward_impl(scores, gap: float, temp: float, lengths=None):
+    value, mp, _ = _lin_forward(scores, gap, temp, lengths)
+    return [value, mp]
+
+
+def sw_forward_t_impl(scores, gap, temp, lengths):
+    value, mp, _ = _lin_forward(scores, _scalar(gap, "gap"), _scalar(temp, "temperature"), lengths)
+    return [value, mp]
+
+
+def sw_value_grad_params_impl(scores, gap: float, temp: float, lengths=None):
+    _, _, g = _lin_forward(scores, gap, temp, lengths, want_map=False, want_grads=True)
+    return g[0], g[1]
+
+
+def sw_value_only_impl(scores, gap: float, temp: float, lengths=None):
+    value, _, _ = _lin_forward(scores, gap, temp, lengths, want_map=False)
+    return value
+
+
+def sw_marginals_hvp_impl(scores, v, gap: float, temp: float, lengths=None):
+    _check_scores(scores, short=True)
+    _check_vector(scores, v, "tangent")
+    mt, _ = _lin_tangent(scores, gap, temp, lengths, direction=v)
+    return mt
+
+
+def _lin_sens(param: int):
+    def impl(scores, gap: float, temp: float, lengths=None):
+        _check_scores(scores, short=True)
+        mt, _ = _lin_tangent(scores, gap, temp, lengths, dg=float(param == 0), dT=float(param == 1))
+        return mt
+
+    return impl
+
+
+def sw_marginals_backward_pairs_impl(scores, grad_marginals, gap: float, temp: float, lengths=None):
+    _check_scores(scores)
+    _check_vector(scores, grad_marginals, "grad_marginals")
+    return _lin_tangent(scores, gap, temp, lengths, direction=grad_marginals, contract=True)
+
+
+def sw_marginals_backward_impl(scores, grad_marginals, gap: float, temp: float, lengths=None):
+    gs, c = sw_marginals_backward_pairs_impl(scores, grad_marginals, gap, temp, lengths)
+    tot = c.sum(dim=0)
+    return gs, tot[0:1].clone(), tot[1:2].clone()
+
+
+# --------------------------------------------------------------------------
+# Legacy primitives (src/sw_affine/registry.cpp:27-32, src/sw/registry.cpp:30-35)
+# --------------------------------------------------------------------------
+
+
+def soft_sw_affine_with_grads_impl(scores, gap_open: float, gap_ext: float, temperature: float,
+                                   lengths=None):
+    value, mp, g = _aff_forward(scores, gap_open, gap_ext, temperature, lengths, want_grads=True)
+    return value, mp, g[0], g[1], g[2]
+
+
+def soft_sw_affine_param_jacobian_impl(scores, param_type: int, gap_open: float, gap_ext: float,
+                                       temperature: float, lengths=None):
+    if not 0 <= param_type <= 2:
+        raise RuntimeError("param_type must be 0, 1, or 2")
+    return _aff_sens(int(param_type))(scores, gap_open, gap_ext, temperature, lengths)
+
+
+def soft_sw_with_grads_impl(scores, gap: float, temperature: float, lengths=None):
+    value, mp, g = _lin_forward(scores, gap, temperature, lengths, want_grads=True)
+    return value, mp, g[0], g[1]
+
+
+def soft_sw_param_jacobian_impl(scores, param_type: int, gap: float, temperature: float,
+                                lengths=None):
+    if not 0 <= param_type <= 1:
+        raise RuntimeError("param_type must be 0 or 1")
+    return _lin_sens(int(param_type))(scores, gap, temperature, lengths)
+
+
+# --------------------------------------------------------------------------
+# Registration
+# --------------------------------------------------------------------------
+
+_SCHEMAS = {
+    # affine — namespaced API
+    "sw_affine_forward": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor[]", sw_affine_forward_impl),
+    "sw_affine_forward_t": ("(Tensor scores, Tensor gap_open, Tensor gap_ext, Tensor temp, Tensor lengths) -> Tensor[]", sw_affine_forward_t_impl),
+    "sw_affine_value_grad_params": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> (Tensor, Tensor, Tensor)", sw_affine_value_grad_params_impl),
+    "sw_affine_marginals_backward": ("(Tensor scores, Tensor grad_marginals, float gap_open, float gap_ext, float temp, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor)", sw_affine_marginals_backward_impl),
+    "sw_affine_marginals_hvp": ("(Tensor scores, Tensor v, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", sw_affine_marginals_hvp_impl),
+    "sw_affine_marginals_grad_gap_open": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", _aff_sens(0)),
+    "sw_affine_marginals_grad_gap_ext": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", _aff_sens(1)),
+    "sw_affine_marginals_grad_temp": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", _aff_sens(2)),
+    "sw_affine_value_only": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", sw_affine_value_only_impl),
+    "sw_affine_marginals_backward_pairs": ("(Tensor scores, Tensor grad_marginals, float gap_open, float gap_ext, float temp, Tensor? lengths) -> (Tensor, Tensor)", sw_affine_marginals_backward_pairs_impl),
+    # affine — legacy
+    "soft_sw_affine": ("(Tensor scores, Tensor gap_open, Tensor gap_ext, Tensor temperature, Tensor lengths) -> Tensor[]", sw_affine_forward_t_impl),
+    "soft_sw_affine_float": ("(Tensor scores, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> Tensor[]", sw_affine_forward_impl),
+    "soft_sw_affine_with_grads": ("(Tensor scores, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor, Tensor)", soft_sw_affine_with_grads_impl),
+    "soft_sw_affine_hvp": ("(Tensor scores, Tensor tangent, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> Tensor", sw_affine_marginals_hvp_impl),
+    "soft_sw_affine_param_jacobian": ("(Tensor scores, int param_type, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> Tensor", soft_sw_affine_param_jacobian_impl),
+    "soft_sw_affine_backward_full": ("(Tensor scores, Tensor grad_alignment, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor)", sw_affine_marginals_backward_impl),
+    # linear — namespaced API
+    "sw_forward": ("(Tensor scores, float gap, float temp, Tensor? lengths) -> Tensor[]", sw_forward_impl),
+    "sw_forward_t": ("(Tensor scores, Tensor gap, Tensor temp, Tensor lengths) -> Tensor[]", sw_forward_t_impl),
+    "sw_value_grad_params": ("(Tensor scores, float gap, float temp, Tensor? lengths) -> (Tensor, Tensor)", sw_value_grad_params_impl),
+    "sw_marginals_backward": ("(Tensor scores, Tensor grad_marginals, float gap, float temp, Tensor? lengths) -> (Tensor, Tensor, Tensor)", sw_marginals_backward_impl),
+    "sw_marginals_hvp": ("(Tensor scores, Tensor v, float gap, float temp, Tensor? lengths) -> Tensor", sw_marginals_hvp_impl),
+    "sw_marginals_grad_gap": ("(Tensor scores, float gap, float temp, Tensor? lengths) -> Tensor", _lin_sens(0)),
+    "sw_marginals_grad_temp": ("(Tensor scores, float gap, float temp, Tensor? lengths) -> Tensor", _lin_sens(1)),
+    "sw_value_only": ("(Tensor scores, float gap, float temp, Tensor? lengths) -> Tensor", sw_value_only_impl),
+    "sw_marginals_backward_pairs": ("(Tensor scores, Tensor grad_marginals, float gap, float temp, Tensor? lengths) -> (Tensor, Tensor)", sw_marginals_backward_pairs_impl),
+    # linear — legacy
+    "soft_sw": ("(Tensor scores, Tensor gap, Tensor temperature, Tensor lengths) -> Tensor[]", sw_forward_t_impl),
+    "soft_sw_float": ("(Tensor scores, float gap, float temperature, Tensor? lengths) -> Tensor[]", sw_forward_impl),
+    "soft_sw_with_grads": ("(Tensor scores, float gap, float temperature, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor)", soft_sw_with_grads_impl),
+    "soft_sw_hvp": ("(Tensor scores, Tensor tangent, float gap, float temperature, Tensor? lengths) -> Tensor", sw_marginals_hvp_impl),
+    "soft_sw_param_jacobian": ("(Tensor scores, int param_type, float gap, float temperature, Tensor? lengths) -> Tensor", soft_sw_param_jacobian_impl),
+    "soft_sw_backward_full": ("(Tensor scores, Tensor grad_alignment, float gap, float temperature, Tensor? lengths) -> (Tensor, Tensor, Tensor)", sw_marginals_backward_impl),
+}
+
+
+def _fake_for(name: str, schema: str):
+    """Shape-only kernel: outputs follow the schema's return types."""
+    ret = schema.split("->")[1].strip()
+
+    def like(scores):
+        return torch.empty_like(scores)
+
+    def vec(scores):
+        return scores.new_empty((scores.shape[0],))
+
+    def one(scores):
+        return scores.new_empty((1,))
+
+    def fake(scores, *args):
+        if ret == "Tensor[]":
+            return [vec(scores), like(scores)]
+        if name.endswith("value_grad_params"):
+            n = 3 if "affine" in name else 2
+            return tuple(vec(scores) for _ in range(n))
+        if name.endswith("_with_grads"):
+            n = 3 if "affine" in name else 2
+            return (vec(scores), like(scores), *(vec(scores) for _ in range(n)))
+        if name.endswith("marginals_backward_pairs"):
+            n = 3 if "affine" in name else 2
+            return like(scores), scores.new_empty((scores.shape[0], n))
+        if name.endswith(("marginals_backward", "backward_full")):
+            n = 3 if "affine" in name else 2
+            return (like(scores), *(one(scores) for _ in range(n)))
+        if name.endswith("value_only"):
+            return vec(scores)
+        return like(scores)
+
+    return fake
+
+
+for _name, (_schema, _impl) in _SCHEMAS.items():
+    _LIB.define(_name + _schema)
+    _LIB.impl(_name, _impl, "CPU")
+    _LIB.impl(_name, _impl, "CUDA")
+    torch.library.register_fake(f"orihime::{_name}", _fake_for(_name, _schema), lib=_LIB)
+
+orihime = torch.ops.orihime
+
+# --------------------------------------------------------------------------
+# Autograd of the forward primitives: d(value) = map, d(map) = the map VJP,
+# and for tensor parameters the contracted parameter derivatives
+# (src/sw_affine/torch_hip.cpp:210-400).
+# --------------------------------------------------------------------------
+
+
+def _forward_setup(ctx, inputs, output):
+    scores = inputs[0]
+    ctx.save_for_backward(scores, output[1])
+    ctx.inputs = inputs
+
+
+def _make_forward_backward(affine: bool, tensor_params: bool):
+    nparam = 3 if affine else 2
+
+    def backward(ctx, grads):
+        scores, mp = ctx.saved_tensors
+        inputs = ctx.inputs
+        params = inputs[1:1 + nparam]
+        lengths = inputs[1 + nparam]
+        fparams = tuple(_scalar(p, "param") if isinstance(p, Tensor) else float(p) for p in params)
+        gv = grads[0] if grads is not None else None
+        gm = grads[1] if grads is not None and len(grads) > 1 else None
+        g_scores = torch.zeros_like(scores)
+        g_params = [torch.zeros(1, dtype=scores.dtype, device=scores.device) for _ in range(nparam)]
+        if gv is not None:
+            g_scores = g_scores + gv.reshape(-1, 1, 1) * mp
+            if tensor_params:
+                pg = (orihime.sw_affine_value_grad_params if affine else orihime.sw_value_grad_params)(
+                    scores.detach(), *fparams, lengths)
+                for k in range(nparam):
+                    g_params[k] = g_params[k] + (gv * pg[k]).sum().reshape(1)
+        if gm is not None:
+            mb = (orihime.sw_affine_marginals_backward if affine else orihime.sw_marginals_backward)(
+                scores.detach(), gm.contiguous().to(torch.float32), *fparams, lengths)
+            g_scores = g_scores + mb[0]
+            for k in range(nparam):
+                g_params[k] = g_params[k] + mb[1 + k]
+        if tensor_params:
+            outs = [g.reshape(p.shape).to(p.dtype) for g, p in zip(g_params, params)]
+        else:
+            outs = [None] * nparam
+        return (g_scores, *outs, None)
+
+    return backward
+
+
+for _name, _aff, _tp in (
+    ("sw_affine_forward", True, False), ("sw_affine_forward_t", True, True),
+    ("soft_sw_affine_float", True, False), ("soft_sw_affine", True, True),
+    ("sw_forward", False, False), ("sw_forward_t", False, True),
+    ("soft_sw_float", False, False), ("soft_sw", False, True),
+):
+    torch.library.register_autograd(
+        f"orihime::{_name}", _make_forward_backward(_aff, _tp), setup_context=_forward_setup,
+        lib=_LIB,
+    )
+
+# --------------------------------------------------------------------------
+# vmap: fold the mapped dimension into the native batch (orihime/_ops.py:237-406)
+# --------------------------------------------------------------------------
+
+_VMAP_BATCH_ARGUMENTS = frozenset({"scores", "grad_marginals", "v", "lengths"})
+
+
+def _make_batch_folding_vmap_rule(op_name: str, pairs_op: str | None = None):
+    raw = getattr(orihime, op_name).default
+    names = tuple(a.name for a in raw._schema.arguments)
+    batch_idx = tuple(i for i, n in enumerate(names) if n in _VMAP_BATCH_ARGUMENTS)
+    pairs_raw = getattr(orihime, pairs_op).default if pairs_op else None
+
+    def rule(info, in_dims, *args):
+        for i, d in enumerate(in_dims):
+            if d is not None and i not in batch_idx:
+                raise RuntimeError(
+                    f"orihime::{op_name} does not support mapping scalar argument "
+                    f"{names[i]!r}; keep scalar parameters unbatched"
+                )
+        native = None
+        folded = list(args)
+        for i in batch_idx:
+            a = args[i]
+            if not isinstance(a, Tensor):
+                continue
+            d = in_dims[i]
+            x = a.unsqueeze(0).expand(info.batch_size, *a.shape) if d is None else a.movedim(d, 0)
+            if x.ndim < 2:
+                raise RuntimeError(f"orihime::{op_name} batch argument {names[i]!r} has no native batch dimension")
+            if native is None:
+                native = x.shape[1]
+            elif x.shape[1] != native:
+                raise RuntimeError(
+                    f"orihime::{op_name} received inconsistent native batch sizes {native} and {x.shape[1]}"
+                )
+            folded[i] = x.reshape(info.batch_size * native, *x.shape[2:]).contiguous()
+        if native is None:
+            raise RuntimeError(f"orihime::{op_name} requires a mapped batch-aligned tensor")
+
+        def restore(t):
+            return t.reshape(info.batch_size, native, *t.shape[1:])
+
+        if pairs_raw is not None:
+            gs, contract = pairs_raw(*folded)
+            sums = restore(contract).sum(dim=1)  # [V, nparams]
+            outs = (restore(gs), *(sums[:, k:k + 1] for k in range(sums.shape[1])))
+            return outs, tuple(0 for _ in outs)
+        result = raw(*folded)
+        if isinstance(result, (list, tuple)):
+            outs = [restore(t) for t in result]
+            dims = [0] * len(outs)
+            return (outs, dims) if isinstance(result, list) else (tuple(outs), tuple(dims))
+        return restore(result), 0
+
+    return rule
+
+
+for _family, _fields in (("sw", ("gap", "temp")), ("sw_affine", ("gap_open", "gap_ext", "temp"))):
+    for _op in (f"{_family}_forward", f"{_family}_forward_t", f"{_family}_value_grad_params",
+                f"{_family}_marginals_hvp", f"{_family}_value_only",
+                *(f"{_family}_marginals_grad_{f}" for f in _fields)):
+        torch.library.register_vmap(f"orihime::{_op}", _make_batch_folding_vmap_rule(_op), lib=_LIB)
+    torch.library.register_vmap(
+        f"orihime::{_family}_marginals_backward",
+        _make_batch_folding_vmap_rule(f"{_family}_marginals_backward", f"{_family}_marginals_backward_pairs"),
+        lib=_LIB,
+    )
+
+# --------------------------------------------------------------------------
+# Wrappers used by the kernel adapters (orihime/_ops.py:434-449)
+# --------------------------------------------------------------------------
+
+sw_forward = orihime.sw_forward
+sw_forward_t = orihime.sw_forward_t
+sw_value_grad_params = orihime.sw_value_grad_params
+sw_marginals_backward = orihime.sw_marginals_backward
+sw_marginals_hvp = orihime.sw_marginals_hvp
+sw_marginals_grad_gap = orihime.sw_marginals_grad_gap
+sw_marginals_grad_temp = orihime.sw_marginals_grad_temp
+sw_value_only = orihime.sw_value_only
+
+sw_affine_forward = orihime.sw_affine_forward
+sw_affine_forward_t = orihime.sw_affine_forward_t
+sw_affine_value_grad_params = orihime.sw_affine_value_grad_params
+sw_affine_marginals_backward = orihime.sw_affine_marginals_backward
+sw_affine_marginals_hvp = orihime.sw_affine_marginals_hvp
+sw_affine_marginals_grad_gap_open = orihime.sw_affine_marginals_grad_gap_open
+sw_affine_marginals_grad_gap_ext = orihime.sw_affine_marginals_grad_gap_ext
+sw_affine_marginals_grad_temp = orihime.sw_affine_marginals_grad_temp
+sw_affine_value_only = orihime.sw_affine_value_only
+
+
+def _has_cuda_dispatch() -> bool:
+    try:
+        return torch._C._dispatch_has_kernel_for_dispatch_key("orihime::sw_forward", "CUDA")
+    except (AttributeError, RuntimeError):
+        return False

@@ -43,14 +43,16 @@ inline void lse2(float a, float b, float& y, float& wa, float& wb, float& h) {
     wb = a_hi ? wl : r;
     h = l1 + wl * std::min(hi - lo, 512.0f);
 }
+// y = log2(2^a + 2^b + 2^c) with weights normalised to sum to 1 (they are
+// divided by their own sum, not taken as 2^(x - y) with a rounded y, so no
+// flow is lost through the node) and entropy term h = y - sum w x.
 inline void lse3(float a, float b, float c, float& y, float& wa, float& wb, float& wc, float& h) {
     float hi = std::max(std::max(a, b), c);
-    float lo = std::min(std::min(a, b), c);
-    float md = std::max(std::min(a, b), std::min(std::max(a, b), c));
-    y = hi + lg2(1.0f + ex2(md - hi) + ex2(lo - hi));
-    float la = clampw(a - y), lb = clampw(b - y), lc = clampw(c - y);
-    wa = ex2(la); wb = ex2(lb); wc = ex2(lc);
-    h = -(wa * la + wb * lb + wc * lc);
+    float ea = ex2(a - hi), eb = ex2(b - hi), ec = ex2(c - hi);
+    float sum = ea + eb + ec, r = 1.0f / sum, l = lg2(sum);
+    y = hi + l;
+    wa = ea * r; wb = eb * r; wc = ec * r;
+    h = l + wa * std::min(hi - a, 512.0f) + wb * std::min(hi - b, 512.0f) + wc * std::min(hi - c, 512.0f);
 }
 inline void softplus(float x, float& y, float& w, float& h) {
     float e = ex2(-std::fabs(x)), r = 1.0f / (1.0f + e), l1 = lg2(1.0f + e), ws = e * r;
@@ -65,6 +67,8 @@ inline void pair_w(float la, float lb, float& wa, float& wb) {
     wb = a_lo ? wh : wl;
 }
 inline float sigmoid(float x) { return 1.0f / (1.0f + ex2(-x)); }
+// 1 - sigmoid(x), without cancellation
+inline float sigmoid_c(float x) { return 1.0f / (1.0f + ex2(x)); }
 
 struct Problem {
     const float* scores;
@@ -228,7 +232,7 @@ void affine_pair(const Problem& P, int b, bool want_bwd) {
             if (TAN) {
                 const float dAc = dA[c];
                 const float dpi = pi * ((dAc - dSf) - dTl * lt) * iT;
-                const float dsg = sg * (1.0f - sg) * (dAc - dTl * std::max(Ac, -256.0f)) * iT;
+                const float dsg = sg * sigmoid_c(Ac) * (dAc - dTl * std::max(Ac, -256.0f)) * iT;
                 const float dbMs = has_m ? dbM[at(i + 1, j + 1)] : 0.f;
                 const float dbA = dpi + dbMs * sg + bMs * dsg;
                 const float dwM = wM * ((dM[c] - dAc) - dTl * clampw(M[c] - Ac)) * iT;
@@ -314,57 +318,49 @@ void linear_pair(const Problem& P, int b, bool want_bwd) {
     if (P.value) P.value[b] = (float)(St / P.kap);
     if (!want_bwd) return;
 
+    // Adjoint in edge-flow form: node c splits its adjoint over its three
+    // in-edges with weights normalised at c; the flows fu (edge from
+    // (i-1, j)) and fl (edge from (i, j-1)) are pulled by the predecessors.
     const float Stf = (float)St, dSf = (float)dS, iT = 1.0f / P.T;
-    std::vector<float> bt(n, 0.f), gm(n, 0.f), dbt, dgm;
-    if (TAN) { dbt.assign(n, 0.f); dgm.assign(n, 0.f); }
+    std::vector<float> gm(n, 0.f), fu(n, 0.f), fl(n, 0.f), dgm, dfu, dfl;
+    if (TAN) { dgm.assign(n, 0.f); dfu.assign(n, 0.f); dfl.assign(n, 0.f); }
     double acc0 = 0, acc1 = 0, tac0 = 0, tac1 = 0;
     for (int i = L1; i >= 1; --i)
         for (int j = L2; j >= 1; --j) {
-            const size_t c = at(i, j);
+            const size_t c = at(i, j), up = at(i - 1, j), lf = at(i, j - 1);
             const bool has_i = i < L1, has_d = j < L2, has_m = has_i && has_d;
             const float Ac = A[c];
-            float tu = 0, tl = 0, wu = 0, wl = 0, lu = 0, ll = 0;
-            if (has_i) {
-                const size_t q = at(i + 1, j);
-                lu = clampw(Ac + P.go2 - A[q]);
-                wu = ex2(lu);
-                tu = bt[q] * wu;
-            }
-            if (has_d) {
-                const size_t q = at(i, j + 1);
-                ll = clampw(Ac + P.go2 - A[q]);
-                wl = ex2(ll);
-                tl = bt[q] * wl;
-            }
+            const float cu = i > 1 ? A[up] + P.go2 : kNinf;
+            const float cl = j > 1 ? A[lf] + P.go2 : kNinf;
+            float y, wx, wu, wl, h;
+            lse3(X[c], cu, cl, y, wx, wu, wl, h);
             const float lt = clampw(Ac - Stf), pi = ex2(lt), sg = sigmoid(Ac);
             const float gD = has_m ? gm[at(i + 1, j + 1)] : 0.f;
-            bt[c] = pi + gD * sg + tu + tl;
-            const float lx = clampw(X[c] - Ac), wx = ex2(lx);
-            gm[c] = bt[c] * wx;
+            const float inU = has_i ? fu[at(i + 1, j)] : 0.f;
+            const float inL = has_d ? fl[at(i, j + 1)] : 0.f;
+            const float bt = pi + gD * sg + inU + inL;
+            gm[c] = bt * wx;
+            fu[c] = bt * wu;
+            fl[c] = bt * wl;
             const float sc = s[(int64_t)(i - 1) * r.ld + (j - 1)];
-            acc0 += (double)tu + tl;
+            acc0 += (double)fu[c] + fl[c];
             acc1 += (double)gm[c] * sc;
             if (P.map) P.map[r.off + (int64_t)(i - 1) * r.ld + (j - 1)] = gm[c];
             if (TAN) {
                 const float dAc = dA[c];
-                float dtu = 0, dtl = 0;
-                if (has_i) {
-                    const size_t q = at(i + 1, j);
-                    const float dwu = wu * ((dAc + P.dgo - dA[q]) - dTl * lu) * iT;
-                    dtu = dbt[q] * wu + bt[q] * dwu;
-                }
-                if (has_d) {
-                    const size_t q = at(i, j + 1);
-                    const float dwl = wl * ((dAc + P.dgo - dA[q]) - dTl * ll) * iT;
-                    dtl = dbt[q] * wl + bt[q] * dwl;
-                }
                 const float dpi = pi * ((dAc - dSf) - dTl * lt) * iT;
-                const float dsg = sg * (1.0f - sg) * (dAc - dTl * std::max(Ac, -256.0f)) * iT;
+                const float dsg = sg * sigmoid_c(Ac) * (dAc - dTl * std::max(Ac, -256.0f)) * iT;
                 const float dgD = has_m ? dgm[at(i + 1, j + 1)] : 0.f;
-                dbt[c] = dpi + dgD * sg + gD * dsg + dtu + dtl;
-                const float dwx = wx * ((dX[c] - dAc) - dTl * lx) * iT;
-                dgm[c] = dbt[c] * wx + bt[c] * dwx;
-                tac0 += (double)dtu + dtl;
+                const float dinU = has_i ? dfu[at(i + 1, j)] : 0.f;
+                const float dinL = has_d ? dfl[at(i, j + 1)] : 0.f;
+                const float dbt = dpi + dgD * sg + gD * dsg + dinU + dinL;
+                const float dwx = wx * ((dX[c] - dAc) - dTl * clampw(X[c] - Ac)) * iT;
+                const float dwu = i > 1 ? wu * ((dA[up] + P.dgo - dAc) - dTl * clampw(cu - Ac)) * iT : 0.f;
+                const float dwl = j > 1 ? wl * ((dA[lf] + P.dgo - dAc) - dTl * clampw(cl - Ac)) * iT : 0.f;
+                dgm[c] = dbt * wx + bt * dwx;
+                dfu[c] = dbt * wu + bt * dwu;
+                dfl[c] = dbt * wl + bt * dwl;
+                tac0 += (double)dfu[c] + dfl[c];
                 tac1 += (double)dgm[c] * sc;
                 if (P.map_t) P.map_t[r.off + (int64_t)(i - 1) * r.ld + (j - 1)] = dgm[c];
             }

@@ -102,6 +102,13 @@ DEV float lse3(float a, float b, float c) {
 // log2(1 + 2^x) and its derivative 2^x / (1 + 2^x)
 DEV float softplus2(float x) { return fmaxf(x, 0.0f) + lg2(1.0f + ex2(-fabsf(x))); }
 DEV float sigmoid2(float x) { return rcpf(1.0f + ex2(-x)); }
+// sigmoid2(x) and 1 - sigmoid2(x), both without cancellation.
+DEV void sigmoid2_pair(float x, float& w, float& wc) {
+    float e = ex2(-fabsf(x));
+    float r = rcpf(1.0f + e);
+    w = x >= 0.0f ? r : e * r;
+    wc = x >= 0.0f ? e * r : r;
+}
 DEV void softplus_sigmoid2(float x, float& sp, float& sg) {
     float e = ex2(-fabsf(x));
     float r = rcpf(1.0f + e);
@@ -119,22 +126,15 @@ DEV void pair_weights(float la, float lb, float& wa, float& wb) {
     wa = a_lo ? wl : wh;
     wb = a_lo ? wh : wl;
 }
-// Two weights with 2^la + 2^lb = 1 exactly in real arithmetic, plus their sum
-// derivative form used by the tangent passes: returns them unclamped below 1.
-DEV void lse2_weights(float a, float b, float& y, float& wa, float& wb) {
-    float hi = fmaxf(a, b), lo = fminf(a, b);
-    float e = ex2(lo - hi);
-    float r = rcpf(1.0f + e);
-    y = hi + lg2(1.0f + e);
-    bool a_hi = a >= b;
-    wa = a_hi ? r : e * r;
-    wb = a_hi ? e * r : r;
-}
-DEV void lse3_weights(float a, float b, float c, float& y, float& wa, float& wb, float& wc) {
-    y = lse3(a, b, c);
-    wa = ex2(a - y);
-    wb = ex2(b - y);
-    wc = ex2(c - y);
+// y = log2(2^a + 2^b + 2^c) and the three weights, normalised by their own
+// sum.  Taking 2^(x - y) with a rounded y instead would lose a relative
+// ~ulp(y) of adjoint flow at every node, which compounds along a path.
+DEV void lse3_norm(float a, float b, float c, float& y, float& wa, float& wb, float& wc) {
+    const float hi = fmaxf(fmaxf(a, b), c);
+    const float ea = ex2(a - hi), eb = ex2(b - hi), ec = ex2(c - hi);
+    const float sum = ea + eb + ec, r = rcpf(sum);
+    y = hi + lg2(sum);
+    wa = ea * r; wb = eb * r; wc = ec * r;
 }
 
 // 16 consecutive floats row[c0 .. c0+15]; 0 outside [0, len) or when !ok.
@@ -516,8 +516,8 @@ DEV void affine_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
             const float bIs = from_next_lane(bIp, lane_value(eB, u));
             const float Is = from_next_lane(Ip, lane_value(eI, u));
             const float bMs = from_next_lane(bMp, lane_value(eM, u));
-            const float A = lse3(M, I, D);
-            const float wM = ex2(M - A), wI = ex2(I - A), wD = ex2(D - A);
+            float A, wM, wI, wD;
+            lse3_norm(M, I, D, A, wM, wI, wD);
             const float bA = ex2(fminf(A - St, 0.0f)) + bMdg * sigmoid2(A);
             float wIM, wII, wDM, wDD;
             pair_weights(M + go2 - Is, I + ge2 - Is, wIM, wII);
@@ -678,9 +678,14 @@ __global__ __launch_bounds__(W * 64) void sw_forward_kernel(SwArgs a) {
 }
 
 // ==========================================================================
-// Linear gap: backward.  beta(c) = 2^(A-S) + gamma(i+1,j+1) sig(A)
-//   + beta(i+1,j) w_up(i+1,j) + beta(i,j+1) w_left(i,j+1),
-// gamma(c) = beta(c) 2^(X-A) = map(c).
+// Linear gap: backward, edge-flow form.  Node c = (i, j) splits its adjoint
+// beta(c) = 2^(A-S) + gamma(i+1,j+1) sig(A) + fu(i+1,j) + fl(i,j+1)
+// over its in-edges with weights normalised at c:
+//   gamma(c) = beta wx (to X, = map), fu(c) = beta wu (from (i-1, j)),
+//   fl(c) = beta wl (from (i, j-1)).
+// A(i, j-1) is the table entry one step earlier in the same lane; A(i-1, j)
+// is that entry of lane r-1 (DPP), or the strip above's forward boundary.
+// Boundary entries: forward A at [0]; backward fu, gamma at [4], [5].
 // ==========================================================================
 
 DEV void linear_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, int* prog, float St,
@@ -692,26 +697,33 @@ DEV void linear_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
     const float* srow = a.scores + d.s_off + (int64_t)(i - 1) * d.ld;
     float* mrow = a.map ? a.map + d.s_off + (int64_t)(i - 1) * d.ld : nullptr;
     const float* tab = a.table + d.t_off + (int64_t)s * NT * (kWave * kPlanesLinearFwd);
+    const float* bup = s > 0 ? a.bnd + d.b_off + (int64_t)(s - 1) * L2 * kBndStride : nullptr;
     const float* bin = s + 1 < d.ns ? a.bnd + d.b_off + (int64_t)(s + 1) * L2 * kBndStride : nullptr;
     float* bout = s > 0 ? a.bnd + d.b_off + (int64_t)s * L2 * kBndStride : nullptr;
     const float g2 = a.go2;
+    constexpr int P = kWave * kPlanesLinearFwd;
 
-    float bR = 0.0f, AR = kNinf;                     // beta, A of (i, j+1)
-    float bP = 0.0f, AP = kNinf, gP = 0.0f;          // own values of the previous step
-    float gDg = 0.0f;                                // gamma of (i+1, j+1)
+    float flP = 0.0f;              // fl of (i, j+1)
+    float fuP = 0.0f, gP = 0.0f;   // own fu, gamma of the previous step
+    float gDg = 0.0f;              // gamma of (i+1, j+1)
+    float Acur = tab[(int64_t)(NT - 1) * P + r];
     float cur[16], nxt[16];
     load16(cur, srow, NT - kChunk - r, L2, row_ok);
     const int nchunk = (NT + kChunk - 1) / kChunk;
     for (int k = 0; k < nchunk; ++k) {
         load16(nxt, srow, NT - kChunk * (k + 2) - r, L2, row_ok);
-        float eB = 0.0f, eA = kNinf, eG = 0.0f;
+        float eF = 0.0f, eG = 0.0f, eA = kNinf;
+        if (bup) {
+            const int idx = L2 + 62 - (kChunk * k + lane);  // lane 0's column - 1 at step u = lane
+            if (lane < kChunk && idx >= 0 && idx < L2) eA = bup[(int64_t)idx * kBndStride];
+        }
         if (bin) {
             const int need = chunk_need(k, L2);
             if (need > 0) wait_ready(&prog[s + 1], need);
             const int j = L2 - (kChunk * k + lane);
             if (lane < kChunk && j >= 1) {
                 const float* e = bin + (int64_t)(j - 1) * kBndStride;
-                eB = e[0]; eA = e[1]; eG = e[2];
+                eF = e[4]; eG = e[5];
             }
         }
         float pc[16];
@@ -721,26 +733,24 @@ DEV void linear_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
             const int t = NT - 1 - (kChunk * k + u);
             const int j = t - r + 1;
             const bool ok = row_ok && j >= 1 && j <= L2;
-            const int tt = max(t, 0);
-            const float* q = tab + (int64_t)tt * (kWave * kPlanesLinearFwd) + r;
-            const float A = q[0], X = q[kWave];
-            const float bD = from_next_lane(bP, lane_value(eB, u));   // beta(i+1, j)
-            const float AD = from_next_lane(AP, lane_value(eA, u));   // A(i+1, j)
-            const float gD = from_next_lane(gP, lane_value(eG, u));   // gamma(i+1, j)
-            const float tu = bD * ex2(fminf(A + g2 - AD, 0.0f));
-            const float tl = bR * ex2(fminf(A + g2 - AR, 0.0f));
-            float b = ex2(fminf(A - St, 0.0f)) + gDg * sigmoid2(A) + tu + tl;
-            float g = b * ex2(fminf(X - A, 0.0f));
-            float tg = tu + tl;
-            if (!ok) { b = 0.0f; g = 0.0f; tg = 0.0f; }
-            pc[u] = g;
-            cacc[0] += tg;
-            cacc[1] += g * cur[kChunk - 1 - u];
+            const float Aleft = t >= 1 ? tab[(int64_t)(t - 1) * P + r] : kNinf;
+            const float X = tab[(int64_t)max(t, 0) * P + kWave + r];
+            const float Aup = from_prev_lane(Aleft, lane_value(eA, u));
+            const float inU = from_next_lane(fuP, lane_value(eF, u));
+            const float gD = from_next_lane(gP, lane_value(eG, u));
+            float y, wx, wu, wl;
+            lse3_norm(X, Aup + g2, Aleft + g2, y, wx, wu, wl);
+            const float bt = ex2(fminf(Acur - St, 0.0f)) + gDg * sigmoid2(Acur) + inU + flP;
+            float gm = bt * wx, fu = bt * wu, fl = bt * wl;
+            if (!ok) { gm = 0.0f; fu = 0.0f; fl = 0.0f; }
+            pc[u] = gm;
+            cacc[0] += fu + fl;
+            cacc[1] += gm * cur[kChunk - 1 - u];
             if (bout && r == 0 && ok) {
                 float* e = bout + (int64_t)(j - 1) * kBndStride;
-                e[0] = b; e[1] = A; e[2] = g;
+                e[4] = fu; e[5] = gm;
             }
-            gDg = gD; bR = b; AR = ok ? A : kNinf; bP = b; AP = ok ? A : kNinf; gP = g;
+            gDg = gD; flP = fl; fuP = fu; gP = gm; Acur = Aleft;
         }
 #pragma unroll
         for (int c = 0; c < 2; ++c) kadd(acc[c], comp[c], cacc[c]);

@@ -1,0 +1,394 @@
+"""CPU tests: the C ABI library, the host implementation behind it, and the
+drop-in Python surface (engine, named primitives, ops, nn, raw, ragged).
+
+Numeric expectations come from the golden fixtures (reference CPU kernels)
+and from the FP64 oracle; tolerances follow the reference suite
+(tests/test_soft_sw_affine.py:29-30, :178-195): values 1e-5 relative, maps
+1e-4 absolute.
+"""
+
+import ctypes
+import os
+import re
+import warnings
+
+import numpy as np
+import pytest
+import torch
+
+import orihime_amd as ohm
+from orihime_amd import _lib
+from helpers import abs_err, affine_params, close_or_nearer_truth, golden_files, load, rel_err
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "ohm_sw.h")).read()
+    declared = set(re.findall(r"^\S.*?\b(ohm_\w+)\(", header, re.M))
+    assert len(declared) >= 12
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(_lib.EXPORTS) == declared
+    assert ohm.__version__ == _lib.version()
+
+
+def test_gpu_entry_points_reject_bad_arguments_without_a_device():
+    # argument checks run before any HIP call
+    lib = _lib._lib
+    value = np.zeros(1, np.float32)  # never written: every call below fails its checks
+    vp = value.ctypes.data
+    st = lib.ohm_sw_affine_forward(None, None, None, 1, 4, 4, -1.0, -1.0, 0.0, vp, None, None,
+                                   None, None, None, 0, None)
+    assert st == 1 and b"temperature" in lib.ohm_last_error()
+    st = lib.ohm_sw_affine_forward(None, None, None, 1, 4, 4, -1.0, -1.0, 1.0, vp, None, None,
+                                   None, None, None, 0, None)
+    assert st == 1 and b"scores" in lib.ohm_last_error()
+    st = lib.ohm_sw_affine_tangent(None, None, None, 1, 4, 4, -1.0, -1.0, 1.0, None, 1.0, 0.0,
+                                   0.0, None, vp, None, None, None, 0, None)
+    assert st == 1 and b"pure score direction" in lib.ohm_last_error()
+    assert lib.ohm_sw_affine_workspace(1, 2, 512, 512, -1) > 2 * 512 * 512 * 12
+
+
+# --------------------------------------------------------------------------
+# Host implementation vs golden fixtures and the FP64 oracle
+# --------------------------------------------------------------------------
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a))
+
+
+def entropy_tol(value, temp):
+    """grad_T = (S - E[score]) / T subtracts two numbers of size |S|; FP32
+    rounding of either leaves ~|S| * 2^-23 / T of noise, which dominates at
+    low temperature (the reference's own FP32 result carries it too)."""
+    return 1e-4 + 8.0 * (np.abs(np.asarray(value, np.float64)) + 1.0) * 2.0 ** -23 / temp
+
+
+@pytest.mark.parametrize("path", golden_files("affine"), ids=lambda p: p.split("/")[-1])
+def test_affine_cpu_path_matches_golden(path):
+    d = load(path)
+    go, ge, T = affine_params(d)
+    s, ln = _t(d["scores"]), _t(d["lengths"])
+    v, m, a, b, c = torch.ops.orihime.soft_sw_affine_with_grads(s, go, ge, T, ln)
+    assert rel_err(v, d["value"], 1e-3) < 1e-5
+    assert abs_err(m, d["map"]) < 1e-4
+    for got, ref in ((a, "grad_open"), (b, "grad_ext")):
+        assert rel_err(got, d[ref], 1.0) < 1e-4, ref
+    assert np.all(np.abs(c.numpy() - d["grad_temp"]) <= entropy_tol(d["value"], T) * np.maximum(1, np.abs(d["grad_temp"])))
+    if "hvp" in d:
+        V = _t(d["V"])
+        s64 = d["scores"].astype(np.float64)
+        lnn = d["lengths"]
+        # second-order outputs scale like 1/T: tolerance relative to their size
+        tol = lambda r: 1e-4 * max(1.0, np.abs(r).max())
+        got = torch.ops.orihime.sw_affine_marginals_hvp(s, V, go, ge, T, ln)
+        assert close_or_nearer_truth(got, d["hvp"], lambda: O.sw_affine_hvp(s64, d["V"], go, ge, T, lnn, prec="f64"), tol(d["hvp"]))
+        for p, n in enumerate(("gap_open", "gap_ext", "temp")):
+            got = getattr(torch.ops.orihime, f"sw_affine_marginals_grad_{n}")(s, go, ge, T, ln)
+            truth = lambda p=p: O.sw_affine_param_jacobian(s64, p, go, ge, T, lnn, prec="f64")
+            assert close_or_nearer_truth(got, d[f"sens_{n}"], truth, tol(d[f"sens_{n}"])), n
+        gs, x, y, z = torch.ops.orihime.sw_affine_marginals_backward(s, _t(d["G"]), go, ge, T, ln)
+        truth = lambda: O.sw_affine_backward_full(s64, d["G"], go, ge, T, lnn, prec="f64")[0]
+        assert close_or_nearer_truth(gs, d["mb_scores"], truth, tol(d["mb_scores"]))
+        full = lambda: O.sw_affine_backward_full(s64, d["G"], go, ge, T, lnn, prec="f64")
+        for k, (got, ref) in enumerate(((x, "mb_gap_open"), (y, "mb_gap_ext"), (z, "mb_temp"))):
+            truth = lambda k=k: full()[k + 1]
+            assert close_or_nearer_truth(float(got), float(d[ref]), truth, 1e-3 * max(1.0, abs(float(d[ref])))), ref
+
+
+@pytest.mark.parametrize("path", golden_files("linear"), ids=lambda p: p.split("/")[-1])
+def test_linear_cpu_path_matches_golden(path):
+    d = load(path)
+    g, T = float(d["gap"]), float(d["temp"])
+    s, ln = _t(d["scores"]), _t(d["lengths"])
+    v, m, a, b = torch.ops.orihime.soft_sw_with_grads(s, g, T, ln)
+    assert rel_err(v, d["value"], 1e-3) < 1e-5
+    assert abs_err(m, d["map"]) < 1e-4
+    assert rel_err(a, d["grad_gap"], 1.0) < 1e-4
+    assert np.all(np.abs(b.numpy() - d["grad_temp"]) <= entropy_tol(d["value"], T) * np.maximum(1, np.abs(d["grad_temp"])))
+    if "hvp" in d:
+        s64, lnn = d["scores"].astype(np.float64), d["lengths"]
+        tol = lambda r: 1e-4 * max(1.0, np.abs(r).max())
+        got = torch.ops.orihime.sw_marginals_hvp(s, _t(d["V"]), g, T, ln)
+        assert close_or_nearer_truth(got, d["hvp"], lambda: O.sw_hvp(s64, d["V"], g, T, lnn, prec="f64"), tol(d["hvp"]))
+        for p, n in enumerate(("gap", "temp")):
+            got = getattr(torch.ops.orihime, f"sw_marginals_grad_{n}")(s, g, T, ln)
+            truth = lambda p=p: O.sw_param_jacobian(s64, p, g, T, lnn, prec="f64")
+            assert close_or_nearer_truth(got, d[f"sens_{n}"], truth, tol(d[f"sens_{n}"])), n
+
+
+def test_cpu_path_is_thread_count_invariant():
+    x = torch.randn(6, 21, 17, generator=torch.Generator().manual_seed(5))
+    outs = []
+    old = torch.get_num_threads()
+    try:
+        for n in (1, 2, 4):
+            torch.set_num_threads(n)
+            outs.append(torch.ops.orihime.soft_sw_affine_with_grads(x, -1.0, -0.3, 0.8, None))
+    finally:
+        torch.set_num_threads(old)
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert torch.equal(a, b)
+
+
+# --------------------------------------------------------------------------
+# Public API behaviour
+# --------------------------------------------------------------------------
+
+KW = dict(gap_open_score=-1.0, gap_extend_score=-0.3, temperature=0.9)
+
+
+def test_public_functions_and_autograd():
+    g = torch.Generator().manual_seed(0)
+    x = torch.randn(3, 9, 7, generator=g)
+    ref = O.sw_affine_forward(x.numpy(), -1.0, -0.3, 0.9, prec="f64")
+    assert abs_err(ohm.sw_affine(x, **KW), ref[1]) < 1e-5
+    assert rel_err(ohm.sw_affine_value(x, **KW), ref[0]) < 1e-5
+    assert abs_err(ohm.sw_affine_entropy(x, **KW), ref[4]) < 1e-4
+    xr = x.clone().requires_grad_(True)
+    ohm.sw_affine_value(xr, **KW).sum().backward()
+    assert abs_err(xr.grad, ref[1]) < 1e-5
+    G = torch.randn(x.shape, generator=g)
+    xr = x.clone().requires_grad_(True)
+    (ohm.sw_affine(xr, **KW) * G).sum().backward()
+    assert abs_err(xr.grad, O.sw_affine_hvp(x.numpy(), G.numpy(), -1.0, -0.3, 0.9, prec="f64")) < 1e-5
+    xr = x.clone().requires_grad_(True)
+    ohm.sw_affine_entropy(xr, **KW).sum().backward()
+    dT = O.sw_affine_param_jacobian(x.numpy(), 2, -1.0, -0.3, 0.9, prec="f64")
+    assert abs_err(xr.grad, dT) < 1e-4
+
+
+def test_tensor_parameter_gradients():
+    x = torch.randn(2, 8, 6, generator=torch.Generator().manual_seed(1))
+    go = torch.tensor(-1.0, requires_grad=True)
+    ge = torch.tensor([-0.3], requires_grad=True)
+    T = torch.tensor(0.9, requires_grad=True)
+    ohm.sw_affine_value(x, gap_open_score=go, gap_extend_score=ge, temperature=T).sum().backward()
+    _, _, gO, gE, gT = O.sw_affine_forward(x.numpy(), -1.0, -0.3, 0.9, prec="f64")
+    assert abs(go.grad.item() - gO.sum()) < 1e-4
+    assert ge.grad.shape == (1,) and abs(ge.grad.item() - gE.sum()) < 1e-4
+    assert abs(T.grad.item() - gT.sum()) < 1e-3
+    G = torch.randn(x.shape, generator=torch.Generator().manual_seed(2))
+    go.grad = None
+    (ohm.sw_affine(x, gap_open_score=go, gap_extend_score=-0.3, temperature=0.9) * G).sum().backward()
+    mb = O.sw_affine_backward_full(x.numpy(), G.numpy(), -1.0, -0.3, 0.9, prec="f64")
+    assert abs(go.grad.item() - mb[1]) < 1e-4
+
+
+def test_entropy_parameter_direction_is_rejected():
+    x = torch.randn(1, 4, 4)
+    T = torch.tensor(1.0, requires_grad=True)
+    with pytest.raises(NotImplementedError, match="temperature"):
+        ohm.sw_affine_entropy(x, temperature=T).sum().backward()
+
+
+def test_linear_public_surface():
+    x = torch.randn(2, 7, 9, generator=torch.Generator().manual_seed(3))
+    ref = O.sw_forward(x.numpy(), -0.7, 0.9, prec="f64")
+    assert abs_err(ohm.sw(x, gap_score=-0.7, temperature=0.9), ref[1]) < 1e-5
+    assert rel_err(ohm.sw_value(x, gap_score=-0.7, temperature=0.9), ref[0]) < 1e-5
+    assert abs_err(ohm.sw_entropy(x, gap_score=-0.7, temperature=0.9), ref[3]) < 1e-4
+
+
+def test_ops_named_operations():
+    x = torch.randn(2, 6, 5, generator=torch.Generator().manual_seed(4))
+    G = torch.randn(x.shape, generator=torch.Generator().manual_seed(5))
+    out = ohm.ops.sw_affine.forward(x, **KW)
+    assert set(out) == {"map", "value", "entropy"}
+    assert isinstance(ohm.ops.sw_affine.forward(x, output="value", **KW), torch.Tensor)
+    bw = ohm.ops.sw_affine.backward(x, grad_map=G, **KW)
+    assert set(bw) == {"pair_scores", "gap_open_score", "gap_extend_score", "temperature"}
+    mb = O.sw_affine_backward_full(x.numpy(), G.numpy(), -1.0, -0.3, 0.9, prec="f64")
+    assert abs_err(bw["pair_scores"], mb[0]) < 1e-5
+    for k, ref in zip(("gap_open_score", "gap_extend_score", "temperature"), mb[1:]):
+        assert bw[k].shape == () and abs(bw[k].item() - ref) < 1e-3 * max(1, abs(ref))
+    sens = ohm.ops.sw_affine.sensitivity(x, **KW)
+    for p, k in enumerate(("gap_open_score", "gap_extend_score", "temperature")):
+        ref = O.sw_affine_param_jacobian(x.numpy(), p, -1.0, -0.3, 0.9, prec="f64")
+        assert abs_err(sens[k], ref) < 1e-4
+    with pytest.raises(ValueError, match="unknown output field"):
+        ohm.ops.sw_affine.forward(x, output="nope")
+    assert set(ohm.ops.sw.backward(x, grad_map=G, gap_score=-0.5)) == {"pair_scores", "gap_score", "temperature"}
+
+
+def test_nn_modules():
+    x = torch.randn(2, 5, 6, generator=torch.Generator().manual_seed(6))
+    mod = ohm.nn.SmithWatermanAffine(gap_open_score=-1.0, gap_extend_score=-0.3, temperature=0.9,
+                                     learnable=("gap_open_score", "temperature"))
+    assert isinstance(mod.gap_open_score, torch.nn.Parameter)
+    assert "gap_extend_score" in dict(mod.named_buffers())
+    mod.value(x).sum().backward()
+    _, _, gO, _, gT = O.sw_affine_forward(x.numpy(), -1.0, -0.3, 0.9, prec="f64")
+    assert abs(mod.gap_open_score.grad.item() - gO.sum()) < 1e-4
+    assert abs(mod.temperature.grad.item() - gT.sum()) < 1e-3
+    assert mod(x).shape == x.shape and mod.entropy(x).shape == (2,)
+    lin = ohm.nn.SmithWaterman(gap_score=-0.5, learnable=("gap_score",))
+    lin.value(x).sum().backward()
+    assert lin.gap_score.grad is not None
+    with pytest.raises(ValueError, match="unknown learnable"):
+        ohm.nn.SmithWaterman(learnable=("nope",))
+
+
+def test_raw_tier():
+    x = torch.randn(2, 5, 4, generator=torch.Generator().manual_seed(7))
+    G = torch.randn(x.shape, generator=torch.Generator().manual_seed(8))
+    out = ohm.raw.sw_affine.vjp(x, wrt=("gap_open_score", "temperature"), cotangent=G,
+                                gap_open_score=-1.0, gap_extend_score=-0.3, temperature=0.9)
+    mb = O.sw_affine_backward_full(x.numpy(), G.numpy(), -1.0, -0.3, 0.9, prec="f64")
+    assert abs(out["gap_open_score"].item() - mb[1]) < 1e-4
+    assert abs(out["temperature"].item() - mb[3]) < 1e-3
+    v, m = ohm.raw.sw_affine.forward(x, -1.0, -0.3, 0.9)
+    assert m.shape == x.shape and v.shape == (2,)
+    with pytest.raises(TypeError, match="float32"):
+        ohm.raw.sw_affine.vjp_one(x, wrt="temperature", cotangent=G.double())
+    with pytest.raises(ValueError, match="contiguous"):
+        ohm.raw.sw.vjp_one(x, wrt="gap_score", cotangent=G.transpose(1, 2).contiguous().transpose(1, 2))
+
+
+def test_lengths_and_prefix_masks():
+    x = torch.randn(3, 9, 7, generator=torch.Generator().manual_seed(9))
+    ln = torch.tensor([[9, 7], [5, 4], [0, 3]], dtype=torch.int32)
+    m = ohm.sw_affine(x, lengths=ln, **KW)
+    assert m[1, 5:].abs().max() == 0 and m[1, :, 4:].abs().max() == 0 and m[2].abs().max() == 0
+    v = ohm.sw_affine_value(x, lengths=ln, **KW)
+    v1 = ohm.sw_affine_value(x[1:2, :5, :4].contiguous(), **KW)
+    assert abs(v[1] - v1[0]) < 1e-5 and v[2] == 0
+    mask1 = torch.arange(9)[None, :] < ln[:, 0:1]
+    mask2 = torch.arange(7)[None, :] < ln[:, 1:2]
+    from orihime_amd.sw import _sw_affine_operator
+
+    vm = _sw_affine_operator.value(x, -1.0, -0.3, 0.9, mask1=mask1, mask2=mask2)
+    assert torch.allclose(vm, v)
+    with pytest.raises(ValueError, match="prefix-only"):
+        bad = mask1.clone()
+        bad[0, 0] = False
+        _sw_affine_operator.value(x, -1.0, -0.3, 0.9, mask1=bad)
+
+
+def test_mask_matches_manual_sentinel():
+    x = torch.randn(2, 6, 6, generator=torch.Generator().manual_seed(10))
+    mask = torch.zeros_like(x, dtype=torch.bool)
+    mask[:, 2, 3] = True
+    mask[1, 4] = True
+    a = ohm.sw_affine(x, mask=mask, **KW)
+    b = ohm.sw_affine(x.masked_fill(mask, float("-inf")), **KW)
+    assert torch.equal(a, b)
+    assert a[mask].abs().max() == 0
+
+
+def test_validation_messages():
+    x = torch.randn(2, 5, 5)
+    with pytest.raises(ValueError, match=r"must have shape \[B, L1, L2\]"):
+        ohm.sw_affine(torch.randn(5, 5))
+    with pytest.raises(ValueError, match="B=0"):
+        ohm.sw_affine(torch.randn(0, 5, 5))
+    with pytest.raises(TypeError, match="float32"):
+        ohm.sw_affine(x.double())
+    with pytest.raises(ValueError, match="strictly positive"):
+        ohm.sw_affine(x, temperature=0.0)
+    with pytest.raises(RuntimeError, match="numerical domain"):
+        ohm.sw_affine(x * 100, temperature=1.0)
+    with pytest.raises(RuntimeError, match="unsupported infinity"):
+        ohm.sw_affine(x.masked_fill(x > 1, float("inf")))
+    with pytest.raises(RuntimeError, match="within the padded input shape"):
+        ohm.sw_affine(x, lengths=torch.tensor([[6, 5], [5, 5]], dtype=torch.int32))
+    with pytest.raises(TypeError, match="int32"):
+        ohm.sw_affine(x, lengths=torch.tensor([[5, 5], [5, 5]]))
+    with pytest.raises(ValueError, match=r"shape \[\] or \[1\]"):
+        ohm.sw_affine(x, gap_open_score=torch.zeros(2))
+    with pytest.raises(ValueError, match="only torch.float32"):
+        ohm.sw_affine(x, dtype=torch.float64)
+    with pytest.raises(TypeError, match="mask must be a boolean"):
+        ohm.sw_affine(x, mask=torch.zeros_like(x))
+    with pytest.warns(UserWarning, match="positive"):
+        ohm.sw_affine(x, gap_open_score=0.5)
+    # native boundary messages (src/sw_affine/torch_hip.cpp:35-99)
+    with pytest.raises(RuntimeError, match=r"lengths\[1,0\] must be between 0 and 5, got 9"):
+        torch.ops.orihime.sw_affine_forward(x, -1.0, -1.0, 1.0,
+                                            torch.tensor([[5, 5], [9, 5]], dtype=torch.int32))
+    with pytest.raises(RuntimeError, match="scores must be 3D"):
+        torch.ops.orihime.sw_affine_forward(torch.randn(5, 5), -1.0, -1.0, 1.0, None)
+    with pytest.raises(RuntimeError, match="same shape"):
+        torch.ops.orihime.sw_affine_marginals_hvp(x, torch.randn(2, 5, 4), -1.0, -1.0, 1.0, None)
+    with pytest.raises(RuntimeError, match="must be contiguous"):
+        ohm.ops._kernels["sw_affine"].marginals_hvp(x, torch.randn(2, 5, 5).transpose(1, 2), -1, -1, 1)
+
+
+def test_dtype_escape_hatch_and_autocast():
+    x = torch.randn(2, 5, 6, generator=torch.Generator().manual_seed(11))
+    a = ohm.sw_affine_value(x.bfloat16(), dtype=torch.float32, **KW)
+    b = ohm.sw_affine_value(x.bfloat16().float(), **KW)
+    assert torch.equal(a, b)
+    xr = x.bfloat16().requires_grad_(True)
+    with torch.autocast("cpu", dtype=torch.bfloat16):
+        v = ohm.sw_affine_value(xr, **KW)
+    assert v.dtype == torch.float32
+    v.sum().backward()
+    assert xr.grad.dtype == torch.bfloat16
+
+
+def test_vmap_over_named_primitives():
+    x = torch.randn(3, 2, 5, 4, generator=torch.Generator().manual_seed(12))
+    vm = torch.vmap(lambda s: torch.ops.orihime.sw_affine_forward(s, -1.0, -0.3, 0.9, None)[0])(x)
+    for k in range(3):
+        assert torch.allclose(vm[k], torch.ops.orihime.sw_affine_forward(x[k], -1.0, -0.3, 0.9, None)[0])
+    G = torch.randn(x.shape, generator=torch.Generator().manual_seed(13))
+    out = torch.vmap(lambda s, g: torch.ops.orihime.sw_affine_marginals_backward(s, g, -1.0, -0.3, 0.9, None))(x, G)
+    for k in range(3):
+        ref = torch.ops.orihime.sw_affine_marginals_backward(x[k], G[k].contiguous(), -1.0, -0.3, 0.9, None)
+        assert torch.allclose(out[0][k], ref[0], atol=1e-6)
+        for j in range(1, 4):
+            assert torch.allclose(out[j][k], ref[j], atol=1e-5, rtol=1e-5)
+
+
+def test_fake_tensor_shapes():
+    from torch._subclasses.fake_tensor import FakeTensorMode
+
+    with FakeTensorMode():
+        x = torch.empty(4, 7, 9)
+        v, m = torch.ops.orihime.sw_affine_forward(x, -1.0, -1.0, 1.0, None)
+        assert v.shape == (4,) and m.shape == (4, 7, 9)
+        gs, a, b, c = torch.ops.orihime.sw_affine_marginals_backward(x, x, -1.0, -1.0, 1.0, None)
+        assert gs.shape == x.shape and a.shape == (1,)
+        assert torch.ops.orihime.sw_value_grad_params(x, -1.0, 1.0, None)[0].shape == (4,)
+
+
+def test_forward_primitive_autograd():
+    x = torch.randn(2, 6, 5, generator=torch.Generator().manual_seed(14)).requires_grad_(True)
+    v, m = torch.ops.orihime.sw_affine_forward(x, -1.0, -0.3, 0.9, None)
+    G = torch.randn(x.shape, generator=torch.Generator().manual_seed(15))
+    (v.sum() + (m * G).sum()).backward()
+    ref = O.sw_affine_forward(x.detach().numpy(), -1.0, -0.3, 0.9, prec="f64")[1]
+    H = O.sw_affine_hvp(x.detach().numpy(), G.numpy(), -1.0, -0.3, 0.9, prec="f64")
+    assert abs_err(x.grad, ref + H) < 1e-5
+    go = torch.tensor([-1.0], requires_grad=True)
+    ln = torch.tensor([[6, 5], [4, 5]], dtype=torch.int32)
+    v, m = torch.ops.orihime.sw_affine_forward_t(x.detach(), go, torch.tensor([-0.3]),
+                                                 torch.tensor([0.9]), ln)
+    v.sum().backward()
+    gO = O.sw_affine_forward(x.detach().numpy(), -1.0, -0.3, 0.9, ln.numpy(), prec="f64")[2]
+    assert abs(go.grad.item() - gO.sum()) < 1e-4
+
+
+def test_ragged_packed_matches_padded_lengths():
+    g = torch.Generator().manual_seed(16)
+    pairs = [torch.randn(a, b, generator=g) for a, b in ((9, 7), (5, 4), (1, 8), (12, 3))]
+    flat, ln, off = ohm.ragged.pack(pairs)
+    pad = torch.zeros(4, 12, 8)
+    for k, p in enumerate(pairs):
+        pad[k, :p.shape[0], :p.shape[1]] = p
+    v = ohm.ragged.sw_affine_value(flat, ln, off, **KW)
+    vp = ohm.sw_affine_value(pad, lengths=ln, **KW)
+    assert torch.allclose(v, vp, rtol=1e-6)
+    m = ohm.ragged.sw_affine(flat, ln, off, **KW)
+    mp = ohm.sw_affine(pad, lengths=ln, **KW)
+    for k, mk in enumerate(ohm.ragged.unpack(m, ln, off)):
+        assert torch.allclose(mk, mp[k, :mk.shape[0], :mk.shape[1]], atol=1e-7)
+    fr = flat.clone().requires_grad_(True)
+    ohm.ragged.sw_affine_value(fr, ln, off, **KW).sum().backward()
+    assert torch.allclose(fr.grad, m, atol=1e-7)
