@@ -1,0 +1,433 @@
+#!/usr/bin/env python3
+"""Soft Smith-Waterman (affine gap) forward+backward throughput on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ragged|fixed]
+
+Workload "ragged" (default): 2048 pairs with L1, L2 drawn independently and
+uniformly from [64, 1024], then each pair's N(0,1) FP32 scores in pair order
+(torch.Generator seed 2); gap_open=-8, gap_extend=-0.5, temperature=0.5.
+Workload "fixed": [256, 512, 512] scores N(0,1)*2 - 0.5 (seed 1), gap_open=-11,
+gap_extend=-1, temperature=1.
+
+A step is one forward (value, log-partition) plus backward (marginal map and
+the gradients of gap_open, gap_extend and temperature) over the whole batch:
+one call of ohm_sw_affine_forward with every output requested, i.e. the
+planning kernel, the forward sweep and the reverse sweep.  The ragged batch
+runs as one packed persistent launch; it is compared with the padded launch
+([B, 1024, 1024] with the cells beyond each pair's lengths masked, i.e. set
+to the reference's mask sentinel), which is how the batch runs without ragged
+support.  GCUPS counts the useful cells sum_b L1_b * L2_b for both.
+
+Timed steps replay a HIP graph of one step on inputs already in HBM.  Kernel
+durations come from HIP events recorded between the launches of the same
+calls (ohm_set_stage_events) in a separate eager pass of the same K steps.
+
+Roofline per cell, affine (algorithmic, max-shifted FP32 log-sum-exp):
+forward 11 transcendentals and 4 bytes (read the score), reverse 11 and 4
+bytes (write the map); the binding roof is the transcendental issue rate.
+
+The CPU baseline is the reference's CPU kernels (compiled from source into
+oracle/_ref) or, where that build is absent, the oracle's C restatement of
+them, timed on a bounded prefix of the same batch.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "GCUPS fwd+bwd soft-SW-affine, ragged vs padded, % of roofline, 1 GPU"
+
+# MI355X published / derived peaks
+HBM_PEAK_GBS = 8000.0
+CUS, CLOCK_HZ = 256, 2.4e9
+VALU_LANE_OPS = CUS * 64 * CLOCK_HZ          # non-packed FP32 lane-ops/s (39.3 T)
+TRANS_PEAK = VALU_LANE_OPS / 4               # v_exp/v_log/v_rcp at quarter rate (9.83 T/s)
+TRANS_PER_CELL = {"forward": 11, "reverse": 11}
+BYTES_PER_CELL = {"forward": 4, "reverse": 4}
+KERNEL_NAMES = {
+    "plan": "sw_plan_kernel",
+    "forward": "sw_affine_forward_kernel<true, W>",
+    "reverse": "sw_affine_backward_kernel<W>",
+}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# --------------------------------------------------------------------------
+# Workloads (host-generated, seeded)
+# --------------------------------------------------------------------------
+
+
+def ragged_pairs(torch, seed=2, B=2048, lo=64, hi=1024):
+    g = torch.Generator().manual_seed(seed)
+    lens = torch.randint(lo, hi + 1, (B, 2), generator=g)
+    return [torch.randn(int(a), int(b), generator=g) for a, b in lens.tolist()]
+
+
+def fixed_batch(torch, seed=1, B=256, L=512):
+    g = torch.Generator().manual_seed(seed)
+    return torch.randn(B, L, L, generator=g) * 2 - 0.5
+
+
+# --------------------------------------------------------------------------
+# HIP events through the runtime torch already loaded
+# --------------------------------------------------------------------------
+
+
+class Hip:
+    def __init__(self):
+        self.lib = ctypes.CDLL("libamdhip64.so.7")
+        vp = ctypes.c_void_p
+        self.lib.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+        self.lib.hipEventRecord.argtypes = [vp, vp]
+        self.lib.hipEventSynchronize.argtypes = [vp]
+        self.lib.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+        self.lib.hipEventDestroy.argtypes = [vp]
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+    def event(self) -> int:
+        ev = ctypes.c_void_p()
+        self._ok(self.lib.hipEventCreate(ctypes.byref(ev)), "hipEventCreate")
+        return ev.value
+
+    def record(self, ev: int, stream: int) -> None:
+        self._ok(self.lib.hipEventRecord(ev, stream or None), "hipEventRecord")
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        self._ok(self.lib.hipEventSynchronize(b), "hipEventSynchronize")
+        ms = ctypes.c_float()
+        self._ok(self.lib.hipEventElapsedTime(ctypes.byref(ms), a, b), "hipEventElapsedTime")
+        return float(ms.value)
+
+    def destroy(self, ev: int) -> None:
+        self.lib.hipEventDestroy(ev)
+
+
+# --------------------------------------------------------------------------
+# One configured launch (packed or padded), with its outputs and workspace
+# --------------------------------------------------------------------------
+
+
+class Launch:
+    def __init__(self, torch, lib, scores, lengths, offsets, L1max, L2max, params):
+        self.torch, self.lib = torch, lib
+        self.scores, self.lengths, self.offsets = scores, lengths, offsets
+        self.L1max, self.L2max = L1max, L2max
+        self.params = params
+        B = lengths.shape[0] if lengths is not None else scores.shape[0]
+        dev = scores.device
+        self.B = B
+        self.value = torch.empty(B, device=dev)
+        self.map = torch.empty_like(scores)
+        self.grads = [torch.empty(B, device=dev) for _ in range(3)]
+        total = scores.numel() if offsets is not None else -1
+        nbytes = lib.workspace_bytes(True, lib.OP_FORWARD, B, L1max, L2max, total)
+        self.ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        self.ws_value = torch.empty(lib.workspace_bytes(True, lib.OP_VALUE, B, L1max, L2max, total),
+                                    dtype=torch.uint8, device=dev)
+        self.value_only = torch.empty(B, device=dev)
+        self.graph = None
+
+    def step(self):
+        go, ge, T = self.params
+        self.lib.affine_forward(self.scores, self.lengths, self.offsets, go, ge, T,
+                                value=self.value, map=self.map, grad_open=self.grads[0],
+                                grad_ext=self.grads[1], grad_temp=self.grads[2],
+                                L1max=self.L1max, L2max=self.L2max, workspace=self.ws)
+
+    def forward_only(self):
+        go, ge, T = self.params
+        self.lib.affine_forward(self.scores, self.lengths, self.offsets, go, ge, T,
+                                value=self.value_only, L1max=self.L1max, L2max=self.L2max,
+                                workspace=self.ws_value)
+
+    def outputs(self):
+        return [self.value, self.map] + self.grads
+
+    def capture(self):
+        torch = self.torch
+        self.graph = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.step()  # one eager call on the capture stream first
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(self.graph):
+            self.step()
+        torch.cuda.synchronize()
+
+
+def timed(torch, fn, steps):
+    """Mean ms per call over `steps` calls, bracketed by device syncs."""
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(steps):
+        fn()
+    end.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return start.elapsed_time(end) / steps, wall
+
+
+def kernel_durations(torch, lib, hip, launch, steps):
+    """Mean duration (ms) of each of the step's launches, from HIP events
+    recorded on the launch stream between them."""
+    stream = torch.cuda.current_stream().cuda_stream
+    evs = [[hip.event() for _ in range(4)] for _ in range(steps)]
+    torch.cuda.synchronize()
+    try:
+        for e in evs:
+            hip.record(e[0], stream)
+            lib.set_stage_events(e[1:])
+            launch.step()
+    finally:
+        lib.set_stage_events([])
+    torch.cuda.synchronize()
+    out = {"plan": 0.0, "forward": 0.0, "reverse": 0.0}
+    for e in evs:
+        out["plan"] += hip.elapsed_ms(e[0], e[1])
+        out["forward"] += hip.elapsed_ms(e[1], e[2])
+        out["reverse"] += hip.elapsed_ms(e[2], e[3])
+        for x in e:
+            hip.destroy(x)
+    return {k: v / steps for k, v in out.items()}
+
+
+# --------------------------------------------------------------------------
+# CPU baseline: the reference CPU kernels on a bounded prefix of the batch
+# --------------------------------------------------------------------------
+
+
+def cpu_baseline(torch, pairs, params, seconds):
+    import numpy as np
+
+    from oracle import oracle as O
+
+    kind = "reference" if O.ref_available() else "port"
+    threads = min(16, os.cpu_count() or 1) if kind == "reference" else 1
+    old = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    go, ge, T = params
+    cells = done = 0
+    elapsed = 0.0
+    chunk = 32
+    try:
+        while done < len(pairs) and elapsed < seconds:
+            part = pairs[done:done + chunk]
+            L1 = max(p.shape[0] for p in part)
+            L2 = max(p.shape[1] for p in part)
+            pad = np.zeros((len(part), L1, L2), np.float32)
+            ln = np.zeros((len(part), 2), np.int32)
+            for i, p in enumerate(part):
+                pad[i, :p.shape[0], :p.shape[1]] = p
+                ln[i] = p.shape
+            t0 = time.perf_counter()
+            O.sw_affine_forward(pad, go, ge, T, ln, impl="ref" if kind == "reference" else "oracle")
+            elapsed += time.perf_counter() - t0
+            cells += int((ln[:, 0].astype(np.int64) * ln[:, 1]).sum())
+            done += len(part)
+    finally:
+        torch.set_num_threads(old)
+    return {
+        "value": round(cells / elapsed / 1e9, 6),
+        "unit": "GCUPS",
+        "cores": threads,
+        "kind": kind,
+        "sample": (f"first {done} of {len(pairs)} pairs ({cells} cells), forward+backward "
+                   f"(value, map, gap and temperature gradients), {elapsed:.1f} s"),
+    }
+
+
+# --------------------------------------------------------------------------
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=("ragged", "fixed"), default="ragged")
+    ap.add_argument("--no-padded", action="store_true", help="skip the padded comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-graph", action="store_true", help="time eager calls instead of a graph")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (pairs would be "
+              "sharded across GPUs by cost with no exchange but one all-reduce of three "
+              "parameter-gradient scalars); run it with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import orihime_amd  # noqa: F401
+    from orihime_amd import _lib, ragged
+
+    if not torch.cuda.is_available():
+        sys.exit("bench.py needs a GPU")
+    dev = torch.device("cuda:0")
+    log(f"device {torch.cuda.get_device_name(0)}")
+
+    if args.workload == "ragged":
+        params = (-8.0, -0.5, 0.5)
+        t0 = time.perf_counter()
+        pairs = ragged_pairs(torch)
+        log(f"generated {len(pairs)} pairs in {time.perf_counter() - t0:.1f} s")
+        flat, lengths, offsets = ragged.pack(pairs)
+        flat, lengths, offsets = flat.to(dev), lengths.to(dev), offsets.to(dev)
+        L1max = L2max = 1024
+        cells = int(sum(p.numel() for p in pairs))
+        main_launch = Launch(torch, _lib, flat, lengths, offsets, L1max, L2max, params)
+        workload = (f"ragged affine soft-SW fwd+bwd, 2048 pairs, L1,L2 ~ U[64,1024] (seed 2), "
+                    f"gap_open=-8 gap_extend=-0.5 T=0.5, packed persistent launch")
+        cpu_pairs = [p.numpy() for p in pairs]
+    else:
+        params = (-11.0, -1.0, 1.0)
+        x = fixed_batch(torch)
+        cells = x.numel()
+        main_launch = Launch(torch, _lib, x.to(dev), None, None, 512, 512, params)
+        workload = "fixed-shape affine soft-SW fwd+bwd, [256, 512, 512] (seed 1), go=-11 ge=-1 T=1"
+        cpu_pairs = [x[i].numpy() for i in range(x.shape[0])]
+
+    # warm-up, sanity, graph
+    for _ in range(max(1, args.warmup)):
+        main_launch.step()
+    torch.cuda.synchronize()
+    if not bool(torch.isfinite(main_launch.value).all()):
+        sys.exit("non-finite values: workspace overflow or invalid input")
+    eager = [t.clone() for t in main_launch.outputs()]
+    use_graph = not args.no_graph
+    if use_graph:
+        main_launch.capture()
+        for _ in range(2):
+            main_launch.graph.replay()
+            torch.cuda.synchronize()
+            for a, b in zip(eager, main_launch.outputs()):
+                if not torch.equal(a, b):
+                    sys.exit("graph replay differs from the eager call")
+    log("warm-up done")
+
+    # timed region
+    run = main_launch.graph.replay if use_graph else main_launch.step
+    torch.cuda.synchronize()
+    ms, wall_ms = timed(torch, run, args.steps)
+    gcups = cells / (ms * 1e-3) / 1e9
+    log(f"packed fwd+bwd: {ms:.3f} ms/step ({wall_ms:.3f} wall), {gcups:.1f} GCUPS")
+    fwd_ms, _ = timed(torch, main_launch.forward_only, args.steps)
+    log(f"forward only: {fwd_ms:.3f} ms, {cells / (fwd_ms * 1e-3) / 1e9:.1f} GCUPS")
+
+    hip = Hip()
+    kd = kernel_durations(torch, _lib, hip, main_launch, args.steps)
+    log("kernel durations (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in kd.items()))
+    dom = max(("forward", "reverse"), key=lambda k: kd[k])
+    t_dom = kd[dom] * 1e-3
+    ach_gbs = BYTES_PER_CELL[dom] * cells / t_dom / 1e9
+    ach_trans = TRANS_PER_CELL[dom] * cells / t_dom
+    step_roof = min(HBM_PEAK_GBS * 1e9 / sum(BYTES_PER_CELL.values()),
+                    TRANS_PEAK / sum(TRANS_PER_CELL.values())) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(ach_gbs, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(ach_gbs / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": KERNEL_NAMES[dom],
+        "kernel_ms": round(kd[dom], 4),
+        "binding": {
+            "bound": "transcendental",
+            "achieved": round(ach_trans / 1e12, 4),
+            "peak": round(TRANS_PEAK / 1e12, 4),
+            "unit": "Top/s",
+            "frac": round(ach_trans / TRANS_PEAK, 4),
+        },
+        "step_roof_gcups": round(step_roof, 1),
+        "step_frac": round(gcups / step_roof, 4),
+    }
+
+    extra = {}
+    if args.workload == "ragged" and not args.no_padded:
+        # the same batch padded to [B, 1024, 1024], cells beyond the lengths masked
+        from orihime_amd.operator import _normalize_masked
+
+        B = len(pairs)
+        pad = torch.full((B, L1max, L2max), float("-inf"), device=dev)
+        for i, p in enumerate(ragged.unpack(flat, lengths, offsets)):
+            pad[i, :p.shape[0], :p.shape[1]] = p
+        pad = _normalize_masked(pad, params[2]).contiguous()
+        padded = Launch(torch, _lib, pad, None, None, L1max, L2max, params)
+        for _ in range(max(1, args.warmup)):
+            padded.step()
+        torch.cuda.synchronize()
+        if use_graph:
+            padded.capture()
+        prun = padded.graph.replay if use_graph else padded.step
+        pms, _ = timed(torch, prun, max(3, args.steps // 4))
+        pg = cells / (pms * 1e-3) / 1e9
+        log(f"padded+mask fwd+bwd: {pms:.3f} ms/step, {pg:.1f} GCUPS (useful cells)")
+        vdiff = float(((padded.value - main_launch.value).abs()
+                       / main_launch.value.abs().clamp_min(1e-3)).max())
+        extra.update({"padded_ms_per_step": round(pms, 4), "padded_gcups": round(pg, 2),
+                      "ragged_over_padded": round(gcups / pg, 3),
+                      "padded_vs_packed_value_rel_diff": vdiff})
+        del padded, pad
+        torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"CPU baseline (about {args.cpu_seconds:.0f} s)")
+        cpu = cpu_baseline(torch, cpu_pairs, params, args.cpu_seconds)
+        log(f"CPU {cpu['kind']}: {cpu['value']} GCUPS on {cpu['cores']} threads")
+
+    line = {
+        "metric": METRIC,
+        "value": round(gcups, 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": workload,
+            "cells": cells,
+            "forward_gcups": round(cells / (fwd_ms * 1e-3) / 1e9, 3),
+            "kernel_ms": {k: round(v, 4) for k, v in kd.items()},
+            "hip_graph": use_graph,
+            "wall_ms_per_step": round(wall_ms, 4),
+            **extra,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -60,6 +60,16 @@ const char* ohm_last_error(void);
 /* Library version, "major.minor.patch". */
 const char* ohm_version(void);
 
+/* Kernel timing hook (diagnostics; the reference has no counterpart).  While
+ * set, every GPU entry point records events[k] (a hipEvent_t created by the
+ * caller) on its stream right after its k-th kernel launch: k = 0 the
+ * planning kernel, k = 1 the forward sweep (or tangent forward sweep), k = 2
+ * the reverse sweep.  NULL entries are skipped; count 0 clears the hook.
+ * The setting is process-wide and unsynchronised: change it only while no
+ * other thread is inside the library, and never while a HIP graph is being
+ * captured. */
+ohm_status ohm_set_stage_events(void* const* events, int32_t count);
+
 /* Operation kinds for the workspace queries. */
 #define OHM_OP_VALUE 0    /* value only */
 #define OHM_OP_FORWARD 1  /* value + map + parameter gradients */

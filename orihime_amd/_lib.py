@@ -35,6 +35,7 @@ _sz = ctypes.c_size_t
 EXPORTS = (
     "ohm_last_error",
     "ohm_version",
+    "ohm_set_stage_events",
     "ohm_sw_affine_workspace",
     "ohm_sw_workspace",
     "ohm_sw_affine_forward",
@@ -57,6 +58,7 @@ def _sig(name, restype, *argtypes):
 
 _last_error = _sig("ohm_last_error", ctypes.c_char_p)
 _version = _sig("ohm_version", ctypes.c_char_p)
+_stage_events = _sig("ohm_set_stage_events", ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), _i32)
 _ws_affine = _sig("ohm_sw_affine_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 _ws_linear = _sig("ohm_sw_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 
@@ -101,6 +103,14 @@ def _ptr(t: torch.Tensor | None) -> int | None:
 
 _ws_lock = threading.Lock()
 _ws_cache: dict[tuple[int, int], torch.Tensor] = {}
+
+
+def set_stage_events(events) -> None:
+    """Record the given hipEvent_t handles (ints, up to 3; None skips one)
+    after the planning, forward and reverse launches of every GPU call; an
+    empty sequence clears the hook."""
+    arr = (ctypes.c_void_p * 3)(*[(e or None) for e in events])
+    _check(_stage_events(arr, len(events)), "ohm_set_stage_events")
 
 
 def workspace_bytes(affine: bool, op: int, B: int, L1: int, L2: int, total_cells: int = -1) -> int:

@@ -1,7 +1,9 @@
 """Build orihime_amd/libohm_sw.so: the gfx950 HIP kernels and the host (CPU)
 implementation behind the C ABI declared in include/ohm_sw.h.
 
-    python -m orihime_amd.build_native          # or build() from Python
+    python orihime_amd/build_native.py          # or build() from Python
+
+(run it as a script: importing the package loads the library it builds).
 
 The library is built in-tree so it travels with the source tree.
 """

@@ -837,6 +837,13 @@ int env_int(const char* name, int fallback) {
 // critical path (its strips run concurrently), fewer waste less on short
 // pairs.  Workgroups: enough persistent groups for 16 waves per CU, never more
 // than there are pairs.  OHM_WAVES / OHM_WG_PER_CU override for tuning.
+// Optional per-launch timing events (ohm_set_stage_events).
+void* g_stage_events[3] = {nullptr, nullptr, nullptr};
+
+void stage_mark(int k, hipStream_t stream) {
+    if (g_stage_events[k]) (void)hipEventRecord(static_cast<hipEvent_t>(g_stage_events[k]), stream);
+}
+
 struct Geometry {
     int waves, groups;
 };
@@ -913,6 +920,7 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     hipLaunchKernelGGL(sw_plan_kernel, dim3(1), dim3(kPlanThreads), 0, L.stream, lengths, offsets,
                        B, L1max, L2max, planes, cap, const_cast<PairDesc*>(a.desc),
                        const_cast<int32_t*>(a.order), a.claims, (float*)nullptr);
+    stage_mark(0, L.stream);
     return check_launch("plan kernel");
 }
 
@@ -949,10 +957,12 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
         if (need_bwd) OHM_DISPATCH_W(L, (sw_forward_kernel<true, W>));
         else OHM_DISPATCH_W(L, (sw_forward_kernel<false, W>));
     }
+    stage_mark(1, L.stream);
     if ((st = check_launch("forward kernel")) != OHM_OK) return st;
     if (!need_bwd) return OHM_OK;
     if (affine) OHM_DISPATCH_W(L, (sw_affine_backward_kernel<W>));
     else OHM_DISPATCH_W(L, (sw_backward_kernel<W>));
+    stage_mark(2, L.stream);
     return check_launch("backward kernel");
 }
 
@@ -980,22 +990,33 @@ ohm_status tangent_impl(bool affine, const float* scores, const int32_t* lengths
     if (affine) {
         if (dir) {
             OHM_DISPATCH_W(L, (sw_affine_hvp_forward_kernel<W>));
+            stage_mark(1, L.stream);
             if ((st = check_launch("tangent forward")) != OHM_OK) return st;
             OHM_DISPATCH_W(L, (sw_affine_hvp_backward_kernel<W>));
         } else {
             OHM_DISPATCH_W(L, (sw_affine_param_grad_forward_kernel<W>));
+            stage_mark(1, L.stream);
             if ((st = check_launch("tangent forward")) != OHM_OK) return st;
             OHM_DISPATCH_W(L, (sw_affine_param_grad_backward_kernel<W>));
         }
     } else {
         OHM_DISPATCH_W(L, (sw_hvp_forward_kernel<W>));
+        stage_mark(1, L.stream);
         if ((st = check_launch("tangent forward")) != OHM_OK) return st;
         OHM_DISPATCH_W(L, (sw_hvp_backward_kernel<W>));
     }
+    stage_mark(2, L.stream);
     return check_launch("tangent backward");
 }
 
 }  // namespace
+
+extern "C" ohm_status ohm_set_stage_events(void* const* events, int32_t count) {
+    if (count < 0 || count > 3 || (count > 0 && !events))
+        return fail(OHM_EINVAL, "stage events: count must be 0..3 with a non-null array");
+    for (int k = 0; k < 3; ++k) g_stage_events[k] = k < count ? events[k] : nullptr;
+    return OHM_OK;
+}
 
 extern "C" size_t ohm_sw_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
                                           int64_t total_cells) {

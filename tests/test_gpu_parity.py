@@ -1,0 +1,439 @@
+"""GPU parity: the gfx950 kernels behind the C ABI against the reference.
+
+Every case compares the HIP path with
+* the golden fixtures (the reference's own CPU kernels, src/sw*/kernels_cpu.cpp,
+  compiled from source) at the reference suite's tolerances -- values 1e-5
+  relative, maps 1e-4 absolute (tests/test_soft_sw_affine.py:29-30);
+* the oracle (a plain-C restatement of those kernels, FP32 and FP64 builds)
+  on fresh seeded inputs, and reports the error against the FP64 build.
+
+Second-order outputs (HVP, parameter sensitivities, marginals backward) scale
+like 1/T and carry FP32 noise in the reference itself; they pass when they
+match the FP32 reference to 1e-4 relative to their size, or else lie within
+twice the reference's own FP32 distance from the FP64 result
+(helpers.close_or_nearer_truth).
+
+At full sizes (256 x 512 x 512, 4096-long pairs) the checks are sampled pairs
+against the oracle plus size-independent properties: bitwise repeatability,
+exact scaling covariance (s, gaps, T all doubled leaves the map bit-identical
+and doubles the value), packed == padded per pair.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import orihime_amd as ohm
+from orihime_amd import _lib, ragged
+from helpers import abs_err, affine_params, close_or_nearer_truth, golden_files, load, rel_err
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def case(msg):
+    """Name the case before any GPU work (a fault leaves this as the last line)."""
+    print(f"[gpu case] {msg}", flush=True)
+
+
+def g(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def h(t):
+    return t.detach().cpu().numpy()
+
+
+def tol2(ref):
+    return 1e-4 * max(1.0, float(np.abs(np.asarray(ref)).max()) if np.asarray(ref).size else 1.0)
+
+
+def entropy_ok(got, ref, value, T):
+    tol = 1e-4 + 8.0 * (np.abs(np.asarray(value, np.float64)) + 1.0) * 2.0 ** -23 / T
+    return bool(np.all(np.abs(np.asarray(got, np.float64) - ref) <= tol * np.maximum(1, np.abs(ref))))
+
+
+def report(name, got, f64):
+    print(f"  {name}: max abs err vs FP64 oracle {abs_err(got, f64):.3e}", flush=True)
+
+
+# --------------------------------------------------------------------------
+# Golden fixtures (reference CPU kernels)
+# --------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("path", golden_files("affine"), ids=lambda p: p.split("/")[-1])
+def test_affine_gpu_matches_golden(path):
+    d = load(path)
+    go, ge, T = affine_params(d)
+    case(f"affine golden {os.path.basename(path)} shape={d['scores'].shape}")
+    s, ln = g(d["scores"]), g(d["lengths"])
+    v, m, a, b, c = torch.ops.orihime.soft_sw_affine_with_grads(s, go, ge, T, ln)
+    torch.cuda.synchronize()
+    assert rel_err(h(v), d["value"], 1e-3) < 1e-5
+    assert abs_err(h(m), d["map"]) < 1e-4
+    assert rel_err(h(a), d["grad_open"], 1.0) < 1e-4
+    assert rel_err(h(b), d["grad_ext"], 1.0) < 1e-4
+    assert entropy_ok(h(c), d["grad_temp"], d["value"], T)
+    s64, lnn = d["scores"].astype(np.float64), d["lengths"]
+    f64 = O.sw_affine_forward(s64, go, ge, T, lnn, prec="f64")
+    report("value", h(v), f64[0])
+    report("map", h(m), f64[1])
+    if "hvp" in d:
+        hv = torch.ops.orihime.sw_affine_marginals_hvp(s, g(d["V"]), go, ge, T, ln)
+        assert close_or_nearer_truth(h(hv), d["hvp"],
+                                     lambda: O.sw_affine_hvp(s64, d["V"], go, ge, T, lnn, prec="f64"),
+                                     tol2(d["hvp"]))
+        for p, n in enumerate(("gap_open", "gap_ext", "temp")):
+            got = getattr(torch.ops.orihime, f"sw_affine_marginals_grad_{n}")(s, go, ge, T, ln)
+            truth = lambda p=p: O.sw_affine_param_jacobian(s64, p, go, ge, T, lnn, prec="f64")
+            assert close_or_nearer_truth(h(got), d[f"sens_{n}"], truth, tol2(d[f"sens_{n}"])), n
+        gs, x, y, z = torch.ops.orihime.sw_affine_marginals_backward(s, g(d["G"]), go, ge, T, ln)
+        full = O.sw_affine_backward_full(s64, d["G"], go, ge, T, lnn, prec="f64")
+        assert close_or_nearer_truth(h(gs), d["mb_scores"], lambda: full[0], tol2(d["mb_scores"]))
+        for k, (got, ref) in enumerate(((x, "mb_gap_open"), (y, "mb_gap_ext"), (z, "mb_temp"))):
+            assert close_or_nearer_truth(float(got), float(d[ref]), lambda k=k: full[k + 1],
+                                         1e-3 * max(1.0, abs(float(d[ref])))), ref
+
+
+@pytest.mark.parametrize("path", golden_files("linear"), ids=lambda p: p.split("/")[-1])
+def test_linear_gpu_matches_golden(path):
+    d = load(path)
+    gp, T = float(d["gap"]), float(d["temp"])
+    case(f"linear golden {os.path.basename(path)} shape={d['scores'].shape}")
+    s, ln = g(d["scores"]), g(d["lengths"])
+    v, m, a, b = torch.ops.orihime.soft_sw_with_grads(s, gp, T, ln)
+    torch.cuda.synchronize()
+    assert rel_err(h(v), d["value"], 1e-3) < 1e-5
+    assert abs_err(h(m), d["map"]) < 1e-4
+    assert rel_err(h(a), d["grad_gap"], 1.0) < 1e-4
+    assert entropy_ok(h(b), d["grad_temp"], d["value"], T)
+    s64, lnn = d["scores"].astype(np.float64), d["lengths"]
+    f64 = O.sw_forward(s64, gp, T, lnn, prec="f64")
+    report("value", h(v), f64[0])
+    report("map", h(m), f64[1])
+    if "hvp" in d:
+        hv = torch.ops.orihime.sw_marginals_hvp(s, g(d["V"]), gp, T, ln)
+        assert close_or_nearer_truth(h(hv), d["hvp"],
+                                     lambda: O.sw_hvp(s64, d["V"], gp, T, lnn, prec="f64"),
+                                     tol2(d["hvp"]))
+        for p, n in enumerate(("gap", "temp")):
+            got = getattr(torch.ops.orihime, f"sw_marginals_grad_{n}")(s, gp, T, ln)
+            truth = lambda p=p: O.sw_param_jacobian(s64, p, gp, T, lnn, prec="f64")
+            assert close_or_nearer_truth(h(got), d[f"sens_{n}"], truth, tol2(d[f"sens_{n}"])), n
+
+
+# --------------------------------------------------------------------------
+# Fresh seeded inputs against the oracle: strip edges, skinny and ragged shapes
+# --------------------------------------------------------------------------
+
+SHAPES = [(2, 1, 1), (2, 1, 70), (2, 70, 1), (3, 63, 65), (2, 64, 64), (2, 65, 17), (1, 129, 200),
+          (2, 200, 129), (2, 257, 31), (1, 16, 300), (2, 191, 193)]
+PARAMS = [(-11.0, -1.0, 1.0), (-8.0, -0.5, 0.5), (-1.0, -0.3, 2.0)]
+
+
+def _inputs(B, L1, L2, seed, ragged_lengths=True):
+    rng = np.random.default_rng(seed)
+    s = (rng.normal(size=(B, L1, L2)) * 2 - 0.5).astype(np.float32)
+    V = rng.normal(size=s.shape).astype(np.float32)
+    G = rng.normal(size=s.shape).astype(np.float32)
+    ln = np.tile(np.array([[L1, L2]], np.int32), (B, 1))
+    if ragged_lengths and B > 1:
+        ln[1] = [rng.integers(0, L1 + 1), rng.integers(0, L2 + 1)]
+    return s, V, G, ln
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("params", PARAMS, ids=lambda p: "go%g_ge%g_T%g" % p)
+def test_affine_gpu_vs_oracle(shape, params):
+    go, ge, T = params
+    s, V, G, ln = _inputs(*shape, seed=hash((shape, params)) % 2**31)
+    case(f"affine oracle shape={shape} params={params} lengths={ln.tolist()}")
+    st, lt = g(s), g(ln)
+    v, m, a, b, c = torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)
+    ref = O.sw_affine_forward(s, go, ge, T, ln)
+    f64 = O.sw_affine_forward(s.astype(np.float64), go, ge, T, ln, prec="f64")
+    assert rel_err(h(v), ref[0], 1e-3) < 1e-5 and rel_err(h(v), f64[0], 1e-3) < 1e-5
+    assert abs_err(h(m), ref[1]) < 1e-4
+    assert rel_err(h(a), ref[2], 1.0) < 1e-4 and rel_err(h(b), ref[3], 1.0) < 1e-4
+    assert entropy_ok(h(c), ref[4], ref[0], T)
+    report("map", h(m), f64[1])
+    s64 = s.astype(np.float64)
+    hv = torch.ops.orihime.sw_affine_marginals_hvp(st, g(V), go, ge, T, lt)
+    r = O.sw_affine_hvp(s, V, go, ge, T, ln)
+    assert close_or_nearer_truth(h(hv), r, lambda: O.sw_affine_hvp(s64, V, go, ge, T, ln, prec="f64"), tol2(r))
+    for p, n in enumerate(("gap_open", "gap_ext", "temp")):
+        got = getattr(torch.ops.orihime, f"sw_affine_marginals_grad_{n}")(st, go, ge, T, lt)
+        r = O.sw_affine_param_jacobian(s, p, go, ge, T, ln)
+        truth = lambda p=p: O.sw_affine_param_jacobian(s64, p, go, ge, T, ln, prec="f64")
+        assert close_or_nearer_truth(h(got), r, truth, tol2(r)), n
+    gs, x, y, z = torch.ops.orihime.sw_affine_marginals_backward(st, g(G), go, ge, T, lt)
+    r = O.sw_affine_backward_full(s, G, go, ge, T, ln)
+    full = O.sw_affine_backward_full(s64, G, go, ge, T, ln, prec="f64")
+    assert close_or_nearer_truth(h(gs), r[0], lambda: full[0], tol2(r[0]))
+    for k, got in enumerate((x, y, z)):
+        assert close_or_nearer_truth(float(got), r[k + 1], lambda k=k: full[k + 1],
+                                     1e-3 * max(1.0, abs(r[k + 1])))
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("params", [(-1.0, 1.0), (-0.4, 0.5), (-2.0, 2.0)], ids=lambda p: "g%g_T%g" % p)
+def test_linear_gpu_vs_oracle(shape, params):
+    gp, T = params
+    s, V, G, ln = _inputs(*shape, seed=hash((shape, params, 1)) % 2**31)
+    case(f"linear oracle shape={shape} params={params} lengths={ln.tolist()}")
+    st, lt = g(s), g(ln)
+    v, m, a, b = torch.ops.orihime.soft_sw_with_grads(st, gp, T, lt)
+    ref = O.sw_forward(s, gp, T, ln)
+    f64 = O.sw_forward(s.astype(np.float64), gp, T, ln, prec="f64")
+    assert rel_err(h(v), ref[0], 1e-3) < 1e-5 and rel_err(h(v), f64[0], 1e-3) < 1e-5
+    assert abs_err(h(m), ref[1]) < 1e-4
+    assert rel_err(h(a), ref[2], 1.0) < 1e-4
+    assert entropy_ok(h(b), ref[3], ref[0], T)
+    report("map", h(m), f64[1])
+    s64 = s.astype(np.float64)
+    hv = torch.ops.orihime.sw_marginals_hvp(st, g(V), gp, T, lt)
+    r = O.sw_hvp(s, V, gp, T, ln)
+    assert close_or_nearer_truth(h(hv), r, lambda: O.sw_hvp(s64, V, gp, T, ln, prec="f64"), tol2(r))
+    for p, n in enumerate(("gap", "temp")):
+        got = getattr(torch.ops.orihime, f"sw_marginals_grad_{n}")(st, gp, T, lt)
+        r = O.sw_param_jacobian(s, p, gp, T, ln)
+        truth = lambda p=p: O.sw_param_jacobian(s64, p, gp, T, ln, prec="f64")
+        assert close_or_nearer_truth(h(got), r, truth, tol2(r)), n
+    gs, x, y = torch.ops.orihime.sw_marginals_backward(st, g(G), gp, T, lt)
+    r = O.sw_backward_full(s, G, gp, T, ln)
+    full = O.sw_backward_full(s64, G, gp, T, ln, prec="f64")
+    assert close_or_nearer_truth(h(gs), r[0], lambda: full[0], tol2(r[0]))
+    for k, got in enumerate((x, y)):
+        assert close_or_nearer_truth(float(got), r[k + 1], lambda k=k: full[k + 1],
+                                     1e-3 * max(1.0, abs(r[k + 1])))
+
+
+# --------------------------------------------------------------------------
+# Packed ragged launches
+# --------------------------------------------------------------------------
+
+
+def _ragged_pairs(n, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    sizes = [(int(rng.integers(lo, hi + 1)), int(rng.integers(lo, hi + 1))) for _ in range(n)]
+    pairs = [rng.normal(size=sz).astype(np.float32) for sz in sizes]
+    return pairs
+
+
+@pytest.mark.parametrize("affine", [True, False], ids=["affine", "linear"])
+def test_packed_ragged_matches_oracle_per_pair(affine):
+    pairs = _ragged_pairs(24, 0, 300, seed=11 if affine else 12)
+    pairs[3] = np.zeros((0, 5), np.float32)
+    pairs[7] = np.zeros((4, 0), np.float32)
+    case(f"packed ragged affine={affine} shapes={[p.shape for p in pairs]}")
+    flat, ln, off = ragged.pack([torch.from_numpy(p).to(DEV) for p in pairs])
+    go, ge, T = -8.0, -0.5, 0.5
+    rng = np.random.default_rng(5)
+    V = [rng.normal(size=p.shape).astype(np.float32) for p in pairs]
+    Vf = ragged.pack([torch.from_numpy(x).to(DEV) for x in V])[0]
+    if affine:
+        v, m, a, b, c = ragged.sw_affine_forward(flat, ln, off, gap_open_score=go,
+                                                 gap_extend_score=ge, temperature=T)
+        mt, con = ragged.sw_affine_tangent(flat, ln, off, gap_open_score=go, gap_extend_score=ge,
+                                           temperature=T, direction=Vf, contract=True)
+    else:
+        v, m, a, c = ragged.sw_forward(flat, ln, off, gap_score=go, temperature=T)
+        mt, con = ragged.sw_tangent(flat, ln, off, gap_score=go, temperature=T, direction=Vf,
+                                    contract=True)
+    maps = ragged.unpack(m, ln, off)
+    tans = ragged.unpack(mt, ln, off)
+    for bi, p in enumerate(pairs):
+        s = p[None]
+        if affine:
+            ref = O.sw_affine_forward(s, go, ge, T)
+            hv = O.sw_affine_hvp(s, V[bi][None], go, ge, T)
+            hv64 = lambda: O.sw_affine_hvp(s.astype(np.float64), V[bi][None], go, ge, T, prec="f64")
+        else:
+            ref = O.sw_forward(s, go, T)
+            hv = O.sw_hvp(s, V[bi][None], go, T)
+            hv64 = lambda: O.sw_hvp(s.astype(np.float64), V[bi][None], go, T, prec="f64")
+        assert rel_err(h(v[bi:bi + 1]), ref[0], 1e-3) < 1e-5, bi
+        assert abs_err(h(maps[bi]), ref[1][0]) < 1e-4, bi
+        assert rel_err(h(a[bi:bi + 1]), ref[2], 1.0) < 1e-4, bi
+        assert close_or_nearer_truth(h(tans[bi]), hv[0], lambda: hv64()[0], tol2(hv)), bi
+
+
+def test_padded_mask_launch_equals_packed_launch():
+    """The two launches compared by the benchmark compute the same thing."""
+    pairs = _ragged_pairs(10, 1, 200, seed=13)
+    L1m = max(p.shape[0] for p in pairs)
+    L2m = max(p.shape[1] for p in pairs)
+    case(f"padded+mask vs packed, {len(pairs)} pairs up to {L1m}x{L2m}")
+    pad = torch.zeros(len(pairs), L1m, L2m)
+    mask = torch.ones(len(pairs), L1m, L2m, dtype=torch.bool)
+    for i, p in enumerate(pairs):
+        pad[i, :p.shape[0], :p.shape[1]] = torch.from_numpy(p)
+        mask[i, :p.shape[0], :p.shape[1]] = False
+    kw = dict(gap_open_score=-8.0, gap_extend_score=-0.5, temperature=0.5)
+    padt = pad.to(DEV).requires_grad_(True)
+    vm = ohm.sw_affine_value(padt, mask=mask.to(DEV), **kw)
+    vm.sum().backward()
+    flat, ln, off = ragged.pack([torch.from_numpy(p).to(DEV) for p in pairs])
+    vp, mp, *_ = ragged.sw_affine_forward(flat, ln, off, **kw)
+    assert rel_err(h(vm), h(vp), 1e-3) < 1e-5
+    for i, m in enumerate(ragged.unpack(mp, ln, off)):
+        assert abs_err(h(padt.grad[i, :m.shape[0], :m.shape[1]]), h(m)) < 1e-5
+        assert float(padt.grad[i][mask[i].to(DEV)].abs().max()) == 0.0
+
+
+# --------------------------------------------------------------------------
+# Full-size properties
+# --------------------------------------------------------------------------
+
+
+def test_full_size_fixed_shape_sampled_pairs_and_invariants():
+    """256 pairs of 512 x 512 (N(0,1)*2 - 0.5, seed 1; go=-11, ge=-1, T=1)."""
+    case("affine 256x512x512 seed 1 fwd+bwd")
+    gen = torch.Generator().manual_seed(1)
+    s = torch.randn(256, 512, 512, generator=gen) * 2 - 0.5
+    st = s.to(DEV)
+    go, ge, T = -11.0, -1.0, 1.0
+    out1 = torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, None)
+    out2 = torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, None)
+    for x, y in zip(out1, out2):
+        assert torch.equal(x, y)  # bitwise repeatable
+    case("affine 256x512x512 scaled by 2")
+    sc = torch.ops.orihime.soft_sw_affine_with_grads(st * 2, go * 2, ge * 2, T * 2, None)
+    assert torch.equal(sc[0], out1[0] * 2)
+    assert torch.equal(sc[1], out1[1])
+    idx = [0, 97, 255]
+    sub = s[idx].numpy()
+    ref = O.sw_affine_forward(sub, go, ge, T)
+    f64 = O.sw_affine_forward(sub.astype(np.float64), go, ge, T, prec="f64")
+    v, m, a, b, c = (h(x[idx]) for x in out1)
+    assert rel_err(v, ref[0], 1e-3) < 1e-5 and rel_err(v, f64[0], 1e-3) < 1e-5
+    assert abs_err(m, ref[1]) < 1e-4
+    assert rel_err(a, ref[2], 1.0) < 1e-4 and rel_err(b, ref[3], 1.0) < 1e-4
+    assert entropy_ok(c, ref[4], ref[0], T)
+    report("value", v, f64[0])
+    report("map", m, f64[1])
+
+
+def test_long_pair_forward_backward_and_hvp():
+    """One 2048 x 2048 pair: 32 row strips pipelined across a workgroup's waves."""
+    case("affine 1x2048x2048 seed 3 fwd+bwd+hvp")
+    gen = torch.Generator().manual_seed(3)
+    s = torch.randn(1, 2048, 2048, generator=gen)
+    V = torch.randn(1, 2048, 2048, generator=gen)
+    go, ge, T = -11.0, -1.0, 1.0
+    st = s.to(DEV)
+    v, m, a, b, c = torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, None)
+    hv = torch.ops.orihime.sw_affine_marginals_hvp(st, V.to(DEV), go, ge, T, None)
+    ref = O.sw_affine_forward(s.numpy(), go, ge, T)
+    assert rel_err(h(v), ref[0], 1e-3) < 1e-5
+    assert abs_err(h(m), ref[1]) < 1e-4
+    assert rel_err(h(a), ref[2], 1.0) < 1e-4 and rel_err(h(b), ref[3], 1.0) < 1e-4
+    r = O.sw_affine_hvp(s.numpy(), V.numpy(), go, ge, T)
+    truth = lambda: O.sw_affine_hvp(s.numpy().astype(np.float64), V.numpy(), go, ge, T, prec="f64")
+    assert close_or_nearer_truth(h(hv), r, truth, tol2(r))
+
+
+def test_c3_style_ragged_batch_sampled():
+    """Lengths uniform in [64, 1024] (go=-8, ge=-0.5, T=0.5), packed, 96 pairs."""
+    pairs = _ragged_pairs(96, 64, 1024, seed=2)
+    case("ragged 96 pairs lengths 64..1024 fwd+bwd")
+    flat, ln, off = ragged.pack([torch.from_numpy(p).to(DEV) for p in pairs])
+    v, m, a, b, c = ragged.sw_affine_forward(flat, ln, off, gap_open_score=-8.0,
+                                             gap_extend_score=-0.5, temperature=0.5)
+    v2, m2, *_ = ragged.sw_affine_forward(flat, ln, off, gap_open_score=-8.0,
+                                          gap_extend_score=-0.5, temperature=0.5)
+    assert torch.equal(v, v2) and torch.equal(m, m2)
+    maps = ragged.unpack(m, ln, off)
+    for bi in (0, 17, 50, 95):
+        ref = O.sw_affine_forward(pairs[bi][None], -8.0, -0.5, 0.5)
+        assert rel_err(h(v[bi:bi + 1]), ref[0], 1e-3) < 1e-5
+        assert abs_err(h(maps[bi]), ref[1][0]) < 1e-4
+        assert rel_err(h(a[bi:bi + 1]), ref[2], 1.0) < 1e-4
+
+
+# --------------------------------------------------------------------------
+# The boundary itself
+# --------------------------------------------------------------------------
+
+
+def test_c_abi_called_directly_with_device_pointers():
+    """ohm_sw_affine_forward with plain pointers, a caller-owned workspace and
+    the null stream: what a ctypes binding in the reference would do."""
+    s, _, _, ln = _inputs(3, 70, 90, seed=21)
+    case("C ABI direct affine 3x70x90")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    st, lt = g(s), g(ln)
+    B = 3
+    need = lib.ohm_sw_affine_workspace
+    need.restype = ctypes.c_size_t
+    need.argtypes = [ctypes.c_int32] * 4 + [ctypes.c_int64]
+    nbytes = need(_lib.OP_FORWARD, B, 70, 90, -1)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=DEV)
+    value = torch.empty(B, device=DEV)
+    mp = torch.empty_like(st)
+    go_, ge_, gt_ = (torch.empty(B, device=DEV) for _ in range(3))
+    fn = lib.ohm_sw_affine_forward
+    fn.restype = ctypes.c_int
+    vp = ctypes.c_void_p
+    fn.argtypes = [vp, vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                   ctypes.c_float, ctypes.c_float, vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
+    torch.cuda.synchronize()
+    st_ = fn(st.data_ptr(), lt.data_ptr(), None, B, 70, 90, -8.0, -0.5, 0.5, value.data_ptr(),
+             mp.data_ptr(), go_.data_ptr(), ge_.data_ptr(), gt_.data_ptr(), ws.data_ptr(), nbytes,
+             None)
+    assert st_ == 0
+    torch.cuda.synchronize()
+    ref = O.sw_affine_forward(s, -8.0, -0.5, 0.5, ln)
+    assert rel_err(h(value), ref[0], 1e-3) < 1e-5
+    assert abs_err(h(mp), ref[1]) < 1e-4
+    # a workspace one byte short is refused before anything is launched
+    assert fn(st.data_ptr(), lt.data_ptr(), None, B, 70, 90, -8.0, -0.5, 0.5, value.data_ptr(),
+              mp.data_ptr(), None, None, None, ws.data_ptr(), nbytes - 1, None) == 2
+    lib.ohm_last_error.restype = ctypes.c_char_p
+    assert b"workspace" in lib.ohm_last_error()
+
+
+def test_gpu_tensors_never_take_the_host_path(monkeypatch):
+    """Every op on a GPU tensor runs the HIP entry points: the host twins are
+    made to raise, and the GPU results still come back."""
+    def boom(*a, **k):
+        raise AssertionError("host path called for a GPU tensor")
+
+    for name in ("_aff_fwd_cpu", "_aff_tan_cpu", "_lin_fwd_cpu", "_lin_tan_cpu"):
+        monkeypatch.setattr(_lib, name, boom)
+    case("no host fallback for GPU tensors")
+    x = torch.randn(2, 33, 47, device=DEV, requires_grad=True)
+    kw = dict(gap_open_score=-1.0, gap_extend_score=-0.3, temperature=0.9)
+    m = ohm.sw_affine(x, **kw)
+    (m * torch.randn_like(m)).sum().backward()
+    ohm.sw_affine_entropy(x.detach(), **kw)
+    ohm.sw(x.detach(), gap_score=-1.0, temperature=0.9)
+    torch.ops.orihime.sw_affine_marginals_hvp(x.detach(), torch.randn_like(x), -1.0, -0.3, 0.9, None)
+    torch.cuda.synchronize()
+
+
+def test_autograd_through_gpu_kernels():
+    rng = np.random.default_rng(31)
+    s = rng.normal(size=(3, 40, 52)).astype(np.float32)
+    G = rng.normal(size=s.shape).astype(np.float32)
+    case("autograd affine 3x40x52")
+    go = torch.tensor(-1.2, device=DEV, requires_grad=True)
+    ge = torch.tensor(-0.4, device=DEV, requires_grad=True)
+    T = torch.tensor(0.8, device=DEV, requires_grad=True)
+    x = g(s).requires_grad_(True)
+    v = ohm.sw_affine_value(x, gap_open_score=go, gap_extend_score=ge, temperature=T)
+    v.sum().backward()
+    ref = O.sw_affine_forward(s.astype(np.float64), -1.2, -0.4, 0.8, prec="f64")
+    assert abs_err(h(x.grad), ref[1]) < 1e-4
+    assert abs(float(go.grad) - ref[2].sum()) < 1e-3 * max(1, abs(ref[2].sum()))
+    assert abs(float(T.grad) - ref[4].sum()) < 1e-3 * max(1, abs(ref[4].sum()))
+    x.grad = None
+    m = ohm.sw_affine(x, gap_open_score=-1.2, gap_extend_score=-0.4, temperature=0.8)
+    (m * g(G)).sum().backward()
+    full = O.sw_affine_backward_full(s.astype(np.float64), G, -1.2, -0.4, 0.8, prec="f64")
+    assert abs_err(h(x.grad), full[0]) < 1e-4 * max(1.0, np.abs(full[0]).max())
