@@ -44,6 +44,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <string>
 
 #include "ohm_sw.h"
@@ -74,6 +75,68 @@ DEV float from_next_lane(float v, float edge) {
         float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
                                            __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
 }
+// Boundary queues: the 16 boundary entries of a chunk sit in lanes 0..15
+// (forward, entry u in lane u) or 48..63 (reverse, entry u in lane 63 - u);
+// one rotation per step brings entry u to lane 0 / lane 63 at step u, where
+// from_prev_lane / from_next_lane pick it up as their edge value.
+DEV float rotate_down(float q) {  // lane l <- lane l+1 (lane 63 <- 0)
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, q), 0x130, 0xf, 0xf, true));
+}
+DEV float rotate_up(float q) {  // lane l <- lane l-1 (lane 0 <- 0)
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, q), 0x138, 0xf, 0xf, true));
+}
+// True when a > b in any active lane.  The compare writes a scalar lane
+// mask directly, so the common (false) case costs one vector compare and a
+// scalar branch.
+DEV bool any_greater(float a, float b) {
+    uint64_t m;
+    asm volatile("v_cmp_gt_f32_e64 %0, %1, %2" : "=s"(m) : "v"(a), "v"(b));
+    return m != 0;
+}
+// One step of a queue: returns the neighbour shift of v with the queue's
+// current head as edge value and advances the queue.  The rotation is issued
+// first so the shift can overwrite the old queue register in place.
+DEV float shift_in_prev(float v, float& q) {
+    const float nq = rotate_down(q);
+    const float x = from_prev_lane(v, q);
+    q = nq;
+    return x;
+}
+DEV float shift_in_next(float v, float& q) {
+    const float nq = rotate_up(q);
+    const float x = from_next_lane(v, q);
+    q = nq;
+    return x;
+}
+
+// Table rows through a raw buffer resource: a 32-bit lane offset plus a
+// wave-uniform byte offset in a scalar register, so a step's address update
+// is one scalar add.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+DEV Rsrc make_rsrc(const void* base, int64_t bytes) {
+    const int64_t cap = 0x7ffffff0;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0,
+                                            (int)(bytes < cap ? bytes : cap), 0x00020000);
+}
+DEV float bload(Rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+DEV void bstore(float v, Rsrc r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+}
+
+// Global accesses as (wave-uniform base pointer) + (32-bit lane offset): the
+// compiler emits the scalar-base form, so per-step address updates are scalar
+// adds and cost no vector ALU.
+DEV float gload(const float* base, uint32_t byte_off) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+DEV void gstore(float* base, uint32_t byte_off, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
 DEV float lane_value(float v, int l) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
 }
@@ -172,6 +235,39 @@ DEV void store16(float* row, int c0, int len, bool ok, const float (&src)[16]) {
     }
 }
 
+// 8-float versions (two unaligned 16-byte accesses on the fast path).
+DEV void load8(float (&dst)[8], const float* row, int c0, int len, bool ok) {
+    if (ok && c0 >= 0 && c0 + 8 <= len) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            f4u v = *reinterpret_cast<const f4u*>(row + c0 + 4 * q);
+            dst[4 * q + 0] = v.x; dst[4 * q + 1] = v.y; dst[4 * q + 2] = v.z; dst[4 * q + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int c = c0 + u;
+            dst[u] = (ok && c >= 0 && c < len) ? row[c] : 0.0f;
+        }
+    }
+}
+DEV void store8(float* row, int c0, int len, bool ok, const float (&src)[8]) {
+    if (ok && c0 >= 0 && c0 + 8 <= len) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            f4u v;
+            v.x = src[4 * q + 0]; v.y = src[4 * q + 1]; v.z = src[4 * q + 2]; v.w = src[4 * q + 3];
+            *reinterpret_cast<f4u*>(row + c0 + 4 * q) = v;
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int c = c0 + u;
+            if (ok && c >= 0 && c < len) row[c] = src[u];
+        }
+    }
+}
+
 // Strip hand-off.  Progress counters live in LDS; boundary data in global
 // memory written by the producer wave before the release.
 DEV void wait_ready(int* flag, int need) {
@@ -257,7 +353,7 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
                 pd.ns = 0;
                 pd.L1 = 0;
                 pd.L2 = 0;
-                if (value) value[b] = __builtin_nanf("");
+                if (value) value[b] = __builtin_bit_cast(float, 0x7fc00000u);  // quiet NaN
             }
             desc[b] = pd;
             if (sort) {
@@ -352,77 +448,101 @@ DEV int claim_pair(const SwArgs& a, int* s_pair, int* prog, int slot) {
     if (threadIdx.x == 0) *s_pair = atomicAdd(&a.claims[slot], 1);
     for (int q = threadIdx.x; q < a.max_strips; q += blockDim.x) prog[q] = 0;
     __syncthreads();
-    int pi = *s_pair;
-    return pi < a.B ? a.order[pi] : -1;
+    // wave-uniform from here on: descriptor loads and buffer resources derived
+    // from the pair index stay in scalar registers
+    const int pi = __builtin_amdgcn_readfirstlane(*s_pair);
+    return pi < a.B ? __builtin_amdgcn_readfirstlane(a.order[pi]) : -1;
 }
 
 // ==========================================================================
 // Affine gap: forward (value; optionally the DP table for the backward)
 // ==========================================================================
 
+// Chunks in which every lane's cell is inside the pair (a full strip, past
+// the 63-step ramp and before the right edge) run without per-cell masks.
+DEV bool interior_fwd(int nrow, int k, int L2) { return nrow == kStrip && k >= 4 && kChunk * k + kChunk <= L2; }
+
 template <bool STORE>
 DEV void affine_fwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, int* prog,
                           float& zref, float& z, float& zc_comp) {
     const int r = lane;
-    const int i = s * kStrip + r + 1;
-    const bool row_ok = i <= d.L1;
+    const int nrow = min(kStrip, d.L1 - s * kStrip);
+    const bool row_ok = r < nrow;
     const int L2 = d.L2, NT = L2 + 63;
-    const float* srow = a.scores + d.s_off + (int64_t)(i - 1) * d.ld;
-    float* tab = a.table + d.t_off + (int64_t)s * NT * (kWave * kPlanesAffineFwd);
-    const float* bin = s > 0 ? a.bnd + d.b_off + (int64_t)(s - 1) * L2 * kBndStride : nullptr;
-    float* bout = s + 1 < d.ns ? a.bnd + d.b_off + (int64_t)s * L2 * kBndStride : nullptr;
+    const float* srow = a.scores + d.s_off + (int64_t)(s * kStrip + r) * d.ld;
+    constexpr int P = kWave * kPlanesAffineFwd;  // floats per step
+    float* tab = a.table + d.t_off + (int64_t)s * NT * P + r;  // this lane's column of the table
+    const bool has_in = s > 0, has_out = s + 1 < d.ns;
+    const int64_t blk = (int64_t)L2 * kBndStride;
+    const float* bin = a.bnd + d.b_off + (int64_t)(has_in ? s - 1 : 0) * blk;
+    float* bout = a.bnd + d.b_off + (int64_t)s * blk;
+    const uint32_t vt = (uint32_t)r * 4;
     const float kap = a.kappa, go2 = a.go2, ge2 = a.ge2;
 
     float Ml = kNinf, Dl = kNinf;             // M, D of (i, j-1)
     float Mp = kNinf, Ip = kNinf, Ap = kNinf;  // own values of the previous step
     float Adg = kNinf;                         // A of (i-1, j-1)
+    float zlim = zref + 64.0f;
     float cur[16], nxt[16];
     load16(cur, srow, -r, L2, row_ok);
     const int nchunk = (NT + kChunk - 1) / kChunk;
     for (int k = 0; k < nchunk; ++k) {
         load16(nxt, srow, kChunk * (k + 1) - r, L2, row_ok);
-        float eM = kNinf, eI = kNinf, eA = kNinf;
-        if (bin) {
+        // boundary row above: entry u (column 16k + u + 1) in lane u
+        float qM = kNinf, qI = kNinf, qA = kNinf;
+        if (has_in) {
             const int need = chunk_need(k, L2);
             if (need > 0) wait_ready(&prog[s - 1], need);
-            const int j = kChunk * k + lane + 1;
-            if (lane < kChunk && j <= L2) {
-                const float* e = bin + (int64_t)(j - 1) * kBndStride;
-                eM = e[0]; eI = e[1]; eA = e[2];
+            if (lane < kChunk && kChunk * k + lane < L2) {
+                const uint32_t off = (uint32_t)(kChunk * k + lane) * (kBndStride * 4);
+                qM = gload(bin, off); qI = gload(bin, off + 4); qA = gload(bin, off + 8);
             }
         }
-        float zc = 0.0f;
+        auto body = [&](auto edge) {
+            constexpr bool EDGE = decltype(edge)::value;
+            float* tk = tab + (int64_t)kChunk * k * P;  // step 16k; steps u add constant offsets
+            float zc = 0.0f;
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-            const int t = kChunk * k + u;
-            const int j = t - r + 1;
-            const bool ok = row_ok && j >= 1 && j <= L2;
-            const float Mu = from_prev_lane(Mp, lane_value(eM, u));
-            const float Iu = from_prev_lane(Ip, lane_value(eI, u));
-            const float Au = from_prev_lane(Ap, lane_value(eA, u));
-            float M = cur[u] * kap + softplus2(Adg);
-            float I = lse2(Mu + go2, Iu + ge2);
-            float D = lse2(Ml + go2, Dl + ge2);
-            float A = lse3(M, I, D);
-            if (!ok) { M = kNinf; I = kNinf; D = kNinf; A = kNinf; }
-            if (ok && A > zref + 64.0f) {
-                const float f = ex2(zref - A);
-                z *= f; zc *= f; zc_comp *= f;
-                zref = A;
+            for (int u = 0; u < kChunk; ++u) {
+                const int t = kChunk * k + u;
+                const int j = t - r + 1;
+                const bool ok = !EDGE || (row_ok && j >= 1 && j <= L2);
+                const float Mu = shift_in_prev(Mp, qM);
+                const float Iu = shift_in_prev(Ip, qI);
+                const float Au = shift_in_prev(Ap, qA);
+                float M = cur[u] * kap + softplus2(Adg);
+                float I = lse2(Mu + go2, Iu + ge2);
+                float D = lse2(Ml + go2, Dl + ge2);
+                float A = lse3(M, I, D);
+                if (EDGE && !ok) { M = kNinf; I = kNinf; D = kNinf; A = kNinf; }
+                // partition: rescale when A outgrows the lane's reference by
+                // 2^64 (rare, so the wave branches around it on a scalar mask)
+                if (any_greater(A, zlim)) {
+                    if (A > zlim && ok) {
+                        const float f = ex2(zref - A);
+                        z *= f; zc *= f; zc_comp *= f;
+                        zref = A;
+                        zlim = A + 64.0f;
+                    }
+                }
+                const float e = ex2(A - zref);
+                zc += (!EDGE || ok) ? e : 0.0f;
+                if (STORE && (!EDGE || t < NT)) {
+                    tk[u * P] = M;
+                    tk[u * P + kWave] = I;
+                    tk[u * P + 2 * kWave] = D;
+                }
+                if (has_out && r == kWave - 1 && ok) {
+                    float* e = bout + (int64_t)(t - 63) * kBndStride;  // column j - 1 of lane 63
+                    e[0] = M; e[1] = I; e[2] = A;
+                }
+                Adg = Au; Ml = M; Dl = D; Mp = M; Ip = I; Ap = A;
             }
-            zc += ok ? ex2(A - zref) : 0.0f;
-            if (STORE && t < NT) {
-                float* q = tab + (int64_t)t * (kWave * kPlanesAffineFwd) + r;
-                q[0] = M; q[kWave] = I; q[2 * kWave] = D;
-            }
-            if (bout && r == kWave - 1 && ok) {
-                float* e = bout + (int64_t)(j - 1) * kBndStride;
-                e[0] = M; e[1] = I; e[2] = A;
-            }
-            Adg = Au; Ml = M; Dl = D; Mp = M; Ip = I; Ap = A;
-        }
-        kadd(z, zc_comp, zc);
-        if (bout) publish(&prog[s], chunk_done(k, L2), lane);
+            kadd(z, zc_comp, zc);
+        };
+        if (interior_fwd(nrow, k, L2)) body(std::false_type{});
+        else body(std::true_type{});
+        if (has_out) publish(&prog[s], chunk_done(k, L2), lane);
 #pragma unroll
         for (int u = 0; u < 16; ++u) cur[u] = nxt[u];
     }
@@ -451,7 +571,7 @@ __global__ __launch_bounds__(W * 64) void sw_affine_forward_kernel(SwArgs a) {
     extern __shared__ int s_prog[];
     __shared__ int s_pair;
     __shared__ float s_red[2 * W];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (;;) {
         const int p = claim_pair(a, &s_pair, s_prog, 0);
         if (p < 0) break;
@@ -472,82 +592,121 @@ __global__ __launch_bounds__(W * 64) void sw_affine_forward_kernel(SwArgs a) {
 // Strips run bottom-up, columns right to left.
 // ==========================================================================
 
+DEV bool interior_bwd(int nrow, int k, int L2) { return nrow == kStrip && k >= 4 && kChunk * k + kChunk <= L2; }
+
+// Table values are prefetched kPF steps ahead into a small register ring.
+constexpr int kPF = 4;
+
 DEV void affine_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, int* prog, float St,
                           float (&acc)[3], float (&comp)[3]) {
     const int r = lane;
-    const int i = s * kStrip + r + 1;
-    const bool row_ok = i <= d.L1;
+    const int nrow = min(kStrip, d.L1 - s * kStrip);
+    const bool row_ok = r < nrow;
     const int L2 = d.L2, NT = L2 + 63;
-    const float* srow = a.scores + d.s_off + (int64_t)(i - 1) * d.ld;
-    float* mrow = a.map ? a.map + d.s_off + (int64_t)(i - 1) * d.ld : nullptr;
-    const float* tab = a.table + d.t_off + (int64_t)s * NT * (kWave * kPlanesAffineFwd);
-    const float* bin = s + 1 < d.ns ? a.bnd + d.b_off + (int64_t)(s + 1) * L2 * kBndStride : nullptr;
-    float* bout = s > 0 ? a.bnd + d.b_off + (int64_t)s * L2 * kBndStride : nullptr;
+    const float* srow = a.scores + d.s_off + (int64_t)(s * kStrip + r) * d.ld;
+    float* mrow = a.map ? a.map + d.s_off + (int64_t)(s * kStrip + r) * d.ld : nullptr;
+    constexpr int P = kWave * kPlanesAffineFwd;
+    const float* tab = a.table + d.t_off + (int64_t)s * NT * P + r;  // this lane's column
+    const bool has_in = s + 1 < d.ns, has_out = s > 0;
+    const int64_t blk = (int64_t)L2 * kBndStride;
+    const float* bin = a.bnd + d.b_off + (int64_t)(has_in ? s + 1 : s) * blk;
+    float* bout = a.bnd + d.b_off + (int64_t)s * blk;
+    const uint32_t vt = (uint32_t)r * 4;
     const float go2 = a.go2, ge2 = a.ge2;
 
     float bDs = 0.0f, Ds = kNinf;              // beta_D, D of (i, j+1)
     float bIp = 0.0f, Ip = kNinf, bMp = 0.0f;  // own values of the previous step
     float bMdg = 0.0f;                         // beta_M of (i+1, j+1)
-    float cur[16], nxt[16];
-    load16(cur, srow, NT - kChunk - r, L2, row_ok);
+    // table ring: slot u % kPF holds step NT - 1 - (16k + u)
+    float rM[kPF], rI[kPF], rD[kPF];
+#pragma unroll
+    for (int q = 0; q < kPF; ++q) {
+        const float* qq = tab + (int64_t)max(NT - 1 - q, 0) * P;
+        rM[q] = qq[0]; rI[q] = qq[kWave]; rD[q] = qq[2 * kWave];
+    }
     const int nchunk = (NT + kChunk - 1) / kChunk;
     for (int k = 0; k < nchunk; ++k) {
-        load16(nxt, srow, NT - kChunk * (k + 2) - r, L2, row_ok);
-        float eB = 0.0f, eI = kNinf, eM = 0.0f;
-        if (bin) {
+        // boundary row below (written by its lane 0): entry u (column
+        // L2 - 16k - u) in lane 63 - u
+        float qB = 0.0f, qI = kNinf, qM = 0.0f;
+        if (has_in) {
             const int need = chunk_need(k, L2);
             if (need > 0) wait_ready(&prog[s + 1], need);
-            const int j = L2 - (kChunk * k + lane);
-            if (lane < kChunk && j >= 1) {
-                const float* e = bin + (int64_t)(j - 1) * kBndStride;
-                eB = e[0]; eI = e[1]; eM = e[2];
+            const int u = kWave - 1 - lane;
+            if (u < kChunk && kChunk * k + u < L2) {
+                const uint32_t off = (uint32_t)(L2 - 1 - kChunk * k - u) * (kBndStride * 4);
+                qB = gload(bin, off); qI = gload(bin, off + 4); qM = gload(bin, off + 8);
             }
         }
-        float pc[16];
         float cacc[3] = {0.0f, 0.0f, 0.0f};
+        auto body = [&](auto edge) {
+            constexpr bool EDGE = decltype(edge)::value;
+            // prefetch target of step u is step t - kPF = NT - 1 - kPF - 16k - u:
+            // a per-chunk lane pointer plus constant offsets (clamped at 0 in
+            // the last chunks, where the values are never used)
+            const bool pf_tail = EDGE && (NT - 1 - kPF - kChunk * k - (kChunk - 1) < 0);
+            const float* tk = tab + (int64_t)(NT - 1 - kPF - kChunk * k) * P;
+            // two halves of 8 steps: the half's scores (needed only for
+            // sum(P * s) at its end) and its map values stay in registers
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-            const int t = NT - 1 - (kChunk * k + u);
-            const int j = t - r + 1;
-            const bool ok = row_ok && j >= 1 && j <= L2;
-            const int tt = max(t, 0);
-            const float* q = tab + (int64_t)tt * (kWave * kPlanesAffineFwd) + r;
-            const float M = q[0], I = q[kWave], D = q[2 * kWave];
-            const float bIs = from_next_lane(bIp, lane_value(eB, u));
-            const float Is = from_next_lane(Ip, lane_value(eI, u));
-            const float bMs = from_next_lane(bMp, lane_value(eM, u));
-            float A, wM, wI, wD;
-            lse3_norm(M, I, D, A, wM, wI, wD);
-            const float bA = ex2(fminf(A - St, 0.0f)) + bMdg * sigmoid2(A);
-            float wIM, wII, wDM, wDD;
-            pair_weights(M + go2 - Is, I + ge2 - Is, wIM, wII);
-            pair_weights(M + go2 - Ds, D + ge2 - Ds, wDM, wDD);
-            float tIM = bIs * wIM, tII = bIs * wII, tDM = bDs * wDM, tDD = bDs * wDD;
-            float bM = bA * wM + tIM + tDM;
-            float bI = bA * wI + tII;
-            float bD = bA * wD + tDD;
-            if (!ok) { bM = 0.0f; bI = 0.0f; bD = 0.0f; tIM = tII = tDM = tDD = 0.0f; }
-            pc[u] = bM;
-            cacc[0] += tIM + tDM;
-            cacc[1] += tII + tDD;
-            cacc[2] += bM * cur[kChunk - 1 - u];
-            if (bout && r == 0 && ok) {
-                float* e = bout + (int64_t)(j - 1) * kBndStride;
-                e[0] = bI; e[1] = I; e[2] = bM;
+            for (int h = 0; h < 2; ++h) {
+                float sc[8], pc[8];
+                // step u = 8h + v is column NT - 16k - u - r - 1
+                const int c0 = NT - kChunk * k - 8 * h - 8 - r;
+                load8(sc, srow, c0, L2, row_ok);
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const int u = 8 * h + v;
+                    const int t = NT - 1 - (kChunk * k + u);
+                    const int j = t - r + 1;
+                    const bool ok = !EDGE || (row_ok && j >= 1 && j <= L2);
+                    const float M = rM[u % kPF], I = rI[u % kPF], D = rD[u % kPF];
+                    {
+                        // unconditional (clamped) so the ring slot is simply
+                        // overwritten and only waited for kPF steps later
+                        const float* qq = pf_tail ? tab + (int64_t)max(t - kPF, 0) * P : tk - u * P;
+                        rM[u % kPF] = qq[0];
+                        rI[u % kPF] = qq[kWave];
+                        rD[u % kPF] = qq[2 * kWave];
+                    }
+                    const float bIs = shift_in_next(bIp, qB);
+                    const float Is = shift_in_next(Ip, qI);
+                    const float bMs = shift_in_next(bMp, qM);
+                    float A, wM, wI, wD;
+                    lse3_norm(M, I, D, A, wM, wI, wD);
+                    const float bA = ex2(fminf(A - St, 0.0f)) + bMdg * sigmoid2(A);
+                    const float mg = M + go2;
+                    float wIM, wII, wDM, wDD;
+                    pair_weights(mg - Is, I + ge2 - Is, wIM, wII);
+                    pair_weights(mg - Ds, D + ge2 - Ds, wDM, wDD);
+                    float tIM = bIs * wIM, tII = bIs * wII, tDM = bDs * wDM, tDD = bDs * wDD;
+                    float tMo = tIM + tDM, tEx = tII + tDD;
+                    float bM = bA * wM + tMo;
+                    float bI = bA * wI + tII;
+                    float bD = bA * wD + tDD;
+                    if (EDGE && !ok) { bM = 0.0f; bI = 0.0f; bD = 0.0f; tMo = 0.0f; tEx = 0.0f; }
+                    pc[7 - v] = bM;
+                    cacc[0] += tMo;
+                    cacc[1] += tEx;
+                    // keep the running sums in step order (the scheduler would
+                    // otherwise defer the adds and hold every summand live)
+                    asm volatile("" : "+v"(cacc[0]), "+v"(cacc[1]));
+                    if (has_out && r == 0 && ok) {
+                        float* e = bout + (int64_t)t * kBndStride;  // column j - 1 of lane 0
+                        e[0] = bI; e[1] = I; e[2] = bM;
+                    }
+                    bMdg = bMs; bDs = bD; Ds = D; bIp = bI; Ip = I; bMp = bM;
+                }
+#pragma unroll
+                for (int v = 0; v < 8; ++v) cacc[2] += pc[v] * sc[v];
+                if (mrow) store8(mrow, c0, L2, row_ok, pc);
             }
-            bMdg = bMs; bDs = bD; Ds = D; bIp = bI; Ip = I; bMp = bM;
-        }
+        };
+        if (interior_bwd(nrow, k, L2)) body(std::false_type{});
+        else body(std::true_type{});
 #pragma unroll
         for (int c = 0; c < 3; ++c) kadd(acc[c], comp[c], cacc[c]);
-        if (mrow) {
-            float outv[16];
-#pragma unroll
-            for (int v = 0; v < 16; ++v) outv[v] = pc[kChunk - 1 - v];
-            store16(mrow, NT - kChunk * (k + 1) - r, L2, row_ok, outv);
-        }
-        if (bout) publish(&prog[s], chunk_done(k, L2), lane);
-#pragma unroll
-        for (int u = 0; u < 16; ++u) cur[u] = nxt[u];
+        if (has_out) publish(&prog[s], chunk_done(k, L2), lane);
     }
 }
 
@@ -573,7 +732,7 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
     extern __shared__ int s_prog[];
     __shared__ int s_pair;
     __shared__ float s_red[3 * W];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (;;) {
         const int p = claim_pair(a, &s_pair, s_prog, 1);
         if (p < 0) break;
@@ -661,7 +820,7 @@ __global__ __launch_bounds__(W * 64) void sw_forward_kernel(SwArgs a) {
     extern __shared__ int s_prog[];
     __shared__ int s_pair;
     __shared__ float s_red[2 * W];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (;;) {
         const int p = claim_pair(a, &s_pair, s_prog, 0);
         if (p < 0) break;
@@ -771,7 +930,7 @@ __global__ __launch_bounds__(W * 64) void sw_backward_kernel(SwArgs a) {
     extern __shared__ int s_prog[];
     __shared__ int s_pair;
     __shared__ float s_red[2 * W];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (;;) {
         const int p = claim_pair(a, &s_pair, s_prog, 1);
         if (p < 0) break;
