@@ -270,6 +270,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-graph", action="store_true", help="time eager calls instead of a graph")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 1 GiB device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -289,6 +291,14 @@ def main():
         sys.exit("bench.py needs a GPU")
     dev = torch.device("cuda:0")
     log(f"device {torch.cuda.get_device_name(0)}")
+
+    if args.copy_probe:
+        src = torch.ones(1 << 28, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy probe: 1 GiB read + 1 GiB written")
 
     if args.workload == "ragged":
         params = (-8.0, -0.5, 0.5)

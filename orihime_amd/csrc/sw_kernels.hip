@@ -528,9 +528,11 @@ DEV void affine_fwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
                 const float e = ex2(A - zref);
                 zc += (!EDGE || ok) ? e : 0.0f;
                 if (STORE && (!EDGE || t < NT)) {
-                    tk[u * P] = M;
-                    tk[u * P + kWave] = I;
-                    tk[u * P + 2 * kWave] = D;
+                    // streaming stores: the table is read back only by the
+                    // reverse sweep, so it should not displace score lines in L2
+                    __builtin_nontemporal_store(M, &tk[u * P]);
+                    __builtin_nontemporal_store(I, &tk[u * P + kWave]);
+                    __builtin_nontemporal_store(D, &tk[u * P + 2 * kWave]);
                 }
                 if (has_out && r == kWave - 1 && ok) {
                     float* e = bout + (int64_t)(t - 63) * kBndStride;  // column j - 1 of lane 63
