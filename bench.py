@@ -265,7 +265,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("ragged", "fixed"), default="ragged")
+    ap.add_argument("--workload", choices=("ragged", "fixed", "long"), default="ragged")
     ap.add_argument("--no-padded", action="store_true", help="skip the padded comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -313,13 +313,24 @@ def main():
         workload = (f"ragged affine soft-SW fwd+bwd, 2048 pairs, L1,L2 ~ U[64,1024] (seed 2), "
                     f"gap_open=-8 gap_extend=-0.5 T=0.5, packed persistent launch")
         cpu_pairs = [p.numpy() for p in pairs]
-    else:
+    elif args.workload == "fixed":
         params = (-11.0, -1.0, 1.0)
         x = fixed_batch(torch)
         cells = x.numel()
         main_launch = Launch(torch, _lib, x.to(dev), None, None, 512, 512, params)
         workload = "fixed-shape affine soft-SW fwd+bwd, [256, 512, 512] (seed 1), go=-11 ge=-1 T=1"
         cpu_pairs = [x[i].numpy() for i in range(x.shape[0])]
+    else:
+        params = (-11.0, -1.0, 1.0)
+        gen = torch.Generator().manual_seed(3)
+        x = torch.randn(16, 4096, 4096, generator=gen)
+        V = torch.randn(16, 4096, 4096, generator=gen)
+        cells = x.numel()
+        torch.cuda.reset_peak_memory_stats()
+        main_launch = Launch(torch, _lib, x.to(dev), None, None, 4096, 4096, params)
+        workload = ("long pairs affine soft-SW fwd+bwd, [16, 4096, 4096] N(0,1) (seed 3), "
+                    "go=-11 ge=-1 T=1, checkpointed reverse sweep")
+        cpu_pairs = [x[0].numpy()]
 
     # warm-up, sanity, graph
     for _ in range(max(1, args.warmup)):
@@ -403,6 +414,31 @@ def main():
                       "ragged_over_padded": round(gcups / pg, 3),
                       "padded_vs_packed_value_rel_diff": vdiff})
         del padded, pad
+        torch.cuda.empty_cache()
+
+    if args.workload == "long":
+        # peak HBM of the step against a reverse sweep that reads a full table
+        peak = torch.cuda.max_memory_allocated()
+        strips = 4096 // 64
+        full_table = 16 * strips * (4096 + 63) * 64 * 3 * 4  # M, I, D per cell slot
+        ckpt_ws = main_launch.ws.numel()
+        bnd = 16 * strips * 4096 * 8 * 4
+        extra.update({"peak_hbm_bytes": int(peak), "workspace_bytes": int(ckpt_ws),
+                      "full_table_workspace_bytes": int(full_table + bnd),
+                      "full_table_peak_bytes": int(peak - ckpt_ws + full_table + bnd)})
+        # Hessian-vector product along V
+        Vd = V.to(dev)
+        mt = torch.empty_like(main_launch.scores)
+        tws = torch.empty(_lib.workspace_bytes(True, _lib.OP_TANGENT, 16, 4096, 4096, -1),
+                          dtype=torch.uint8, device=dev)
+        hvp = lambda: _lib.affine_tangent(main_launch.scores, None, None, *params, direction=Vd,
+                                          map_tangent=mt, workspace=tws)
+        hvp()
+        torch.cuda.synchronize()
+        hms, _ = timed(torch, hvp, max(2, args.steps // 4))
+        log(f"HVP: {hms:.3f} ms, {cells / (hms * 1e-3) / 1e9:.1f} GCUPS")
+        extra.update({"hvp_ms": round(hms, 4), "hvp_gcups": round(cells / (hms * 1e-3) / 1e9, 3)})
+        del Vd, mt, tws
         torch.cuda.empty_cache()
 
     cpu = None

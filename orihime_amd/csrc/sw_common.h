@@ -38,7 +38,8 @@ constexpr float kLn2 = 0.6931471805599453f;
 constexpr int kWave = 64;                   // gfx950 wavefront
 constexpr int kStrip = 64;                  // rows per strip = lanes per wave
 constexpr int kChunk = 16;                  // columns per unrolled chunk
-constexpr int kBndStride = 8;               // floats per boundary entry (one column)
+constexpr int kBndStride = 8;               // floats per boundary entry (one column):
+                                            // [0..3] forward sweep, [4..7] reverse sweep
 
 // Per-pair work descriptor built on the device by the plan kernel.
 struct PairDesc {
@@ -52,17 +53,26 @@ struct PairDesc {
 
 // Floats per table step (one column of a strip) for each pass: the table of
 // a strip is [L2 + 63 steps][planes][64 lanes].
-constexpr int kPlanesAffineFwd = 3;   // M, I, D
-constexpr int kPlanesAffineTan = 6;   // M, I, D and their tangents
-constexpr int kPlanesLinearFwd = 2;   // A, X
-constexpr int kPlanesLinearTan = 4;   // A, X and their tangents
+// What the forward sweep keeps for the reverse sweep, per strip and per
+// stored step: `planes` rows of 64 lanes every `stride` anti-diagonals.
+//   affine: the sweep state every kCkpt steps (checkpoints; the reverse sweep
+//           recomputes the steps in between),
+//   linear and the tangent passes: every step (full table).
+constexpr int kCkpt = 8;                // steps between affine checkpoints
+constexpr int kPlanesAffineCkpt = 5;    // M, I, D, A of the previous step; A of the diagonal
+constexpr int kPlanesAffineTan = 6;     // M, I, D and their tangents
+constexpr int kPlanesLinearFwd = 2;     // A, X
+constexpr int kPlanesLinearTan = 4;     // A, X and their tangents
 
-// Upper bounds (in elements) used to size the workspace without reading the
-// device lengths: with N = sum L1*L2 over pairs,
-//   sum_p ns_p*(L2_p+63)*64 <= sum (L1+63)(L2+63) <= N + 63 B (L1max+L2max) + 3969 B
-//   sum_p ns_p*L2_p         <= N/64 + B L2max
-inline int64_t table_bound(int64_t N, int64_t B, int64_t L1max, int64_t L2max, int planes) {
-    return (int64_t)planes * (N + 63 * B * (L1max + L2max) + 3969 * B);
+struct TableSpec {
+    int planes, stride;
+};
+
+// Upper bound of sum_b ns_b * ceil((L2_b + 63) / stride) * 64 * planes over a
+// batch with N cells in total (ns * 64 <= L1 + 63).
+inline int64_t table_bound(int64_t N, int64_t B, int64_t L1max, int64_t L2max, TableSpec t) {
+    const int64_t full = N + 63 * B * (L1max + L2max) + 3969 * B;  // sum (L1+63)(L2+63)
+    return (int64_t)t.planes * ((full + t.stride - 1) / t.stride + B * (L1max + 63));
 }
 inline int64_t bnd_bound(int64_t N, int64_t B, int64_t L2max) {
     return (int64_t)kBndStride * (N / 64 + 1 + B * L2max);
@@ -90,13 +100,14 @@ inline size_t ws_fixed(int64_t B, size_t* claims, size_t* desc, size_t* order, s
 
 // op: 0 value only, 1 forward + backward, 2 tangent passes.
 inline WsLayout ws_layout(int op, int64_t B, int64_t L1max, int64_t L2max, int64_t N,
-                          int planes_fwd, int planes_tan) {
+                          TableSpec fwd, TableSpec tan) {
     if (N < 0) N = B * L1max * L2max;
     WsLayout w;
     size_t o = ws_fixed(B, &w.claims, &w.desc, &w.order, &w.stil, &w.tgt);
     w.data = o;
-    int planes = op == 0 ? 0 : (op == 1 ? planes_fwd : planes_tan);
-    int64_t elems = table_bound(N, B, L1max, L2max, planes) + bnd_bound(N, B, L2max);
+    const TableSpec none = {0, 1};
+    const TableSpec t = op == 0 ? none : (op == 1 ? fwd : tan);
+    int64_t elems = table_bound(N, B, L1max, L2max, t) + bnd_bound(N, B, L2max);
     w.total = align256(o + (size_t)elems * sizeof(float));
     return w;
 }

@@ -304,7 +304,7 @@ constexpr int kSortMax = 2048;
 
 __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
     const int32_t* __restrict__ lengths, const int64_t* __restrict__ offsets, int B, int L1max,
-    int L2max, int planes, int64_t cap, PairDesc* __restrict__ desc,
+    int L2max, int planes, int tstride, int64_t cap, PairDesc* __restrict__ desc,
     int32_t* __restrict__ order, int* __restrict__ claims, float* __restrict__ value) {
     __shared__ int64_t s_t[kPlanThreads];
     __shared__ int64_t s_carry[1];
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
             L1 = min(max(L1, 0), L1max);
             L2 = min(max(L2, 0), L2max);
             ns = (L1 > 0 && L2 > 0) ? (L1 + kStrip - 1) / kStrip : 0;
-            ts = (int64_t)ns * (L2 + 63) * kWave * planes;
+            ts = (int64_t)ns * ((L2 + 63 + tstride - 1) / tstride) * kWave * planes;
             bs = (int64_t)ns * L2 * kBndStride;
         }
         s_t[tid] = ts + bs;
@@ -470,8 +470,10 @@ DEV void affine_fwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
     const bool row_ok = r < nrow;
     const int L2 = d.L2, NT = L2 + 63;
     const float* srow = a.scores + d.s_off + (int64_t)(s * kStrip + r) * d.ld;
-    constexpr int P = kWave * kPlanesAffineFwd;  // floats per step
-    float* tab = a.table + d.t_off + (int64_t)s * NT * P + r;  // this lane's column of the table
+    // checkpoints: the sweep state before every kCkpt-th step
+    constexpr int CP = kWave * kPlanesAffineCkpt;
+    const int nblk = (NT + kCkpt - 1) / kCkpt;
+    float* ck = a.table + d.t_off + (int64_t)s * nblk * CP + r;  // this lane's column
     const bool has_in = s > 0, has_out = s + 1 < d.ns;
     const int64_t blk = (int64_t)L2 * kBndStride;
     const float* bin = a.bnd + d.b_off + (int64_t)(has_in ? s - 1 : 0) * blk;
@@ -500,13 +502,22 @@ DEV void affine_fwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
         }
         auto body = [&](auto edge) {
             constexpr bool EDGE = decltype(edge)::value;
-            float* tk = tab + (int64_t)kChunk * k * P;  // step 16k; steps u add constant offsets
+            float* ck_k = ck + (int64_t)(kChunk / kCkpt) * k * CP;  // block 2k
             float zc = 0.0f;
 #pragma unroll
             for (int u = 0; u < kChunk; ++u) {
                 const int t = kChunk * k + u;
                 const int j = t - r + 1;
                 const bool ok = !EDGE || (row_ok && j >= 1 && j <= L2);
+                if (STORE && u % kCkpt == 0 && (!EDGE || t < NT)) {
+                    // streaming stores: read back only by the reverse sweep
+                    float* q = ck_k + (u / kCkpt) * CP;
+                    __builtin_nontemporal_store(Mp, q);
+                    __builtin_nontemporal_store(Ip, q + kWave);
+                    __builtin_nontemporal_store(Dl, q + 2 * kWave);
+                    __builtin_nontemporal_store(Ap, q + 3 * kWave);
+                    __builtin_nontemporal_store(Adg, q + 4 * kWave);
+                }
                 const float Mu = shift_in_prev(Mp, qM);
                 const float Iu = shift_in_prev(Ip, qI);
                 const float Au = shift_in_prev(Ap, qA);
@@ -527,13 +538,6 @@ DEV void affine_fwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
                 }
                 const float e = ex2(A - zref);
                 zc += (!EDGE || ok) ? e : 0.0f;
-                if (STORE && (!EDGE || t < NT)) {
-                    // streaming stores: the table is read back only by the
-                    // reverse sweep, so it should not displace score lines in L2
-                    __builtin_nontemporal_store(M, &tk[u * P]);
-                    __builtin_nontemporal_store(I, &tk[u * P + kWave]);
-                    __builtin_nontemporal_store(D, &tk[u * P + 2 * kWave]);
-                }
                 if (has_out && r == kWave - 1 && ok) {
                     float* e = bout + (int64_t)(t - 63) * kBndStride;  // column j - 1 of lane 63
                     e[0] = M; e[1] = I; e[2] = A;
@@ -594,10 +598,34 @@ __global__ __launch_bounds__(W * 64) void sw_affine_forward_kernel(SwArgs a) {
 // Strips run bottom-up, columns right to left.
 // ==========================================================================
 
-DEV bool interior_bwd(int nrow, int k, int L2) { return nrow == kStrip && k >= 4 && kChunk * k + kChunk <= L2; }
+// ==========================================================================
+// Reverse sweep with recomputation.  The forward sweep kept its state every
+// kCkpt = 8 steps; the reverse sweep walks the strip's 8-step blocks from the
+// last to the first and for each block
+//   1. recomputes the block's M, I, D (registers) from its checkpoint, its
+//      scores and the row above (the strip above's forward boundary),
+//   2. runs the pull-form adjoint over the block's steps in reverse order.
+// The block's scores serve both the recomputation and sum(P * s).  A block's
+// inputs are loaded when the block starts; the other waves on the SIMD cover
+// the latency (software prefetch one block ahead cost ~55 VGPRs, halving
+// occupancy).  Strip hand-off (row below -> this strip's lane 63) works in forward
+// 16-step chunks processed in reverse: after chunk k a strip has finished
+// every column >= 16k + 1 of its first row.
+// ==========================================================================
 
-// Table values are prefetched kPF steps ahead into a small register ring.
-constexpr int kPF = 4;
+DEV int rev_need(int k, int L2) {
+    // columns read by lane 63 in chunk k: [16k - 62, 16k - 47] within [1, L2]
+    const int lo = max(1, kChunk * k - 62), hi = min(L2, kChunk * k - 47);
+    return lo > hi ? 0 : L2 - lo + 1;
+}
+DEV int rev_done(int k, int L2) { return max(0, min(L2, L2 - kChunk * k)); }
+
+// Inputs of one 8-step block.
+struct BlockIn {
+    float c[kPlanesAffineCkpt];  // checkpoint: M, I, D, A of the previous step, A of the diagonal
+    float sc[kCkpt];             // scores of the block's columns (sc[v]: step 8b + v)
+    float qM, qI, qA;            // row above, entry v (column 8b + v + 1) in lane v
+};
 
 DEV void affine_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, int* prog, float St,
                           float (&acc)[3], float (&comp)[3]) {
@@ -605,75 +633,95 @@ DEV void affine_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
     const int nrow = min(kStrip, d.L1 - s * kStrip);
     const bool row_ok = r < nrow;
     const int L2 = d.L2, NT = L2 + 63;
+    const int nblk = (NT + kCkpt - 1) / kCkpt;
+    const int nchunk = (NT + kChunk - 1) / kChunk;
     const float* srow = a.scores + d.s_off + (int64_t)(s * kStrip + r) * d.ld;
     float* mrow = a.map ? a.map + d.s_off + (int64_t)(s * kStrip + r) * d.ld : nullptr;
-    constexpr int P = kWave * kPlanesAffineFwd;
-    const float* tab = a.table + d.t_off + (int64_t)s * NT * P + r;  // this lane's column
-    const bool has_in = s + 1 < d.ns, has_out = s > 0;
+    constexpr int CP = kWave * kPlanesAffineCkpt;
+    const float* ck = a.table + d.t_off + (int64_t)s * nblk * CP + r;
+    const bool has_up = s > 0, has_in = s + 1 < d.ns, has_out = s > 0;
     const int64_t blk = (int64_t)L2 * kBndStride;
-    const float* bin = a.bnd + d.b_off + (int64_t)(has_in ? s + 1 : s) * blk;
+    const float* bup = a.bnd + d.b_off + (int64_t)(has_up ? s - 1 : 0) * blk;  // [0..2]: M, I, A
+    const float* bin = a.bnd + d.b_off + (int64_t)(has_in ? s + 1 : s) * blk;  // [4..6]: bI, I, bM
     float* bout = a.bnd + d.b_off + (int64_t)s * blk;
-    const uint32_t vt = (uint32_t)r * 4;
-    const float go2 = a.go2, ge2 = a.ge2;
+    const float kap = a.kappa, go2 = a.go2, ge2 = a.ge2;
+
+    auto load_block = [&](BlockIn& in, int b) {
+        const int t0 = kCkpt * b;
+        if (t0 < NT) {
+            const float* q = ck + (int64_t)b * CP;
+#pragma unroll
+            for (int p = 0; p < kPlanesAffineCkpt; ++p) in.c[p] = q[p * kWave];
+        } else {
+#pragma unroll
+            for (int p = 0; p < kPlanesAffineCkpt; ++p) in.c[p] = kNinf;
+        }
+        load8(in.sc, srow, t0 - r, L2, row_ok);
+        in.qM = kNinf; in.qI = kNinf; in.qA = kNinf;
+        if (has_up && lane < kCkpt && t0 + lane < L2) {
+            const float* e = bup + (int64_t)(t0 + lane) * kBndStride;
+            in.qM = e[0]; in.qI = e[1]; in.qA = e[2];
+        }
+    };
 
     float bDs = 0.0f, Ds = kNinf;              // beta_D, D of (i, j+1)
-    float bIp = 0.0f, Ip = kNinf, bMp = 0.0f;  // own values of the previous step
+    float bIp = 0.0f, Ip = kNinf, bMp = 0.0f;  // own values of the previous (later) step
     float bMdg = 0.0f;                         // beta_M of (i+1, j+1)
-    // table ring: slot u % kPF holds step NT - 1 - (16k + u)
-    float rM[kPF], rI[kPF], rD[kPF];
-#pragma unroll
-    for (int q = 0; q < kPF; ++q) {
-        const float* qq = tab + (int64_t)max(NT - 1 - q, 0) * P;
-        rM[q] = qq[0]; rI[q] = qq[kWave]; rD[q] = qq[2 * kWave];
-    }
-    const int nchunk = (NT + kChunk - 1) / kChunk;
-    for (int k = 0; k < nchunk; ++k) {
-        // boundary row below (written by its lane 0): entry u (column
-        // L2 - 16k - u) in lane 63 - u
-        float qB = 0.0f, qI = kNinf, qM = 0.0f;
+    for (int k = nchunk - 1; k >= 0; --k) {
+        // row below (its lane 0's reverse boundary): entry e (column
+        // 16k - 47 - e, processing order) in lane 63 - e
+        float qB = 0.0f, qIs = kNinf, qBM = 0.0f;
         if (has_in) {
-            const int need = chunk_need(k, L2);
+            const int need = rev_need(k, L2);
             if (need > 0) wait_ready(&prog[s + 1], need);
-            const int u = kWave - 1 - lane;
-            if (u < kChunk && kChunk * k + u < L2) {
-                const uint32_t off = (uint32_t)(L2 - 1 - kChunk * k - u) * (kBndStride * 4);
-                qB = gload(bin, off); qI = gload(bin, off + 4); qM = gload(bin, off + 8);
+            const int e = kWave - 1 - lane;
+            const int j = kChunk * k - 47 - e;
+            if (e < kChunk && j >= 1 && j <= L2) {
+                const float* q = bin + (int64_t)(j - 1) * kBndStride;
+                qB = q[4]; qIs = q[5]; qBM = q[6];
             }
         }
         float cacc[3] = {0.0f, 0.0f, 0.0f};
         auto body = [&](auto edge) {
             constexpr bool EDGE = decltype(edge)::value;
-            // prefetch target of step u is step t - kPF = NT - 1 - kPF - 16k - u:
-            // a per-chunk lane pointer plus constant offsets (clamped at 0 in
-            // the last chunks, where the values are never used)
-            const bool pf_tail = EDGE && (NT - 1 - kPF - kChunk * k - (kChunk - 1) < 0);
-            const float* tk = tab + (int64_t)(NT - 1 - kPF - kChunk * k) * P;
-            // two halves of 8 steps: the half's scores (needed only for
-            // sum(P * s) at its end) and its map values stay in registers
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                float sc[8], pc[8];
-                // step u = 8h + v is column NT - 16k - u - r - 1
-                const int c0 = NT - kChunk * k - 8 * h - 8 - r;
-                load8(sc, srow, c0, L2, row_ok);
+            for (int h = 1; h >= 0; --h) {
+                BlockIn in;
+                const int b = 2 * k + h;
+                load_block(in, b);
+                // 1. recompute the block (forward order)
+                float rm[kCkpt], ri[kCkpt], rd[kCkpt];
+                {
+                    float Mp = in.c[0], Ip_ = in.c[1], Dl = in.c[2], Ap = in.c[3], Adg = in.c[4];
+                    float qM = in.qM, qI = in.qI, qA = in.qA;
 #pragma unroll
-                for (int v = 0; v < 8; ++v) {
-                    const int u = 8 * h + v;
-                    const int t = NT - 1 - (kChunk * k + u);
+                    for (int v = 0; v < kCkpt; ++v) {
+                        const int t = kCkpt * b + v;
+                        const int j = t - r + 1;
+                        const bool ok = !EDGE || (row_ok && j >= 1 && j <= L2);
+                        const float Mu = shift_in_prev(Mp, qM);
+                        const float Iu = shift_in_prev(Ip_, qI);
+                        const float Au = shift_in_prev(Ap, qA);
+                        float M = in.sc[v] * kap + softplus2(Adg);
+                        float I = lse2(Mu + go2, Iu + ge2);
+                        float D = lse2(Mp + go2, Dl + ge2);
+                        float A = lse3(M, I, D);
+                        if (EDGE && !ok) { M = kNinf; I = kNinf; D = kNinf; A = kNinf; }
+                        rm[v] = M; ri[v] = I; rd[v] = D;
+                        Adg = Au; Mp = M; Ip_ = I; Dl = D; Ap = A;
+                    }
+                }
+                // 2. adjoint (reverse order)
+                float pc[kCkpt];
+#pragma unroll
+                for (int v = kCkpt - 1; v >= 0; --v) {
+                    const int t = kCkpt * b + v;
                     const int j = t - r + 1;
                     const bool ok = !EDGE || (row_ok && j >= 1 && j <= L2);
-                    const float M = rM[u % kPF], I = rI[u % kPF], D = rD[u % kPF];
-                    {
-                        // unconditional (clamped) so the ring slot is simply
-                        // overwritten and only waited for kPF steps later
-                        const float* qq = pf_tail ? tab + (int64_t)max(t - kPF, 0) * P : tk - u * P;
-                        rM[u % kPF] = qq[0];
-                        rI[u % kPF] = qq[kWave];
-                        rD[u % kPF] = qq[2 * kWave];
-                    }
+                    const float M = rm[v], I = ri[v], D = rd[v];
                     const float bIs = shift_in_next(bIp, qB);
-                    const float Is = shift_in_next(Ip, qI);
-                    const float bMs = shift_in_next(bMp, qM);
+                    const float Is = shift_in_next(Ip, qIs);
+                    const float bMs = shift_in_next(bMp, qBM);
                     float A, wM, wI, wD;
                     lse3_norm(M, I, D, A, wM, wI, wD);
                     const float bA = ex2(fminf(A - St, 0.0f)) + bMdg * sigmoid2(A);
@@ -687,7 +735,7 @@ DEV void affine_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
                     float bI = bA * wI + tII;
                     float bD = bA * wD + tDD;
                     if (EDGE && !ok) { bM = 0.0f; bI = 0.0f; bD = 0.0f; tMo = 0.0f; tEx = 0.0f; }
-                    pc[7 - v] = bM;
+                    pc[v] = bM;
                     cacc[0] += tMo;
                     cacc[1] += tEx;
                     // keep the running sums in step order (the scheduler would
@@ -695,20 +743,20 @@ DEV void affine_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
                     asm volatile("" : "+v"(cacc[0]), "+v"(cacc[1]));
                     if (has_out && r == 0 && ok) {
                         float* e = bout + (int64_t)t * kBndStride;  // column j - 1 of lane 0
-                        e[0] = bI; e[1] = I; e[2] = bM;
+                        e[4] = bI; e[5] = I; e[6] = bM;
                     }
                     bMdg = bMs; bDs = bD; Ds = D; bIp = bI; Ip = I; bMp = bM;
                 }
 #pragma unroll
-                for (int v = 0; v < 8; ++v) cacc[2] += pc[v] * sc[v];
-                if (mrow) store8(mrow, c0, L2, row_ok, pc);
+                for (int v = 0; v < kCkpt; ++v) cacc[2] += pc[v] * in.sc[v];
+                if (mrow) store8(mrow, kCkpt * b - r, L2, row_ok, pc);
             }
         };
-        if (interior_bwd(nrow, k, L2)) body(std::false_type{});
+        if (interior_fwd(nrow, k, L2)) body(std::false_type{});
         else body(std::true_type{});
 #pragma unroll
         for (int c = 0; c < 3; ++c) kadd(acc[c], comp[c], cacc[c]);
-        if (has_out) publish(&prog[s], chunk_done(k, L2), lane);
+        if (has_out) publish(&prog[s], rev_done(k, L2), lane);
     }
 }
 
@@ -1043,9 +1091,9 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     if (offsets && !lengths) return fail(OHM_EINVAL, "the packed layout needs lengths");
     if ((int64_t)((L1max + kStrip - 1) / kStrip) * 4 > 60 * 1024)
         return fail(OHM_EINVAL, "L1 too long for one launch (max 983040 rows)");
-    const int pf = affine ? kPlanesAffineFwd : kPlanesLinearFwd;
-    const int pt = affine ? kPlanesAffineTan : kPlanesLinearTan;
-    WsLayout w = ws_layout(op, B, L1max, L2max, -1, pf, pt);
+    const TableSpec tf = affine ? TableSpec{kPlanesAffineCkpt, kCkpt} : TableSpec{kPlanesLinearFwd, 1};
+    const TableSpec tt = {affine ? kPlanesAffineTan : kPlanesLinearTan, 1};
+    WsLayout w = ws_layout(op, B, L1max, L2max, -1, tf, tt);
     if (!ws) return fail(OHM_EWORKSPACE, "workspace is null");
     if (ws_bytes < w.data + 256) return fail(OHM_EWORKSPACE, "workspace too small");
     if (!offsets && ws_bytes < w.total)
@@ -1076,10 +1124,10 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     L.geo = choose_geometry(B, L1max);
     L.lds = (size_t)a.max_strips * sizeof(int);
     L.stream = static_cast<hipStream_t>(stream);
-    const int planes = op == 0 ? 0 : (op == 1 ? pf : pt);
+    const TableSpec ts = op == 0 ? TableSpec{0, 1} : (op == 1 ? tf : tt);
     const int64_t cap = (int64_t)((ws_bytes - w.data) / sizeof(float));
     hipLaunchKernelGGL(sw_plan_kernel, dim3(1), dim3(kPlanThreads), 0, L.stream, lengths, offsets,
-                       B, L1max, L2max, planes, cap, const_cast<PairDesc*>(a.desc),
+                       B, L1max, L2max, ts.planes, ts.stride, cap, const_cast<PairDesc*>(a.desc),
                        const_cast<int32_t*>(a.order), a.claims, (float*)nullptr);
     stage_mark(0, L.stream);
     return check_launch("plan kernel");
@@ -1182,13 +1230,15 @@ extern "C" ohm_status ohm_set_stage_events(void* const* events, int32_t count) {
 extern "C" size_t ohm_sw_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
                                           int64_t total_cells) {
     if (B < 0 || L1max < 0 || L2max < 0 || op < 0 || op > 2) return 0;
-    return ws_layout(op, B, L1max, L2max, total_cells, kPlanesAffineFwd, kPlanesAffineTan).total;
+    return ws_layout(op, B, L1max, L2max, total_cells, TableSpec{kPlanesAffineCkpt, kCkpt},
+                     TableSpec{kPlanesAffineTan, 1}).total;
 }
 
 extern "C" size_t ohm_sw_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
                                    int64_t total_cells) {
     if (B < 0 || L1max < 0 || L2max < 0 || op < 0 || op > 2) return 0;
-    return ws_layout(op, B, L1max, L2max, total_cells, kPlanesLinearFwd, kPlanesLinearTan).total;
+    return ws_layout(op, B, L1max, L2max, total_cells, TableSpec{kPlanesLinearFwd, 1},
+                     TableSpec{kPlanesLinearTan, 1}).total;
 }
 
 extern "C" ohm_status ohm_sw_affine_forward(const float* scores, const int32_t* lengths,

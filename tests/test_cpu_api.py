@@ -49,7 +49,11 @@ def test_gpu_entry_points_reject_bad_arguments_without_a_device():
     st = lib.ohm_sw_affine_tangent(None, None, None, 1, 4, 4, -1.0, -1.0, 1.0, None, 1.0, 0.0,
                                    0.0, None, vp, None, None, None, 0, None)
     assert st == 1 and b"pure score direction" in lib.ohm_last_error()
-    assert lib.ohm_sw_affine_workspace(1, 2, 512, 512, -1) > 2 * 512 * 512 * 12
+    # value only < forward + reverse (checkpoints every 8 steps, ~2.5 B per
+    # cell) < tangent passes (full tables, 24 B per cell)
+    ws = [lib.ohm_sw_affine_workspace(op, 2, 512, 512, -1) for op in range(3)]
+    assert ws[0] < ws[1] < ws[2]
+    assert ws[1] > 2 * 512 * 512 * 2.5 and ws[2] > 2 * 512 * 512 * 24
 
 
 # --------------------------------------------------------------------------
