@@ -174,6 +174,23 @@ class Launch:
         torch.cuda.synchronize()
 
 
+def profiled_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed byte-counter profile
+    of this workload (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, calibrated with
+    a device copy of known size), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"sw_affine_{workload}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            table = json.loads(line.split(":", 1)[1])
+            key = "sw_affine_backward_kernel" if kernel == "reverse" else "sw_affine_forward_kernel<true"
+            for k, v in table.items():
+                if k.startswith(key):
+                    return int(v), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def timed(torch, fn, steps):
     """Mean ms per call over `steps` calls, bracketed by device syncs."""
     torch.cuda.synchronize()
@@ -368,13 +385,16 @@ def main():
     ach_trans = TRANS_PER_CELL[dom] * cells / t_dom
     step_roof = min(HBM_PEAK_GBS * 1e9 / sum(BYTES_PER_CELL.values()),
                     TRANS_PEAK / sum(TRANS_PER_CELL.values())) / 1e9
+    traffic, traffic_src = profiled_traffic("ragged2048" if args.workload == "ragged" else args.workload, dom)
     roofline = {
         "bound": "hbm",
         "achieved": round(ach_gbs, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(ach_gbs / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": traffic_src,
+        "algorithmic_bytes": BYTES_PER_CELL[dom] * cells,
         "kernel": KERNEL_NAMES[dom],
         "kernel_ms": round(kd[dom], 4),
         "binding": {
