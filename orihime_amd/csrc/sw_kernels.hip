@@ -627,8 +627,38 @@ struct BlockIn {
     float qM, qI, qA;            // row above, entry v (column 8b + v + 1) in lane v
 };
 
+// Map staging: the reverse sweep writes each cell's map value into a per-wave
+// LDS ring [32 steps][64 lanes] (slot = step mod 32, the same for every lane,
+// so the writes use constant offsets and hit 64 distinct banks) and stores a
+// lane's row to global memory in aligned 16-column segments once a segment is
+// complete (after the chunk that processed its lowest column).  Direct
+// per-lane stores would write 32 misaligned bytes of 64 different rows per
+// block, which the memory system turns into repeated partial-line writes.
+constexpr int kRing = 32;
+
+// Store lane r's staged segment [cs, cs + 16) of its map row (columns
+// outside [0, L2) are skipped).  Column c of lane r was computed at step c + r.
+DEV void flush_segment(const float* ring, int r, float* mrow, int cs, int L2) {
+    const int t0 = cs + r;
+    if (cs >= 0 && cs + 16 <= L2) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            f4u v;
+            v.x = ring[((t0 + 4 * q + 0) & (kRing - 1)) * kWave + r];
+            v.y = ring[((t0 + 4 * q + 1) & (kRing - 1)) * kWave + r];
+            v.z = ring[((t0 + 4 * q + 2) & (kRing - 1)) * kWave + r];
+            v.w = ring[((t0 + 4 * q + 3) & (kRing - 1)) * kWave + r];
+            *reinterpret_cast<f4u*>(mrow + cs + 4 * q) = v;
+        }
+    } else {
+        const int lo = max(0, -cs), hi = min(16, L2 - cs);
+#pragma unroll 1
+        for (int u = lo; u < hi; ++u) mrow[cs + u] = ring[((t0 + u) & (kRing - 1)) * kWave + r];
+    }
+}
+
 DEV void affine_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, int* prog, float St,
-                          float (&acc)[3], float (&comp)[3]) {
+                          float* ring, float (&acc)[3], float (&comp)[3]) {
     const int r = lane;
     const int nrow = min(kStrip, d.L1 - s * kStrip);
     const bool row_ok = r < nrow;
@@ -637,6 +667,7 @@ DEV void affine_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
     const int nchunk = (NT + kChunk - 1) / kChunk;
     const float* srow = a.scores + d.s_off + (int64_t)(s * kStrip + r) * d.ld;
     float* mrow = a.map ? a.map + d.s_off + (int64_t)(s * kStrip + r) * d.ld : nullptr;
+    const int al = (int)((d.s_off + (int64_t)(s * kStrip + r) * d.ld) & 15);  // row start mod 16 floats
     constexpr int CP = kWave * kPlanesAffineCkpt;
     const float* ck = a.table + d.t_off + (int64_t)s * nblk * CP + r;
     const bool has_up = s > 0, has_in = s + 1 < d.ns, has_out = s > 0;
@@ -712,7 +743,6 @@ DEV void affine_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
                     }
                 }
                 // 2. adjoint (reverse order)
-                float pc[kCkpt];
 #pragma unroll
                 for (int v = kCkpt - 1; v >= 0; --v) {
                     const int t = kCkpt * b + v;
@@ -735,29 +765,41 @@ DEV void affine_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
                     float bI = bA * wI + tII;
                     float bD = bA * wD + tDD;
                     if (EDGE && !ok) { bM = 0.0f; bI = 0.0f; bD = 0.0f; tMo = 0.0f; tEx = 0.0f; }
-                    pc[v] = bM;
+                    if (!EDGE || ok) ring[(t & (kRing - 1)) * kWave + r] = bM;
                     cacc[0] += tMo;
                     cacc[1] += tEx;
+                    cacc[2] += bM * in.sc[v];
                     // keep the running sums in step order (the scheduler would
                     // otherwise defer the adds and hold every summand live)
-                    asm volatile("" : "+v"(cacc[0]), "+v"(cacc[1]));
+                    asm volatile("" : "+v"(cacc[0]), "+v"(cacc[1]), "+v"(cacc[2]));
                     if (has_out && r == 0 && ok) {
                         float* e = bout + (int64_t)t * kBndStride;  // column j - 1 of lane 0
                         e[4] = bI; e[5] = I; e[6] = bM;
                     }
                     bMdg = bMs; bDs = bD; Ds = D; bIp = bI; Ip = I; bMp = bM;
                 }
-#pragma unroll
-                for (int v = 0; v < kCkpt; ++v) cacc[2] += pc[v] * in.sc[v];
-                if (mrow) store8(mrow, kCkpt * b - r, L2, row_ok, pc);
             }
         };
         if (interior_fwd(nrow, k, L2)) body(std::false_type{});
         else body(std::true_type{});
+        // Segments are 16 floats aligned in memory (a segment starts at column
+        // cs with (cs + al) % 16 == 0); the row's first and last may be
+        // partial.  Each chunk covers 16 columns of the lane's row, so exactly
+        // one segment start falls in it; that segment is complete once the
+        // chunk is done (higher columns were processed by earlier chunks).
+        // The row's first segment, when it starts below column 0, is stored
+        // after the strip.  Unstored columns span fewer than 32 steps, so the
+        // 32-slot ring is not overwritten before its columns are stored.
+        if (mrow && row_ok) {
+            const int cl = kChunk * k - r, ch = cl + kChunk - 1;
+            const int cb = ((max(cl, 0) + al + 15) & ~15) - al;  // first start >= max(cl, 0)
+            if (cb <= ch && cb < L2) flush_segment(ring, r, mrow, cb, L2);
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) kadd(acc[c], comp[c], cacc[c]);
         if (has_out) publish(&prog[s], rev_done(k, L2), lane);
     }
+    if (mrow && row_ok && al != 0 && L2 > 0) flush_segment(ring, r, mrow, -al, L2);
 }
 
 // Fixed-order sum of per-lane accumulators over the workgroup.
@@ -782,6 +824,7 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
     extern __shared__ int s_prog[];
     __shared__ int s_pair;
     __shared__ float s_red[3 * W];
+    __shared__ float s_ring[W * kRing * kWave];
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (;;) {
         const int p = claim_pair(a, &s_pair, s_prog, 1);
@@ -791,7 +834,7 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
         float acc[3] = {0.0f, 0.0f, 0.0f}, comp[3] = {0.0f, 0.0f, 0.0f};
         // bottom-up: the q-th strip of the pair in processing order is ns-1-q
         for (int q = wv; q < d.ns; q += W)
-            affine_bwd_strip(a, d, d.ns - 1 - q, lane, s_prog, St, acc, comp);
+            affine_bwd_strip(a, d, d.ns - 1 - q, lane, s_prog, St, s_ring + wv * (kRing * kWave), acc, comp);
         float tot[3];
         block_sums<3>(acc, s_red, W, lane, wv, tot);
         if (threadIdx.x == 0) {
