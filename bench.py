@@ -119,6 +119,38 @@ class Hip:
         self.lib.hipEventDestroy(ev)
 
 
+def measured_rates(torch, hip) -> dict:
+    """Issue rates of the instruction kinds the sweeps are made of, in
+    T lane-operations/s, from the probe kernels in libohm_probe.so."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "orihime_amd", "libohm_probe.so"))
+    lib.ohm_probe_launch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.c_void_p]
+    lib.ohm_probe_ops_per_thread.argtypes = [ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_int64)]
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    blocks, iters = cus * 8, 4096  # 8 waves per SIMD
+    sink = torch.empty(blocks * 256, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for name, kind in (("exp", 0), ("log", 1), ("fma", 2), ("dpp", 3), ("exp+4fma", 4)):
+        ops = ctypes.c_int64()
+        if lib.ohm_probe_ops_per_thread(kind, iters, ctypes.byref(ops)) != 0:
+            raise RuntimeError("probe op count")
+        best = None
+        for _ in range(3):
+            a, b = hip.event(), hip.event()
+            hip.record(a, stream)
+            if lib.ohm_probe_launch(kind, blocks, iters, sink.data_ptr(), stream) != 0:
+                raise RuntimeError("probe launch failed")
+            hip.record(b, stream)
+            ms = hip.elapsed_ms(a, b)
+            hip.destroy(a)
+            hip.destroy(b)
+            best = ms if best is None else min(best, ms)
+        out[name] = round(ops.value * blocks * 256 / (best * 1e-3) / 1e12, 3)
+    return out
+
+
 # --------------------------------------------------------------------------
 # One configured launch (packed or padded), with its outputs and workspace
 # --------------------------------------------------------------------------
@@ -377,6 +409,8 @@ def main():
     log(f"forward only: {fwd_ms:.3f} ms, {cells / (fwd_ms * 1e-3) / 1e9:.1f} GCUPS")
 
     hip = Hip()
+    rates = measured_rates(torch, hip)
+    log("measured issue rates (T lane-ops/s): " + ", ".join(f"{k} {v}" for k, v in rates.items()))
     kd = kernel_durations(torch, _lib, hip, main_launch, args.steps)
     log("kernel durations (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in kd.items()))
     dom = max(("forward", "reverse"), key=lambda k: kd[k])
@@ -404,6 +438,7 @@ def main():
             "unit": "Top/s",
             "frac": round(ach_trans / TRANS_PEAK, 4),
         },
+        "measured_issue_rates_tops": rates,
         "step_roof_gcups": round(step_roof, 1),
         "step_frac": round(gcups / step_roof, 4),
     }

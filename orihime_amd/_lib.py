@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "libohm_sw.so")
 if not os.path.exists(LIB_PATH):
     raise ImportError(
         f"orihime_amd native library not found at {LIB_PATH}; build it with "
-        "`python -m orihime_amd.build_native` (requires ROCm hipcc)"
+        "`python orihime_amd/build_native.py` (requires ROCm hipcc)"
     )
 
 _lib = ctypes.CDLL(LIB_PATH)

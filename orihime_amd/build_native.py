@@ -21,6 +21,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libohm_sw.so")
+PROBE_LIB = os.path.join(PKG, "libohm_probe.so")
 
 ARCH = os.environ.get("OHM_OFFLOAD_ARCH", "gfx950")
 
@@ -71,8 +72,15 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    # issue-rate probes (measured roofline constants), a library of their own
+    pobj = os.path.join(BUILD, "peak_probe.o")
+    _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "peak_probe.hip"), "-o", pobj])
+    tmp = PROBE_LIB + ".tmp"
+    _run([hipcc, "-shared", "-fPIC", "-o", tmp, pobj])
+    os.replace(tmp, PROBE_LIB)
     if verbose:
         print(f"built {LIB}")
+        print(f"built {PROBE_LIB}")
     return LIB
 
 

@@ -58,6 +58,9 @@ using namespace ohm;
 // Device helpers
 // --------------------------------------------------------------------------
 
+// Work-item index within the workgroup (read from the hardware register; the
+// threadIdx path goes through an out-of-line library call).
+DEV int tid_x() { return (int)__builtin_amdgcn_workitem_id_x(); }
 DEV float ex2(float x) { return __builtin_amdgcn_exp2f(x); }
 DEV float lg2(float x) { return __builtin_amdgcn_logf(x); }
 DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -310,7 +313,7 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
     __shared__ int64_t s_carry[1];
     __shared__ float s_key[kSortMax];
     __shared__ int s_idx[kSortMax];
-    const int tid = threadIdx.x;
+    const int tid = tid_x();
     if (tid < 64) claims[tid] = 0;
     if (tid == 0) s_carry[0] = 0;
     __syncthreads();
@@ -444,9 +447,10 @@ struct SwArgs {
 
 // Claim the next pair for this workgroup (largest first).  Returns -1 when done.
 // All threads must call it.
+template <int W>
 DEV int claim_pair(const SwArgs& a, int* s_pair, int* prog, int slot) {
-    if (threadIdx.x == 0) *s_pair = atomicAdd(&a.claims[slot], 1);
-    for (int q = threadIdx.x; q < a.max_strips; q += blockDim.x) prog[q] = 0;
+    if (tid_x() == 0) *s_pair = atomicAdd(&a.claims[slot], 1);
+    for (int q = tid_x(); q < a.max_strips; q += 64 * W) prog[q] = 0;
     __syncthreads();
     // wave-uniform from here on: descriptor loads and buffer resources derived
     // from the pair index stay in scalar registers
@@ -562,7 +566,7 @@ DEV float finish_partition(float zref, float z, float* s_red, int W, int lane, i
     if (lane == 0) { s_red[2 * wv] = m; s_red[2 * wv + 1] = zz; }
     __syncthreads();
     float St = 0.0f;
-    if (threadIdx.x == 0) {
+    if (tid_x() == 0) {
         float M = 0.0f;
         for (int w = 0; w < W; ++w) M = fmaxf(M, s_red[2 * w]);
         float Z = ex2(-M);
@@ -577,15 +581,15 @@ __global__ __launch_bounds__(W * 64) void sw_affine_forward_kernel(SwArgs a) {
     extern __shared__ int s_prog[];
     __shared__ int s_pair;
     __shared__ float s_red[2 * W];
-    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = tid_x() & 63, wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     for (;;) {
-        const int p = claim_pair(a, &s_pair, s_prog, 0);
+        const int p = claim_pair<W>(a, &s_pair, s_prog, 0);
         if (p < 0) break;
         const PairDesc d = a.desc[p];
         float zref = kNinf, z = 0.0f, zcomp = 0.0f;
         for (int s = wv; s < d.ns; s += W) affine_fwd_strip<STORE>(a, d, s, lane, s_prog, zref, z, zcomp);
         const float St = finish_partition(zref, z, s_red, W, lane, wv);
-        if (threadIdx.x == 0) {
+        if (tid_x() == 0) {
             a.stil[p] = St;
             a.value[p] = St / a.kappa;
         }
@@ -825,9 +829,9 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
     __shared__ int s_pair;
     __shared__ float s_red[3 * W];
     __shared__ float s_ring[W * kRing * kWave];
-    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = tid_x() & 63, wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     for (;;) {
-        const int p = claim_pair(a, &s_pair, s_prog, 1);
+        const int p = claim_pair<W>(a, &s_pair, s_prog, 1);
         if (p < 0) break;
         const PairDesc d = a.desc[p];
         const float St = a.stil[p];
@@ -837,13 +841,13 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
             affine_bwd_strip(a, d, d.ns - 1 - q, lane, s_prog, St, s_ring + wv * (kRing * kWave), acc, comp);
         float tot[3];
         block_sums<3>(acc, s_red, W, lane, wv, tot);
-        if (threadIdx.x == 0) {
+        if (tid_x() == 0) {
             const float S = a.value[p];
             if (a.g0) a.g0[p] = tot[0];
             if (a.g1) a.g1[p] = tot[1];
             if (a.g2) a.g2[p] = (S - (tot[2] + a.go * tot[0] + a.ge * tot[1])) / a.temp;
         }
-        if (a.padded) zero_padding(a.map, d, a.L1max, a.L2max, threadIdx.x, blockDim.x);
+        if (a.padded) zero_padding(a.map, d, a.L1max, a.L2max, tid_x(), 64 * W);
         __syncthreads();
     }
 }
@@ -913,15 +917,15 @@ __global__ __launch_bounds__(W * 64) void sw_forward_kernel(SwArgs a) {
     extern __shared__ int s_prog[];
     __shared__ int s_pair;
     __shared__ float s_red[2 * W];
-    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = tid_x() & 63, wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     for (;;) {
-        const int p = claim_pair(a, &s_pair, s_prog, 0);
+        const int p = claim_pair<W>(a, &s_pair, s_prog, 0);
         if (p < 0) break;
         const PairDesc d = a.desc[p];
         float zref = kNinf, z = 0.0f, zcomp = 0.0f;
         for (int s = wv; s < d.ns; s += W) linear_fwd_strip<STORE>(a, d, s, lane, s_prog, zref, z, zcomp);
         const float St = finish_partition(zref, z, s_red, W, lane, wv);
-        if (threadIdx.x == 0) {
+        if (tid_x() == 0) {
             a.stil[p] = St;
             a.value[p] = St / a.kappa;
         }
@@ -1023,9 +1027,9 @@ __global__ __launch_bounds__(W * 64) void sw_backward_kernel(SwArgs a) {
     extern __shared__ int s_prog[];
     __shared__ int s_pair;
     __shared__ float s_red[2 * W];
-    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = tid_x() & 63, wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     for (;;) {
-        const int p = claim_pair(a, &s_pair, s_prog, 1);
+        const int p = claim_pair<W>(a, &s_pair, s_prog, 1);
         if (p < 0) break;
         const PairDesc d = a.desc[p];
         const float St = a.stil[p];
@@ -1034,12 +1038,12 @@ __global__ __launch_bounds__(W * 64) void sw_backward_kernel(SwArgs a) {
             linear_bwd_strip(a, d, d.ns - 1 - q, lane, s_prog, St, acc, comp);
         float tot[2];
         block_sums<2>(acc, s_red, W, lane, wv, tot);
-        if (threadIdx.x == 0) {
+        if (tid_x() == 0) {
             const float S = a.value[p];
             if (a.g0) a.g0[p] = tot[0];
             if (a.g1) a.g1[p] = (S - (tot[1] + a.go * tot[0])) / a.temp;
         }
-        if (a.padded) zero_padding(a.map, d, a.L1max, a.L2max, threadIdx.x, blockDim.x);
+        if (a.padded) zero_padding(a.map, d, a.L1max, a.L2max, tid_x(), 64 * W);
         __syncthreads();
     }
 }
