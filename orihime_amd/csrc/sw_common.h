@@ -59,7 +59,9 @@ struct PairDesc {
 //           recomputes the steps in between),
 //   linear and the tangent passes: every step (full table).
 constexpr int kCkpt = 8;                // steps between affine checkpoints
-constexpr int kPlanesAffineCkpt = 5;    // M, I, D, A of the previous step; A of the diagonal
+constexpr int kPlanesAffineCkpt = 6;    // six state rows (log-domain sweep: M, I, D, A of the
+                                        // previous step and A of the diagonal; scaled sweep:
+                                        // g, h, a, c of the previous step, a and c of the diagonal)
 constexpr int kPlanesAffineTan = 6;     // M, I, D and their tangents
 constexpr int kPlanesLinearFwd = 2;     // A, X
 constexpr int kPlanesLinearTan = 4;     // A, X and their tangents
@@ -82,19 +84,25 @@ inline int64_t bnd_bound(int64_t N, int64_t B, int64_t L2max) {
 // and its strip-boundary block are contiguous inside one data region whose
 // offsets the plan kernel computes exactly; `data` bytes bound the region.
 struct WsLayout {
-    size_t claims, desc, order, stil, tgt, data, total;
+    size_t claims, desc, order, stil, tgt, pst, data, total;
 };
+
+// Per-pair state words of the scaled sweeps (4 int32 per pair).
+constexpr int kPstFlag = 0;   // 1: the scaled sweep left its range, the log-domain sweep redoes the pair
+constexpr int kPstZexp = 1;   // partition Z = zm * 2^zexp, zm in [1, 2)
+constexpr int kPstZinv = 2;   // bits of 1 / zm
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 inline size_t ws_fixed(int64_t B, size_t* claims, size_t* desc, size_t* order, size_t* stil,
-                       size_t* tgt) {
+                       size_t* tgt, size_t* pst) {
     size_t o = 0;
     *claims = o; o += 256;
     *desc = o; o = align256(o + (size_t)B * sizeof(PairDesc));
     *order = o; o = align256(o + (size_t)B * sizeof(int32_t));
     *stil = o; o = align256(o + (size_t)B * sizeof(float));
     *tgt = o; o = align256(o + (size_t)B * 4 * sizeof(float));
+    *pst = o; o = align256(o + (size_t)B * 4 * sizeof(int32_t));
     return o;
 }
 
@@ -103,7 +111,7 @@ inline WsLayout ws_layout(int op, int64_t B, int64_t L1max, int64_t L2max, int64
                           TableSpec fwd, TableSpec tan) {
     if (N < 0) N = B * L1max * L2max;
     WsLayout w;
-    size_t o = ws_fixed(B, &w.claims, &w.desc, &w.order, &w.stil, &w.tgt);
+    size_t o = ws_fixed(B, &w.claims, &w.desc, &w.order, &w.stil, &w.tgt, &w.pst);
     w.data = o;
     const TableSpec none = {0, 1};
     const TableSpec t = op == 0 ? none : (op == 1 ? fwd : tan);

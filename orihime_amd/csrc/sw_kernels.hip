@@ -113,22 +113,30 @@ DEV float shift_in_next(float v, float& q) {
     q = nq;
     return x;
 }
-
-// Table rows through a raw buffer resource: a 32-bit lane offset plus a
-// wave-uniform byte offset in a scalar register, so a step's address update
-// is one scalar add.
-typedef __amdgpu_buffer_rsrc_t Rsrc;
-DEV Rsrc make_rsrc(const void* base, int64_t bytes) {
-    const int64_t cap = 0x7ffffff0;
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0,
-                                            (int)(bytes < cap ? bytes : cap), 0x00020000);
+// The same moves for 32-bit integers (the scaled sweeps' exponents).
+DEV int shift_in_prev_i(int v, int& q) {
+    const int nq = __builtin_amdgcn_mov_dpp(q, 0x130, 0xf, 0xf, true);
+    const int x = __builtin_amdgcn_update_dpp(q, v, 0x138, 0xf, 0xf, false);
+    q = nq;
+    return x;
 }
-DEV float bload(Rsrc r, uint32_t voff, uint32_t soff) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+DEV int shift_in_next_i(int v, int& q) {
+    const int nq = __builtin_amdgcn_mov_dpp(q, 0x138, 0xf, 0xf, true);
+    const int x = __builtin_amdgcn_update_dpp(q, v, 0x130, 0xf, 0xf, false);
+    q = nq;
+    return x;
 }
-DEV void bstore(float v, Rsrc r, uint32_t voff, uint32_t soff) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+DEV bool any_greater_i(int a, int b) {
+    uint64_t m;
+    asm volatile("v_cmp_gt_i32_e64 %0, %1, %2" : "=s"(m) : "v"(a), "v"(b));
+    return m != 0;
 }
+// x * 2^e and the binary exponent of x (x = m * 2^e, m in [0.5, 1)), exact.
+DEV float ldx(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
+DEV int fexp(float x) { return __builtin_amdgcn_frexp_expf(x); }
+// Finite test on the bit pattern (the device code is built without NaN
+// semantics, so isfinite() may be folded away).
+DEV bool finite_bits(float x) { return (__builtin_bit_cast(uint32_t, x) & 0x7f800000u) != 0x7f800000u; }
 
 // Global accesses as (wave-uniform base pointer) + (32-bit lane offset): the
 // compiler emits the scalar-base form, so per-step address updates are scalar
@@ -443,12 +451,17 @@ struct SwArgs {
     float go2, ge2;          // gap_open, gap_ext (affine) or gap (linear, go2) in log2 units
     float go, ge, temp;      // natural units
     float d_go, d_ge, d_temp;  // scalar tangent directions
+    float lgo, lge;          // 2^go2, 2^ge2: gap factors of the scaled sweeps
+    int* pst;                // per pair: kPst* state words of the scaled sweeps
+    int sel;                 // log-domain affine sweeps: 0 every pair, 1 only flagged pairs
 };
 
 // Claim the next pair for this workgroup (largest first).  Returns -1 when done.
-// All threads must call it.
+// All threads must call it.  The leading barrier orders the claim after every
+// wave's use of the previous one (s_pair, the progress counters).
 template <int W>
 DEV int claim_pair(const SwArgs& a, int* s_pair, int* prog, int slot) {
+    __syncthreads();
     if (tid_x() == 0) *s_pair = atomicAdd(&a.claims[slot], 1);
     for (int q = tid_x(); q < a.max_strips; q += 64 * W) prog[q] = 0;
     __syncthreads();
@@ -583,8 +596,9 @@ __global__ __launch_bounds__(W * 64) void sw_affine_forward_kernel(SwArgs a) {
     __shared__ float s_red[2 * W];
     const int lane = tid_x() & 63, wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     for (;;) {
-        const int p = claim_pair<W>(a, &s_pair, s_prog, 0);
+        const int p = claim_pair<W>(a, &s_pair, s_prog, a.sel ? 4 : 0);
         if (p < 0) break;
+        if (a.sel && __builtin_amdgcn_readfirstlane(a.pst[4 * p + kPstFlag]) == 0) continue;
         const PairDesc d = a.desc[p];
         float zref = kNinf, z = 0.0f, zcomp = 0.0f;
         for (int s = wv; s < d.ns; s += W) affine_fwd_strip<STORE>(a, d, s, lane, s_prog, zref, z, zcomp);
@@ -653,6 +667,9 @@ DEV void flush_segment(const float* ring, int r, float* mrow, int cs, int L2) {
             v.z = ring[((t0 + 4 * q + 2) & (kRing - 1)) * kWave + r];
             v.w = ring[((t0 + 4 * q + 3) & (kRing - 1)) * kWave + r];
             *reinterpret_cast<f4u*>(mrow + cs + 4 * q) = v;
+            // one group of four in flight at a time (hoisting all sixteen
+            // reads with their addresses costs ~30 registers)
+            asm volatile("" ::: "memory");
         }
     } else {
         const int lo = max(0, -cs), hi = min(16, L2 - cs);
@@ -831,8 +848,9 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
     __shared__ float s_ring[W * kRing * kWave];
     const int lane = tid_x() & 63, wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     for (;;) {
-        const int p = claim_pair<W>(a, &s_pair, s_prog, 1);
+        const int p = claim_pair<W>(a, &s_pair, s_prog, a.sel ? 5 : 1);
         if (p < 0) break;
+        if (a.sel && __builtin_amdgcn_readfirstlane(a.pst[4 * p + kPstFlag]) == 0) continue;
         const PairDesc d = a.desc[p];
         const float St = a.stil[p];
         float acc[3] = {0.0f, 0.0f, 0.0f}, comp[3] = {0.0f, 0.0f, 0.0f};
@@ -851,6 +869,8 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
         __syncthreads();
     }
 }
+
+#include "sw_affine_scaled.hiph"
 
 // ==========================================================================
 // Linear gap: forward.  Table planes: A (cell value), X = s + LSE(A', 0).
@@ -919,8 +939,9 @@ __global__ __launch_bounds__(W * 64) void sw_forward_kernel(SwArgs a) {
     __shared__ float s_red[2 * W];
     const int lane = tid_x() & 63, wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     for (;;) {
-        const int p = claim_pair<W>(a, &s_pair, s_prog, 0);
+        const int p = claim_pair<W>(a, &s_pair, s_prog, a.sel ? 4 : 0);
         if (p < 0) break;
+        if (a.sel && __builtin_amdgcn_readfirstlane(a.pst[4 * p + kPstFlag]) == 0) continue;
         const PairDesc d = a.desc[p];
         float zref = kNinf, z = 0.0f, zcomp = 0.0f;
         for (int s = wv; s < d.ns; s += W) linear_fwd_strip<STORE>(a, d, s, lane, s_prog, zref, z, zcomp);
@@ -1153,6 +1174,7 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     a.order = reinterpret_cast<const int32_t*>(base + w.order);
     a.claims = reinterpret_cast<int*>(base + w.claims);
     a.stil = reinterpret_cast<float*>(base + w.stil);
+    a.pst = reinterpret_cast<int*>(base + w.pst);
     a.tgt = reinterpret_cast<float*>(base + w.tgt);
     a.table = reinterpret_cast<float*>(base + w.data);
     a.bnd = a.table;  // per-pair offsets (t_off, b_off) index the same region
@@ -1168,6 +1190,8 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     a.go = go;
     a.ge = ge;
     a.temp = temp;
+    a.lgo = std::exp2(a.go2);
+    a.lge = std::exp2(a.ge2);
     L.geo = choose_geometry(B, L1max);
     L.lds = (size_t)a.max_strips * sizeof(int);
     L.stream = static_cast<hipStream_t>(stream);
@@ -1191,6 +1215,14 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
         }                                                                                 \
     } while (0)
 
+// The scaled affine sweeps need representable gap factors 2^(g log2(e) / T)
+// with room for products of two of them and a neighbour ratio; outside that
+// (or with OHM_SCALED=0) the log-domain sweeps run alone.
+bool use_scaled(const SwArgs& a) {
+    if (env_int("OHM_SCALED", 1) == 0) return false;
+    return std::fabs(a.go2) <= 120.0f && std::fabs(a.ge2) <= 120.0f;
+}
+
 ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths,
                         const int64_t* offsets, int B, int L1max, int L2max, float go, float ge,
                         float temp, float* value, float* map, float* g0, float* g1, float* g2,
@@ -1206,7 +1238,17 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     L.a.g0 = g0;
     L.a.g1 = g1;
     L.a.g2 = g2;
-    if (affine) {
+    const bool scaled = affine && use_scaled(L.a);
+    if (scaled) {
+        // scaled sweep for every pair, then the log-domain sweep for the pairs
+        // it flagged (a launch whose groups only walk the claim list when
+        // none are)
+        if (need_bwd) OHM_DISPATCH_W(L, (sw_affine_scaled_forward_kernel<true, W>));
+        else OHM_DISPATCH_W(L, (sw_affine_scaled_forward_kernel<false, W>));
+        L.a.sel = 1;
+        if (need_bwd) OHM_DISPATCH_W(L, (sw_affine_forward_kernel<true, W>));
+        else OHM_DISPATCH_W(L, (sw_affine_forward_kernel<false, W>));
+    } else if (affine) {
         if (need_bwd) OHM_DISPATCH_W(L, (sw_affine_forward_kernel<true, W>));
         else OHM_DISPATCH_W(L, (sw_affine_forward_kernel<false, W>));
     } else {
@@ -1216,8 +1258,16 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     stage_mark(1, L.stream);
     if ((st = check_launch("forward kernel")) != OHM_OK) return st;
     if (!need_bwd) return OHM_OK;
-    if (affine) OHM_DISPATCH_W(L, (sw_affine_backward_kernel<W>));
-    else OHM_DISPATCH_W(L, (sw_backward_kernel<W>));
+    if (scaled) {
+        L.a.sel = 0;
+        OHM_DISPATCH_W(L, (sw_affine_scaled_backward_kernel<W>));
+        L.a.sel = 1;
+        OHM_DISPATCH_W(L, (sw_affine_backward_kernel<W>));
+    } else if (affine) {
+        OHM_DISPATCH_W(L, (sw_affine_backward_kernel<W>));
+    } else {
+        OHM_DISPATCH_W(L, (sw_backward_kernel<W>));
+    }
     stage_mark(2, L.stream);
     return check_launch("backward kernel");
 }

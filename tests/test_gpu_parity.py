@@ -180,6 +180,49 @@ def test_affine_gpu_vs_oracle(shape, params):
                                      1e-3 * max(1.0, abs(r[k + 1])))
 
 
+@pytest.mark.parametrize("shape", [(2, 63, 65), (1, 129, 200), (2, 191, 193)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("params", PARAMS, ids=lambda p: "go%g_ge%g_T%g" % p)
+def test_affine_log_domain_sweeps_vs_oracle(shape, params, monkeypatch):
+    """The log-domain affine sweeps (used alone when the scaled sweeps are
+    switched off or the gap factors are out of their range, and for pairs the
+    scaled sweeps flag) against the oracle."""
+    monkeypatch.setenv("OHM_SCALED", "0")
+    go, ge, T = params
+    s, _, _, ln = _inputs(*shape, seed=hash((shape, params, "log")) % 2**31)
+    case(f"affine log-domain shape={shape} params={params}")
+    v, m, a, b, c = torch.ops.orihime.soft_sw_affine_with_grads(g(s), go, ge, T, g(ln))
+    ref = O.sw_affine_forward(s, go, ge, T, ln)
+    assert rel_err(h(v), ref[0], 1e-3) < 1e-5
+    assert abs_err(h(m), ref[1]) < 1e-4
+    assert rel_err(h(a), ref[2], 1.0) < 1e-4 and rel_err(h(b), ref[3], 1.0) < 1e-4
+    assert entropy_ok(h(c), ref[4], ref[0], T)
+
+
+def test_scaled_sweep_out_of_range_pair_is_redone_in_log_domain(monkeypatch):
+    """A pair whose adjacent scores jump from -80 T to +80 T (inside the
+    reference's domain) overflows the scaled sweep's exponent range; it is
+    flagged and recomputed by the log-domain sweeps, so its outputs are the
+    log-domain ones bit for bit, while the other pair of the batch keeps the
+    scaled result.  Both match the oracle."""
+    go, ge, T = -11.0, -1.0, 1.0
+    rng = np.random.default_rng(7)
+    s = rng.normal(size=(2, 40, 50)).astype(np.float32)
+    s[0] = -80.0
+    s[0, 20, 25] = 80.0
+    s[0, 21, 26] = 79.0
+    ln = np.array([[40, 50], [40, 50]], np.int32)
+    case("scaled sweep range fallback")
+    out = [t.clone() for t in torch.ops.orihime.soft_sw_affine_with_grads(g(s), go, ge, T, g(ln))]
+    monkeypatch.setenv("OHM_SCALED", "0")
+    logd = torch.ops.orihime.soft_sw_affine_with_grads(g(s), go, ge, T, g(ln))
+    for x, y in zip(out, logd):
+        assert torch.equal(x[0], y[0])
+    ref = O.sw_affine_forward(s, go, ge, T, ln)
+    assert rel_err(h(out[0]), ref[0], 1e-3) < 1e-5
+    assert abs_err(h(out[1]), ref[1]) < 1e-4
+    assert rel_err(h(out[2]), ref[2], 1.0) < 1e-4 and rel_err(h(out[3]), ref[3], 1.0) < 1e-4
+
+
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("params", [(-1.0, 1.0), (-0.4, 0.5), (-2.0, 2.0)], ids=lambda p: "g%g_T%g" % p)
 def test_linear_gpu_vs_oracle(shape, params):
