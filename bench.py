@@ -57,8 +57,8 @@ TRANS_PER_CELL = {"forward": 11, "reverse": 11}
 BYTES_PER_CELL = {"forward": 4, "reverse": 4}
 KERNEL_NAMES = {
     "plan": "sw_plan_kernel",
-    "forward": "sw_affine_forward_kernel<true, W>",
-    "reverse": "sw_affine_backward_kernel<W>",
+    "forward": "sw_affine_scaled_forward_kernel<true, W>",
+    "reverse": "sw_affine_scaled_backward_kernel<W>",
 }
 
 
@@ -216,7 +216,7 @@ def profiled_traffic(workload: str, kernel: str):
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
             table = json.loads(line.split(":", 1)[1])
-            key = "sw_affine_backward_kernel" if kernel == "reverse" else "sw_affine_forward_kernel<true"
+            key = KERNEL_NAMES[kernel].split("<")[0] + "<" + ("true" if kernel == "forward" else "")
             for k, v in table.items():
                 if k.startswith(key):
                     return int(v), os.path.relpath(path, ROOT)
@@ -241,7 +241,8 @@ def kernel_durations(torch, lib, hip, launch, steps):
     """Mean duration (ms) of each of the step's launches, from HIP events
     recorded on the launch stream between them."""
     stream = torch.cuda.current_stream().cuda_stream
-    evs = [[hip.event() for _ in range(4)] for _ in range(steps)]
+    names = ("plan", "forward", "forward_fallback", "reverse", "reverse_fallback")
+    evs = [[hip.event() for _ in range(len(names) + 1)] for _ in range(steps)]
     torch.cuda.synchronize()
     try:
         for e in evs:
@@ -251,11 +252,10 @@ def kernel_durations(torch, lib, hip, launch, steps):
     finally:
         lib.set_stage_events([])
     torch.cuda.synchronize()
-    out = {"plan": 0.0, "forward": 0.0, "reverse": 0.0}
+    out = {k: 0.0 for k in names}
     for e in evs:
-        out["plan"] += hip.elapsed_ms(e[0], e[1])
-        out["forward"] += hip.elapsed_ms(e[1], e[2])
-        out["reverse"] += hip.elapsed_ms(e[2], e[3])
+        for i, k in enumerate(names):
+            out[k] += hip.elapsed_ms(e[i], e[i + 1])
         for x in e:
             hip.destroy(x)
     return {k: v / steps for k, v in out.items()}
@@ -417,8 +417,11 @@ def main():
     t_dom = kd[dom] * 1e-3
     ach_gbs = BYTES_PER_CELL[dom] * cells / t_dom / 1e9
     ach_trans = TRANS_PER_CELL[dom] * cells / t_dom
+    # transcendental peak: measured on this device (v_exp_f32 issue rate),
+    # the quarter-rate assumption only when the probe is unavailable
+    trans_peak = rates["exp"] * 1e12 if rates else TRANS_PEAK
     step_roof = min(HBM_PEAK_GBS * 1e9 / sum(BYTES_PER_CELL.values()),
-                    TRANS_PEAK / sum(TRANS_PER_CELL.values())) / 1e9
+                    trans_peak / sum(TRANS_PER_CELL.values())) / 1e9
     traffic, traffic_src = profiled_traffic("ragged2048" if args.workload == "ragged" else args.workload, dom)
     roofline = {
         "bound": "hbm",
@@ -433,10 +436,12 @@ def main():
         "kernel_ms": round(kd[dom], 4),
         "binding": {
             "bound": "transcendental",
+            "per_cell": TRANS_PER_CELL[dom],
             "achieved": round(ach_trans / 1e12, 4),
-            "peak": round(TRANS_PEAK / 1e12, 4),
+            "peak": round(trans_peak / 1e12, 4),
+            "peak_source": "measured v_exp_f32 issue rate" if rates else "quarter-rate assumption",
             "unit": "Top/s",
-            "frac": round(ach_trans / TRANS_PEAK, 4),
+            "frac": round(ach_trans / trans_peak, 4),
         },
         "measured_issue_rates_tops": rates,
         "step_roof_gcups": round(step_roof, 1),

@@ -62,9 +62,12 @@ const char* ohm_version(void);
 
 /* Kernel timing hook (diagnostics; the reference has no counterpart).  While
  * set, every GPU entry point records events[k] (a hipEvent_t created by the
- * caller) on its stream right after its k-th kernel launch: k = 0 the
- * planning kernel, k = 1 the forward sweep (or tangent forward sweep), k = 2
- * the reverse sweep.  NULL entries are skipped; count 0 clears the hook.
+ * caller) on its stream after its launches: k = 0 after the planning kernel,
+ * k = 1 after the forward sweep (or tangent forward sweep), k = 2 after the
+ * forward sweep's log-domain launch for flagged pairs, k = 3 after the
+ * reverse sweep, k = 4 after the reverse sweep's log-domain launch (marks
+ * without such a launch are recorded right after the previous one).  NULL
+ * entries are skipped; count 0 clears the hook (count <= 5).
  * The setting is process-wide and unsynchronised: change it only while no
  * other thread is inside the library, and never while a HIP graph is being
  * captured. */

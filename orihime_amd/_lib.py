@@ -106,10 +106,11 @@ _ws_cache: dict[tuple[int, int], torch.Tensor] = {}
 
 
 def set_stage_events(events) -> None:
-    """Record the given hipEvent_t handles (ints, up to 3; None skips one)
-    after the planning, forward and reverse launches of every GPU call; an
-    empty sequence clears the hook."""
-    arr = (ctypes.c_void_p * 3)(*[(e or None) for e in events])
+    """Record the given hipEvent_t handles (ints, up to 5; None skips one)
+    after the planning launch, the forward sweep, its log-domain launch, the
+    reverse sweep and its log-domain launch of every GPU call; an empty
+    sequence clears the hook."""
+    arr = (ctypes.c_void_p * 5)(*[(e or None) for e in events])
     _check(_stage_events(arr, len(events)), "ohm_set_stage_events")
 
 

@@ -91,6 +91,9 @@ struct WsLayout {
 constexpr int kPstFlag = 0;   // 1: the scaled sweep left its range, the log-domain sweep redoes the pair
 constexpr int kPstZexp = 1;   // partition Z = zm * 2^zexp, zm in [1, 2)
 constexpr int kPstZinv = 2;   // bits of 1 / zm
+// Slots of the claims block (64 ints, zeroed by the plan kernel): pair claim
+// counters of the launches, and the count of flagged pairs.
+constexpr int kClaimFlagged = 8;
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 

@@ -596,6 +596,7 @@ __global__ __launch_bounds__(W * 64) void sw_affine_forward_kernel(SwArgs a) {
     __shared__ float s_red[2 * W];
     const int lane = tid_x() & 63, wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     for (;;) {
+        if (a.sel && __builtin_amdgcn_readfirstlane(a.claims[kClaimFlagged]) == 0) return;  // uniform
         const int p = claim_pair<W>(a, &s_pair, s_prog, a.sel ? 4 : 0);
         if (p < 0) break;
         if (a.sel && __builtin_amdgcn_readfirstlane(a.pst[4 * p + kPstFlag]) == 0) continue;
@@ -848,6 +849,7 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
     __shared__ float s_ring[W * kRing * kWave];
     const int lane = tid_x() & 63, wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     for (;;) {
+        if (a.sel && __builtin_amdgcn_readfirstlane(a.claims[kClaimFlagged]) == 0) return;  // uniform
         const int p = claim_pair<W>(a, &s_pair, s_prog, a.sel ? 5 : 1);
         if (p < 0) break;
         if (a.sel && __builtin_amdgcn_readfirstlane(a.pst[4 * p + kPstFlag]) == 0) continue;
@@ -870,6 +872,7 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
     }
 }
 
+#include "sw_teams.hiph"
 #include "sw_affine_scaled.hiph"
 
 // ==========================================================================
@@ -939,6 +942,7 @@ __global__ __launch_bounds__(W * 64) void sw_forward_kernel(SwArgs a) {
     __shared__ float s_red[2 * W];
     const int lane = tid_x() & 63, wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     for (;;) {
+        if (a.sel && __builtin_amdgcn_readfirstlane(a.claims[kClaimFlagged]) == 0) return;  // uniform
         const int p = claim_pair<W>(a, &s_pair, s_prog, a.sel ? 4 : 0);
         if (p < 0) break;
         if (a.sel && __builtin_amdgcn_readfirstlane(a.pst[4 * p + kPstFlag]) == 0) continue;
@@ -1115,9 +1119,11 @@ int env_int(const char* name, int fallback) {
 // pairs.  Workgroups: enough persistent groups for 16 waves per CU, never more
 // than there are pairs.  OHM_WAVES / OHM_WG_PER_CU override for tuning.
 // Optional per-launch timing events (ohm_set_stage_events).
-void* g_stage_events[3] = {nullptr, nullptr, nullptr};
+constexpr int kStageMarks = 5;
+void* g_stage_events[kStageMarks] = {};
 
 void stage_mark(int k, hipStream_t stream) {
+    if (k < 0 || k >= kStageMarks) return;
     if (g_stage_events[k]) (void)hipEventRecord(static_cast<hipEvent_t>(g_stage_events[k]), stream);
 }
 
@@ -1215,6 +1221,33 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
         }                                                                                 \
     } while (0)
 
+// Team kernels (wave-level strip scheduling, one workgroup of kTeamWaves
+// waves per CU) pay off for batches of few large pairs (measured: 4096-long
+// pairs, 16 per batch, 10% faster); for batches with at least a quarter as
+// many pairs as CUs the per-pair workgroup kernels are faster (C2/C3-like
+// batches: 15-20%).  They hold per-slot progress counters and partials in
+// LDS, and the reverse sweep its waves' map rings; past that size (very tall
+// pairs) the workgroup kernels run.  OHM_TEAMS=0/1 forces the choice.
+constexpr size_t kLdsLimit = 160 * 1024;
+size_t team_bwd_lds_bytes(int max_strips) {
+    return team_lds_bytes(max_strips) + (size_t)kTeamWaves * kRing * kWave * sizeof(float);
+}
+bool use_teams(int B, int max_strips) {
+    const int forced = env_int("OHM_TEAMS", -1);
+    if (forced == 0) return false;
+    if (team_bwd_lds_bytes(max_strips) + sizeof(TeamLds) + 1024 > kLdsLimit) return false;
+    return forced == 1 || 4 * B < device_cus();
+}
+template <typename K>
+ohm_status launch_team(K kernel, size_t lds, const Launch& L) {
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return fail(OHM_ELAUNCH, "cannot reserve LDS for the team kernel");
+    hipLaunchKernelGGL(kernel, dim3(device_cus()), dim3(64 * kTeamWaves), lds, L.stream, L.a);
+    return check_launch("team kernel");
+}
+
 // The scaled affine sweeps need representable gap factors 2^(g log2(e) / T)
 // with room for products of two of them and a neighbour ratio; outside that
 // (or with OHM_SCALED=0) the log-domain sweeps run alone.
@@ -1239,12 +1272,21 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     L.a.g1 = g1;
     L.a.g2 = g2;
     const bool scaled = affine && use_scaled(L.a);
+    const bool team = scaled && use_teams(B, L.a.max_strips);
     if (scaled) {
         // scaled sweep for every pair, then the log-domain sweep for the pairs
         // it flagged (a launch whose groups only walk the claim list when
         // none are)
-        if (need_bwd) OHM_DISPATCH_W(L, (sw_affine_scaled_forward_kernel<true, W>));
-        else OHM_DISPATCH_W(L, (sw_affine_scaled_forward_kernel<false, W>));
+        if (team) {
+            const size_t lds = team_lds_bytes(L.a.max_strips);
+            if (need_bwd) st = launch_team(sw_affine_scaled_forward_team<true>, lds, L);
+            else st = launch_team(sw_affine_scaled_forward_team<false>, lds, L);
+            if (st != OHM_OK) return st;
+        } else {
+            if (need_bwd) OHM_DISPATCH_W(L, (sw_affine_scaled_forward_kernel<true, W>));
+            else OHM_DISPATCH_W(L, (sw_affine_scaled_forward_kernel<false, W>));
+        }
+        stage_mark(1, L.stream);
         L.a.sel = 1;
         if (need_bwd) OHM_DISPATCH_W(L, (sw_affine_forward_kernel<true, W>));
         else OHM_DISPATCH_W(L, (sw_affine_forward_kernel<false, W>));
@@ -1255,12 +1297,19 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
         if (need_bwd) OHM_DISPATCH_W(L, (sw_forward_kernel<true, W>));
         else OHM_DISPATCH_W(L, (sw_forward_kernel<false, W>));
     }
-    stage_mark(1, L.stream);
+    if (!scaled) stage_mark(1, L.stream);
+    stage_mark(2, L.stream);
     if ((st = check_launch("forward kernel")) != OHM_OK) return st;
     if (!need_bwd) return OHM_OK;
     if (scaled) {
         L.a.sel = 0;
-        OHM_DISPATCH_W(L, (sw_affine_scaled_backward_kernel<W>));
+        if (team) {
+            if ((st = launch_team(sw_affine_scaled_backward_team, team_bwd_lds_bytes(L.a.max_strips), L)) != OHM_OK)
+                return st;
+        } else {
+            OHM_DISPATCH_W(L, (sw_affine_scaled_backward_kernel<W>));
+        }
+        stage_mark(3, L.stream);
         L.a.sel = 1;
         OHM_DISPATCH_W(L, (sw_affine_backward_kernel<W>));
     } else if (affine) {
@@ -1268,7 +1317,8 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     } else {
         OHM_DISPATCH_W(L, (sw_backward_kernel<W>));
     }
-    stage_mark(2, L.stream);
+    if (!scaled) stage_mark(3, L.stream);
+    stage_mark(4, L.stream);
     return check_launch("backward kernel");
 }
 
@@ -1297,30 +1347,34 @@ ohm_status tangent_impl(bool affine, const float* scores, const int32_t* lengths
         if (dir) {
             OHM_DISPATCH_W(L, (sw_affine_hvp_forward_kernel<W>));
             stage_mark(1, L.stream);
+            stage_mark(2, L.stream);
             if ((st = check_launch("tangent forward")) != OHM_OK) return st;
             OHM_DISPATCH_W(L, (sw_affine_hvp_backward_kernel<W>));
         } else {
             OHM_DISPATCH_W(L, (sw_affine_param_grad_forward_kernel<W>));
             stage_mark(1, L.stream);
+            stage_mark(2, L.stream);
             if ((st = check_launch("tangent forward")) != OHM_OK) return st;
             OHM_DISPATCH_W(L, (sw_affine_param_grad_backward_kernel<W>));
         }
     } else {
         OHM_DISPATCH_W(L, (sw_hvp_forward_kernel<W>));
         stage_mark(1, L.stream);
+        stage_mark(2, L.stream);
         if ((st = check_launch("tangent forward")) != OHM_OK) return st;
         OHM_DISPATCH_W(L, (sw_hvp_backward_kernel<W>));
     }
-    stage_mark(2, L.stream);
+    stage_mark(3, L.stream);
+    stage_mark(4, L.stream);
     return check_launch("tangent backward");
 }
 
 }  // namespace
 
 extern "C" ohm_status ohm_set_stage_events(void* const* events, int32_t count) {
-    if (count < 0 || count > 3 || (count > 0 && !events))
-        return fail(OHM_EINVAL, "stage events: count must be 0..3 with a non-null array");
-    for (int k = 0; k < 3; ++k) g_stage_events[k] = k < count ? events[k] : nullptr;
+    if (count < 0 || count > kStageMarks || (count > 0 && !events))
+        return fail(OHM_EINVAL, "stage events: count must be 0..5 with a non-null array");
+    for (int k = 0; k < kStageMarks; ++k) g_stage_events[k] = k < count ? events[k] : nullptr;
     return OHM_OK;
 }
 

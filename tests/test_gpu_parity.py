@@ -198,6 +198,36 @@ def test_affine_log_domain_sweeps_vs_oracle(shape, params, monkeypatch):
     assert entropy_ok(h(c), ref[4], ref[0], T)
 
 
+@pytest.mark.parametrize("shape", [(3, 63, 65), (2, 257, 31), (2, 191, 193)], ids=lambda s: "x".join(map(str, s)))
+def test_affine_per_pair_workgroup_kernels_vs_oracle(shape, monkeypatch):
+    """The scaled sweeps' per-pair workgroup kernels (used for pairs too tall
+    for the wave-scheduled team kernels) against the oracle."""
+    monkeypatch.setenv("OHM_TEAMS", "0")
+    go, ge, T = PARAMS[1]
+    s, _, _, ln = _inputs(*shape, seed=hash((shape, "wg")) % 2**31)
+    case(f"affine per-pair workgroups shape={shape}")
+    v, m, a, b, c = torch.ops.orihime.soft_sw_affine_with_grads(g(s), go, ge, T, g(ln))
+    ref = O.sw_affine_forward(s, go, ge, T, ln)
+    assert rel_err(h(v), ref[0], 1e-3) < 1e-5
+    assert abs_err(h(m), ref[1]) < 1e-4
+    assert rel_err(h(a), ref[2], 1.0) < 1e-4 and rel_err(h(b), ref[3], 1.0) < 1e-4
+
+
+@pytest.mark.parametrize("shape", [(3, 63, 65), (2, 257, 31), (4, 191, 193)], ids=lambda s: "x".join(map(str, s)))
+def test_affine_team_kernels_vs_oracle(shape, monkeypatch):
+    """The wave-scheduled team kernels (chosen for batches of few large
+    pairs) against the oracle, forced on for small batches."""
+    monkeypatch.setenv("OHM_TEAMS", "1")
+    go, ge, T = PARAMS[0]
+    s, _, _, ln = _inputs(*shape, seed=hash((shape, "team")) % 2**31)
+    case(f"affine team kernels shape={shape}")
+    v, m, a, b, c = torch.ops.orihime.soft_sw_affine_with_grads(g(s), go, ge, T, g(ln))
+    ref = O.sw_affine_forward(s, go, ge, T, ln)
+    assert rel_err(h(v), ref[0], 1e-3) < 1e-5
+    assert abs_err(h(m), ref[1]) < 1e-4
+    assert rel_err(h(a), ref[2], 1.0) < 1e-4 and rel_err(h(b), ref[3], 1.0) < 1e-4
+
+
 def test_scaled_sweep_out_of_range_pair_is_redone_in_log_domain(monkeypatch):
     """A pair whose adjacent scores jump from -80 T to +80 T (inside the
     reference's domain) overflows the scaled sweep's exponent range; it is
