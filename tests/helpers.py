@@ -117,3 +117,10 @@ def close_or_nearer_truth(got, ref, truth, tol):
         return True
     t = truth()
     return abs_err(got, t) <= max(tol, 2.0 * abs_err(ref, t))
+
+
+def stable_seed(*parts) -> int:
+    """A seed from the test's parameters that is the same in every process
+    (Python's hash() of a str is salted per process)."""
+    import zlib
+    return zlib.crc32(repr(parts).encode()) & 0x7FFFFFFF

@@ -28,7 +28,7 @@ import torch
 
 import orihime_amd as ohm
 from orihime_amd import _lib, ragged
-from helpers import abs_err, affine_params, close_or_nearer_truth, golden_files, load, rel_err
+from helpers import stable_seed, abs_err, affine_params, close_or_nearer_truth, golden_files, load, rel_err
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -55,6 +55,19 @@ def tol2(ref):
 def entropy_ok(got, ref, value, T):
     tol = 1e-4 + 8.0 * (np.abs(np.asarray(value, np.float64)) + 1.0) * 2.0 ** -23 / T
     return bool(np.all(np.abs(np.asarray(got, np.float64) - ref) <= tol * np.maximum(1, np.abs(ref))))
+
+
+def entropy_close(got, ref, truth, value, T):
+    """grad_T = (S - (sum P s + go dS/dgo + ge dS/dge)) / T cancels: its FP32
+    noise grows with |S| / T.  Within the FP32 reference's noise, or at most
+    twice as far from the FP64 result (truth(), evaluated lazily) as the FP32
+    reference is."""
+    if entropy_ok(got, ref, value, T):
+        return True
+    t = np.asarray(truth(), np.float64)
+    got = np.asarray(got, np.float64)
+    tol = 1e-4 * np.maximum(1, np.abs(t))
+    return bool(np.all(np.abs(got - t) <= np.maximum(tol, 2 * np.abs(np.asarray(ref, np.float64) - t))))
 
 
 def report(name, got, f64):
@@ -151,7 +164,7 @@ def _inputs(B, L1, L2, seed, ragged_lengths=True):
 @pytest.mark.parametrize("params", PARAMS, ids=lambda p: "go%g_ge%g_T%g" % p)
 def test_affine_gpu_vs_oracle(shape, params):
     go, ge, T = params
-    s, V, G, ln = _inputs(*shape, seed=hash((shape, params)) % 2**31)
+    s, V, G, ln = _inputs(*shape, seed=stable_seed(shape, params))
     case(f"affine oracle shape={shape} params={params} lengths={ln.tolist()}")
     st, lt = g(s), g(ln)
     v, m, a, b, c = torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)
@@ -188,14 +201,15 @@ def test_affine_log_domain_sweeps_vs_oracle(shape, params, monkeypatch):
     scaled sweeps flag) against the oracle."""
     monkeypatch.setenv("OHM_SCALED", "0")
     go, ge, T = params
-    s, _, _, ln = _inputs(*shape, seed=hash((shape, params, "log")) % 2**31)
+    s, _, _, ln = _inputs(*shape, seed=stable_seed(shape, params, "log"))
     case(f"affine log-domain shape={shape} params={params}")
     v, m, a, b, c = torch.ops.orihime.soft_sw_affine_with_grads(g(s), go, ge, T, g(ln))
     ref = O.sw_affine_forward(s, go, ge, T, ln)
     assert rel_err(h(v), ref[0], 1e-3) < 1e-5
     assert abs_err(h(m), ref[1]) < 1e-4
     assert rel_err(h(a), ref[2], 1.0) < 1e-4 and rel_err(h(b), ref[3], 1.0) < 1e-4
-    assert entropy_ok(h(c), ref[4], ref[0], T)
+    f64 = lambda: O.sw_affine_forward(s.astype(np.float64), go, ge, T, ln, prec="f64")[4]
+    assert entropy_close(h(c), ref[4], f64, ref[0], T)
 
 
 @pytest.mark.parametrize("shape", [(3, 63, 65), (2, 257, 31), (2, 191, 193)], ids=lambda s: "x".join(map(str, s)))
@@ -204,7 +218,7 @@ def test_affine_per_pair_workgroup_kernels_vs_oracle(shape, monkeypatch):
     for the wave-scheduled team kernels) against the oracle."""
     monkeypatch.setenv("OHM_TEAMS", "0")
     go, ge, T = PARAMS[1]
-    s, _, _, ln = _inputs(*shape, seed=hash((shape, "wg")) % 2**31)
+    s, _, _, ln = _inputs(*shape, seed=stable_seed(shape, "wg"))
     case(f"affine per-pair workgroups shape={shape}")
     v, m, a, b, c = torch.ops.orihime.soft_sw_affine_with_grads(g(s), go, ge, T, g(ln))
     ref = O.sw_affine_forward(s, go, ge, T, ln)
@@ -219,7 +233,7 @@ def test_affine_team_kernels_vs_oracle(shape, monkeypatch):
     pairs) against the oracle, forced on for small batches."""
     monkeypatch.setenv("OHM_TEAMS", "1")
     go, ge, T = PARAMS[0]
-    s, _, _, ln = _inputs(*shape, seed=hash((shape, "team")) % 2**31)
+    s, _, _, ln = _inputs(*shape, seed=stable_seed(shape, "team"))
     case(f"affine team kernels shape={shape}")
     v, m, a, b, c = torch.ops.orihime.soft_sw_affine_with_grads(g(s), go, ge, T, g(ln))
     ref = O.sw_affine_forward(s, go, ge, T, ln)
@@ -257,7 +271,7 @@ def test_scaled_sweep_out_of_range_pair_is_redone_in_log_domain(monkeypatch):
 @pytest.mark.parametrize("params", [(-1.0, 1.0), (-0.4, 0.5), (-2.0, 2.0)], ids=lambda p: "g%g_T%g" % p)
 def test_linear_gpu_vs_oracle(shape, params):
     gp, T = params
-    s, V, G, ln = _inputs(*shape, seed=hash((shape, params, 1)) % 2**31)
+    s, V, G, ln = _inputs(*shape, seed=stable_seed(shape, params, 1))
     case(f"linear oracle shape={shape} params={params} lengths={ln.tolist()}")
     st, lt = g(s), g(ln)
     v, m, a, b = torch.ops.orihime.soft_sw_with_grads(st, gp, T, lt)
