@@ -319,6 +319,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-graph", action="store_true", help="time eager calls instead of a graph")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the automatic launch shape instead of autotuning it on the batch")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -380,6 +382,19 @@ def main():
         workload = ("long pairs affine soft-SW fwd+bwd, [16, 4096, 4096] N(0,1) (seed 3), "
                     "go=-11 ge=-1 T=1, checkpointed reverse sweep")
         cpu_pairs = [x[0].numpy()]
+
+    # launch shape: autotuned on this batch (orihime_amd.tune), results do
+    # not depend on it
+    tuned = None
+    if not args.no_tune:
+        from orihime_amd import tune
+        t0 = time.perf_counter()
+        cfg = tune.autotune(main_launch.scores, *params, lengths=main_launch.lengths,
+                            offsets=main_launch.offsets)
+        tuned = {"waves": cfg.waves, "groups_per_cu": cfg.groups_per_cu,
+                 "teams": {0: "auto", 1: "on", 2: "off"}[cfg.teams],
+                 "tuning_s": round(time.perf_counter() - t0, 2)}
+        log(f"autotuned launch shape: {tuned}")
 
     # warm-up, sanity, graph
     for _ in range(max(1, args.warmup)):
@@ -459,6 +474,10 @@ def main():
             pad[i, :p.shape[0], :p.shape[1]] = p
         pad = _normalize_masked(pad, params[2]).contiguous()
         padded = Launch(torch, _lib, pad, None, None, L1max, L2max, params)
+        if not args.no_tune:
+            pcfg = tune.autotune(pad, *params)
+            extra["padded_launch_config"] = {"waves": pcfg.waves, "groups_per_cu": pcfg.groups_per_cu,
+                                             "teams": pcfg.teams}
         for _ in range(max(1, args.warmup)):
             padded.step()
         torch.cuda.synchronize()
@@ -526,6 +545,7 @@ def main():
             "forward_gcups": round(cells / (fwd_ms * 1e-3) / 1e9, 3),
             "kernel_ms": {k: round(v, 4) for k, v in kd.items()},
             "hip_graph": use_graph,
+            "launch_config": tuned or "automatic",
             "wall_ms_per_step": round(wall_ms, 4),
             **extra,
         },

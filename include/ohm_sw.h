@@ -73,6 +73,16 @@ const char* ohm_version(void);
  * captured. */
 ohm_status ohm_set_stage_events(void* const* events, int32_t count);
 
+/* Launch-configuration hook (tuning; the reference has no counterpart, its
+ * launches have fixed shapes).  waves: waves per workgroup of the per-pair
+ * kernels (1, 2, 4, 8; 0 = automatic); groups_per_cu: persistent workgroups
+ * per CU (1..64; 0 = automatic); teams: 0 automatic, 1 wave-scheduled team
+ * kernels, 2 per-pair workgroup kernels (affine sweeps).  Results do not
+ * depend on the configuration, only the speed.  Process-wide and
+ * unsynchronised like ohm_set_stage_events; orihime_amd/tune.py chooses it
+ * by timing the caller's batch. */
+ohm_status ohm_set_launch_config(int32_t waves, int32_t groups_per_cu, int32_t teams);
+
 /* Operation kinds for the workspace queries. */
 #define OHM_OP_VALUE 0    /* value only */
 #define OHM_OP_FORWARD 1  /* value + map + parameter gradients */

@@ -36,6 +36,7 @@ EXPORTS = (
     "ohm_last_error",
     "ohm_version",
     "ohm_set_stage_events",
+    "ohm_set_launch_config",
     "ohm_sw_affine_workspace",
     "ohm_sw_workspace",
     "ohm_sw_affine_forward",
@@ -59,6 +60,7 @@ def _sig(name, restype, *argtypes):
 _last_error = _sig("ohm_last_error", ctypes.c_char_p)
 _version = _sig("ohm_version", ctypes.c_char_p)
 _stage_events = _sig("ohm_set_stage_events", ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), _i32)
+_launch_config = _sig("ohm_set_launch_config", ctypes.c_int, _i32, _i32, _i32)
 _ws_affine = _sig("ohm_sw_affine_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 _ws_linear = _sig("ohm_sw_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 
@@ -112,6 +114,11 @@ def set_stage_events(events) -> None:
     sequence clears the hook."""
     arr = (ctypes.c_void_p * 5)(*[(e or None) for e in events])
     _check(_stage_events(arr, len(events)), "ohm_set_stage_events")
+
+
+def set_launch_config(waves: int = 0, groups_per_cu: int = 0, teams: int = 0) -> None:
+    """Launch shape of the GPU kernels (0 = automatic; see include/ohm_sw.h)."""
+    _check(_launch_config(waves, groups_per_cu, teams), "ohm_set_launch_config")
 
 
 def workspace_bytes(affine: bool, op: int, B: int, L1: int, L2: int, total_cells: int = -1) -> int:

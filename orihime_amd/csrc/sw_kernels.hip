@@ -37,6 +37,8 @@
 // Numerics: FP32 max-shifted log-sum-exp in log2 units (see sw_common.h).
 
 #include <hip/hip_runtime.h>
+
+#include <atomic>
 #include <stdint.h>
 #include <stdio.h>
 
@@ -1131,15 +1133,19 @@ struct Geometry {
     int waves, groups;
 };
 
+// Launch configuration set through ohm_set_launch_config (0 = automatic);
+// it takes precedence over the OHM_* environment overrides.
+std::atomic<int> g_cfg_waves{0}, g_cfg_per_cu{0}, g_cfg_teams{0};
+
 Geometry choose_geometry(int B, int L1max) {
     const int cus = device_cus();
     const int strips = (L1max + kStrip - 1) / kStrip;
     int waves = (B >= 2 * cus) ? 4 : 8;
     while (waves > 1 && waves > strips) waves >>= 1;
-    int v = env_int("OHM_WAVES", 0);
+    int v = g_cfg_waves.load() ? g_cfg_waves.load() : env_int("OHM_WAVES", 0);
     if (v == 1 || v == 2 || v == 4 || v == 8) waves = v;
     int per_cu = 16 / waves;
-    v = env_int("OHM_WG_PER_CU", 0);
+    v = g_cfg_per_cu.load() ? g_cfg_per_cu.load() : env_int("OHM_WG_PER_CU", 0);
     if (v >= 1 && v <= 64) per_cu = v;
     long groups = (long)cus * per_cu;
     if (groups > B) groups = B;
@@ -1233,7 +1239,8 @@ size_t team_bwd_lds_bytes(int max_strips) {
     return team_lds_bytes(max_strips) + (size_t)kTeamWaves * kRing * kWave * sizeof(float);
 }
 bool use_teams(int B, int max_strips) {
-    const int forced = env_int("OHM_TEAMS", -1);
+    const int cfg = g_cfg_teams.load();
+    const int forced = cfg == 1 ? 1 : cfg == 2 ? 0 : env_int("OHM_TEAMS", -1);
     if (forced == 0) return false;
     if (team_bwd_lds_bytes(max_strips) + sizeof(TeamLds) + 1024 > kLdsLimit) return false;
     return forced == 1 || 4 * B < device_cus();
@@ -1375,6 +1382,18 @@ extern "C" ohm_status ohm_set_stage_events(void* const* events, int32_t count) {
     if (count < 0 || count > kStageMarks || (count > 0 && !events))
         return fail(OHM_EINVAL, "stage events: count must be 0..5 with a non-null array");
     for (int k = 0; k < kStageMarks; ++k) g_stage_events[k] = k < count ? events[k] : nullptr;
+    return OHM_OK;
+}
+
+extern "C" ohm_status ohm_set_launch_config(int32_t waves, int32_t groups_per_cu, int32_t teams) {
+    if (!(waves == 0 || waves == 1 || waves == 2 || waves == 4 || waves == 8))
+        return fail(OHM_EINVAL, "launch config: waves must be 0, 1, 2, 4 or 8");
+    if (groups_per_cu < 0 || groups_per_cu > 64)
+        return fail(OHM_EINVAL, "launch config: groups_per_cu must be 0..64");
+    if (teams < 0 || teams > 2) return fail(OHM_EINVAL, "launch config: teams must be 0, 1 or 2");
+    g_cfg_waves.store(waves);
+    g_cfg_per_cu.store(groups_per_cu);
+    g_cfg_teams.store(teams);
     return OHM_OK;
 }
 

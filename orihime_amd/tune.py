@@ -1,0 +1,119 @@
+"""Launch-shape autotuner for the GPU sweeps.
+
+The kernels' results do not depend on the launch shape (waves per workgroup,
+persistent workgroups per CU, per-pair workgroups or wave-scheduled team
+kernels) — only their speed does, and the best shape depends on the batch:
+many mid-size pairs favour 4 waves per pair, few long pairs the team
+kernels.  ``autotune`` times a small set of candidate shapes on the caller's
+own batch (one forward+backward call each, after a warm-up), applies the
+fastest through ``ohm_set_launch_config`` and caches it per batch signature
+(pair count bucket, maximum lengths, layout), so a training loop pays the
+search once.
+
+    cfg = orihime_amd.tune.autotune(scores, 2.0, -0.5, 1.0, lengths=lens)
+
+``reset()`` returns to the automatic choice.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+
+# candidate shapes: (waves, groups per CU, teams) with teams 1 = on, 2 = off
+CANDIDATES = (
+    (4, 0, 2),
+    (4, 8, 2),
+    (2, 0, 2),
+    (8, 0, 2),
+    (0, 0, 1),
+)
+
+
+@dataclass(frozen=True)
+class LaunchConfig:
+    waves: int
+    groups_per_cu: int
+    teams: int
+    ms: float
+
+
+_cache: dict[tuple, LaunchConfig] = {}
+
+
+def _signature(scores, lengths, offsets, affine):
+    B = int(lengths.shape[0]) if lengths is not None else int(scores.shape[0])
+    if offsets is not None:
+        L1max = int(lengths[:, 0].max())
+        L2max = int(lengths[:, 1].max())
+    else:
+        L1max, L2max = int(scores.shape[-2]), int(scores.shape[-1])
+    bucket = 1 << max(0, (B - 1).bit_length())
+    return (affine, offsets is not None, bucket, L1max, L2max, scores.device.index)
+
+
+def _time(fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        best = min(best, (time.perf_counter() - t0) * 1e3)
+    return best
+
+
+def autotune(scores, gap_open, gap_ext=None, temperature=1.0, *, lengths=None, offsets=None,
+             reps: int = 3, candidates=CANDIDATES, apply: bool = True) -> LaunchConfig:
+    """Choose the fastest launch shape for this batch (affine when ``gap_ext``
+    is given, linear otherwise) and apply it.  ``scores`` is a GPU tensor in
+    the padded ([B, L1, L2]) or packed layout (``offsets``, see
+    orihime_amd.ragged)."""
+    if not scores.is_cuda:
+        raise ValueError("autotune needs GPU tensors")
+    affine = gap_ext is not None
+    key = _signature(scores, lengths, offsets, affine)
+    if key in _cache:
+        cfg = _cache[key]
+    else:
+        B = int(lengths.shape[0]) if lengths is not None else int(scores.shape[0])
+        L1max, L2max = key[3], key[4]
+        total = -1 if offsets is None else int(scores.numel())
+        ws = torch.empty(_lib.workspace_bytes(affine, _lib.OP_FORWARD, B, L1max, L2max, total),
+                         dtype=torch.uint8, device=scores.device)
+        value = torch.empty(B, device=scores.device)
+        mp = torch.empty_like(scores)
+        g0, g1, g2 = (torch.empty(B, device=scores.device) for _ in range(3))
+
+        def call():
+            if affine:
+                _lib.affine_forward(scores, lengths, offsets, gap_open, gap_ext, temperature,
+                                    value=value, map=mp, grad_open=g0, grad_ext=g1, grad_temp=g2,
+                                    L1max=L1max, L2max=L2max, workspace=ws)
+            else:
+                _lib.linear_forward(scores, lengths, offsets, gap_open, temperature, value=value,
+                                    map=mp, grad_gap=g0, grad_temp=g1, L1max=L1max, L2max=L2max,
+                                    workspace=ws)
+
+        results = []
+        try:
+            for w, g, t in candidates:
+                _lib.set_launch_config(w, g, t)
+                results.append(LaunchConfig(w, g, t, _time(call, reps)))
+        finally:
+            _lib.set_launch_config()
+        cfg = min(results, key=lambda c: c.ms)
+        _cache[key] = cfg
+    if apply:
+        _lib.set_launch_config(cfg.waves, cfg.groups_per_cu, cfg.teams)
+    return cfg
+
+
+def reset() -> None:
+    """Back to the automatic launch shape (the cache is kept)."""
+    _lib.set_launch_config()

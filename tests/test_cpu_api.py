@@ -35,6 +35,29 @@ def test_library_exports_every_declared_symbol():
     assert ohm.__version__ == _lib.version()
 
 
+def test_probe_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "ohm_probe.h")).read()
+    declared = set(re.findall(r"^\S.*?\b(ohm_\w+)\(", header, re.M))
+    assert declared == {"ohm_probe_ops_per_thread", "ohm_probe_launch"}
+    lib = ctypes.CDLL(os.path.join(ROOT, "orihime_amd", "libohm_probe.so"))
+    for name in declared:
+        assert hasattr(lib, name), name
+    # host-side argument checks (no device needed)
+    lib.ohm_probe_ops_per_thread.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+    ops = ctypes.c_int64()
+    assert lib.ohm_probe_ops_per_thread(4, 10, ctypes.byref(ops)) == 0 and ops.value == 400
+    assert lib.ohm_probe_ops_per_thread(9, 10, ctypes.byref(ops)) == 1
+
+
+def test_launch_config_hook_validates_and_resets():
+    with pytest.raises(RuntimeError):
+        _lib.set_launch_config(waves=3)
+    with pytest.raises(RuntimeError):
+        _lib.set_launch_config(teams=5)
+    _lib.set_launch_config(4, 2, 2)
+    _lib.set_launch_config()
+
+
 def test_gpu_entry_points_reject_bad_arguments_without_a_device():
     # argument checks run before any HIP call
     lib = _lib._lib

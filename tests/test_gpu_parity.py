@@ -242,6 +242,29 @@ def test_affine_team_kernels_vs_oracle(shape, monkeypatch):
     assert rel_err(h(a), ref[2], 1.0) < 1e-4 and rel_err(h(b), ref[3], 1.0) < 1e-4
 
 
+def test_autotuned_launch_shape_changes_speed_not_results():
+    """The autotuner (orihime_amd.tune) times candidate launch shapes on the
+    batch and applies the fastest.  Maps are bitwise identical under every
+    shape (each cell's arithmetic does not depend on it); per-pair sums are
+    combined per wave or per strip, so values and gradients agree to FP32
+    rounding."""
+    from orihime_amd import tune
+    go, ge, T = PARAMS[1]
+    s, _, _, ln = _inputs(6, 150, 170, seed=stable_seed("tune"))
+    st, lt = g(s), g(ln)
+    case("autotune")
+    base = [t.clone() for t in torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)]
+    try:
+        cfg = tune.autotune(st, go, ge, T, lengths=lt)
+        assert cfg.ms > 0 and (cfg.waves, cfg.groups_per_cu, cfg.teams) in [c[:3] for c in tune.CANDIDATES]
+        out = torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)
+    finally:
+        tune.reset()
+    assert torch.equal(out[1], base[1])
+    for k in (0, 2, 3, 4):
+        assert rel_err(h(out[k]), h(base[k]), 1.0) < 1e-5
+
+
 def test_scaled_sweep_out_of_range_pair_is_redone_in_log_domain(monkeypatch):
     """A pair whose adjacent scores jump from -80 T to +80 T (inside the
     reference's domain) overflows the scaled sweep's exponent range; it is
