@@ -309,12 +309,67 @@ def cpu_baseline(torch, pairs, params, seconds):
 # --------------------------------------------------------------------------
 
 
+def embed_workload(torch, args, dev):
+    """Embedding-scored alignment end to end: Q [512, M, 128], K [512, N, 128]
+    BF16 N(0,1)/sqrt(128), M, N ~ U[128, 768] (seed 4); scores Q K^T by a BF16
+    GEMM with FP32 accumulation, ragged soft-SW affine (go=-4, ge=-0.5,
+    T=0.25, per-pair lengths on padded storage), dQ and dK through the map.
+    A step is the forward (GEMM + sweeps) and the backward (dQ, dK GEMMs)."""
+    from orihime_amd.embed import soft_align_embeddings
+    gen = torch.Generator().manual_seed(4)
+    B, Lmax, D = 512, 768, 128
+    ln = torch.randint(128, Lmax + 1, (B, 2), generator=gen).to(torch.int32)
+    Q = (torch.randn(B, Lmax, D, generator=gen) / D ** 0.5).to(torch.bfloat16).to(dev)
+    K = (torch.randn(B, Lmax, D, generator=gen) / D ** 0.5).to(torch.bfloat16).to(dev)
+    lnd = ln.to(dev)
+    cells = int((ln[:, 0].long() * ln[:, 1].long()).sum())
+    Q.requires_grad_(True)
+    K.requires_grad_(True)
+
+    def step():
+        Q.grad = None
+        K.grad = None
+        soft_align_embeddings(Q, K, lnd, -4.0, -0.5, 0.25).sum().backward()
+
+    tuned = "automatic"
+    if not args.no_tune:
+        from orihime_amd import tune
+        from orihime_amd.embed import pair_scores
+        with torch.no_grad():
+            cfg = tune.autotune(pair_scores(Q, K), -4.0, -0.5, 0.25, lengths=lnd)
+        tuned = {"waves": cfg.waves, "groups_per_cu": cfg.groups_per_cu,
+                 "teams": {0: "auto", 1: "on", 2: "off"}[cfg.teams]}
+        log(f"autotuned launch shape: {tuned}")
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    ms, wall = timed(torch, step, args.steps)
+    gflop = 3 * 2.0 * B * Lmax * Lmax * D / 1e9  # scores, dQ, dK on padded storage
+    gcups = cells / (ms * 1e-3) / 1e9
+    log(f"embedding-scored fwd+bwd: {ms:.3f} ms/step, {gcups:.1f} GCUPS (sweep cells), "
+        f"{gflop / ms:.1f} TFLOP/s of GEMM work per step time")
+    line = {
+        "metric": "GCUPS embedding-scored soft-SW-affine fwd+bwd (Q K^T, sweeps, dQ/dK), 1 GPU",
+        "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 GEMM inputs, f32 sweeps",
+        "data": "synthetic",
+        "config": {"workload": "embedding-scored ragged soft-SW affine, Q/K [512, <=768, 128] "
+                               "BF16, M,N ~ U[128,768] (seed 4), go=-4 ge=-0.5 T=0.25",
+                   "cells": cells, "gemm_gflop_per_step": round(gflop, 1), "launch_config": tuned,
+                   "wall_ms_per_step": round(wall, 4)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("ragged", "fixed", "long"), default="ragged")
+    ap.add_argument("--workload", choices=("ragged", "fixed", "long", "embed"), default="ragged")
     ap.add_argument("--no-padded", action="store_true", help="skip the padded comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -371,6 +426,8 @@ def main():
         main_launch = Launch(torch, _lib, x.to(dev), None, None, 512, 512, params)
         workload = "fixed-shape affine soft-SW fwd+bwd, [256, 512, 512] (seed 1), go=-11 ge=-1 T=1"
         cpu_pairs = [x[i].numpy() for i in range(x.shape[0])]
+    elif args.workload == "embed":
+        return embed_workload(torch, args, dev)
     else:
         params = (-11.0, -1.0, 1.0)
         gen = torch.Generator().manual_seed(3)

@@ -419,3 +419,29 @@ def test_ragged_packed_matches_padded_lengths():
     fr = flat.clone().requires_grad_(True)
     ohm.ragged.sw_affine_value(fr, ln, off, **KW).sum().backward()
     assert torch.allclose(fr.grad, m, atol=1e-7)
+
+
+def _embed_truth(Q, K, ln, go, ge, T):
+    """FP64 oracle of the embedding-scored alignment: scores Q K^T in float64,
+    the oracle's value and map, dQ = P K and dK = P^T Q per pair."""
+    Q64, K64 = Q.double().numpy(), K.double().numpy()
+    S = np.einsum("bmd,bnd->bmn", Q64, K64)
+    out = O.sw_affine_forward(S, go, ge, T, ln.numpy(), prec="f64")
+    P = out[1]
+    return out[0], np.einsum("bmn,bnd->bmd", P, K64), np.einsum("bmn,bmd->bnd", P, Q64)
+
+
+def test_embedding_scored_alignment_cpu_matches_fp64_oracle():
+    from orihime_amd.embed import soft_align_embeddings
+    g = torch.Generator().manual_seed(4)
+    B, M, N, D = 3, 40, 50, 16
+    go, ge, T = -4.0, -0.5, 0.25
+    Q = (torch.randn(B, M, D, generator=g) / D ** 0.5).requires_grad_()
+    K = (torch.randn(B, N, D, generator=g) / D ** 0.5).requires_grad_()
+    ln = torch.tensor([[40, 50], [31, 22], [7, 50]], dtype=torch.int32)
+    v = soft_align_embeddings(Q, K, ln, go, ge, T)
+    v.sum().backward()
+    val, dQ, dK = _embed_truth(Q.detach(), K.detach(), ln, go, ge, T)
+    assert rel_err(v.detach().numpy(), val, 1e-3) < 1e-5
+    assert abs_err(Q.grad.numpy(), dQ) < 1e-4 and abs_err(K.grad.numpy(), dK) < 1e-4
+    assert float(Q.grad[1, 31:].abs().max()) == 0.0 and float(K.grad[1, 22:].abs().max()) == 0.0

@@ -547,3 +547,32 @@ def test_autograd_through_gpu_kernels():
     (m * g(G)).sum().backward()
     full = O.sw_affine_backward_full(s.astype(np.float64), G, -1.2, -0.4, 0.8, prec="f64")
     assert abs_err(h(x.grad), full[0]) < 1e-4 * max(1.0, np.abs(full[0]).max())
+
+
+def test_embedding_scored_alignment_bf16_matches_fp64_oracle():
+    """C5-style end to end on the GPU: BF16 Q, K; scores Q K^T by a BF16 GEMM
+    with FP32 accumulation; ragged soft-SW affine with per-pair lengths;
+    dQ, dK through the map.  Checked against the FP64 oracle on the same BF16
+    values (scores formed exactly in float64)."""
+    from orihime_amd.embed import soft_align_embeddings
+    go, ge, T = -4.0, -0.5, 0.25
+    gen = torch.Generator().manual_seed(4)
+    B, M, N, D = 6, 150, 170, 128
+    Qc = (torch.randn(B, M, D, generator=gen) / D ** 0.5).to(torch.bfloat16)
+    Kc = (torch.randn(B, N, D, generator=gen) / D ** 0.5).to(torch.bfloat16)
+    ln = torch.stack([torch.randint(1, M + 1, (B,), generator=gen),
+                      torch.randint(1, N + 1, (B,), generator=gen)], 1).to(torch.int32)
+    case(f"embedding-scored alignment B={B} M={M} N={N} lengths={ln.tolist()}")
+    Q, K = Qc.to(DEV).requires_grad_(), Kc.to(DEV).requires_grad_()
+    v = soft_align_embeddings(Q, K, ln.to(DEV), go, ge, T)
+    v.sum().backward()
+    Q64, K64 = Qc.double().numpy(), Kc.double().numpy()
+    S = np.einsum("bmd,bnd->bmn", Q64, K64)
+    ref = O.sw_affine_forward(S, go, ge, T, ln.numpy(), prec="f64")
+    dQ = np.einsum("bmn,bnd->bmd", ref[1], K64)
+    dK = np.einsum("bmn,bmd->bnd", ref[1], Q64)
+    assert rel_err(h(v), ref[0], 1e-3) < 1e-5
+    # Q.grad / K.grad are BF16 (the inputs' dtype): 2^-8 relative rounding
+    assert abs_err(h(Q.grad.float()), dQ) < 4e-3 * max(1.0, float(np.abs(dQ).max()))
+    assert abs_err(h(K.grad.float()), dK) < 4e-3 * max(1.0, float(np.abs(dK).max()))
+    report("embed value", h(v), ref[0])
