@@ -290,6 +290,38 @@ def test_scaled_sweep_out_of_range_pair_is_redone_in_log_domain(monkeypatch):
     assert rel_err(h(out[2]), ref[2], 1.0) < 1e-4 and rel_err(h(out[3]), ref[3], 1.0) < 1e-4
 
 
+def test_masked_padding_stays_in_the_scaled_sweep():
+    """Padded storage masked with -inf (replaced by the sentinel far below the
+    scores, orihime/operator.py:334-386): masked cells have E = 0 and, hundreds
+    of rows or columns past a pair's end, neighbours decayed below 2^-127.
+    The scaled sweep keeps such pairs (no range flag, so no log-domain redo)
+    and matches the oracle on the same sentinel-filled scores; masked cells
+    carry no probability (the floor of their factor E is 2^-140)."""
+    from orihime_amd.operator import _normalize_masked
+    go, ge, T = -8.0, -0.5, 0.5
+    rng = np.random.default_rng(stable_seed("masked padding"))
+    B, L1, L2 = 3, 300, 420
+    s = rng.normal(size=(B, L1, L2)).astype(np.float32)
+    for b, (m, n) in enumerate([(300, 420), (90, 400), (250, 60)]):
+        s[b, m:, :] = -np.inf
+        s[b, :, n:] = -np.inf
+    case("masked padding, scaled sweep range")
+    st = _normalize_masked(g(s), T).contiguous()
+    ws = torch.zeros(_lib.workspace_bytes(True, _lib.OP_FORWARD, B, L1, L2), dtype=torch.uint8, device=DEV)
+    out = [torch.empty(B, device=DEV), torch.empty_like(st)] + [torch.empty(B, device=DEV) for _ in range(3)]
+    _lib.affine_forward(st, None, None, go, ge, T, value=out[0], map=out[1], grad_open=out[2],
+                        grad_ext=out[3], grad_temp=out[4], workspace=ws)
+    torch.cuda.synchronize()
+    flagged = int(ws[:256].view(torch.int32)[8].item())
+    assert flagged == 0, f"{flagged} masked pairs fell back to the log-domain sweeps"
+    ref = O.sw_affine_forward(h(st), go, ge, T)
+    assert rel_err(h(out[0]), ref[0], 1e-3) < 1e-5
+    assert abs_err(h(out[1]), ref[1]) < 1e-4
+    assert float(out[1][1, 90:].abs().max()) < 1e-30 and float(out[1][2, :, 60:].abs().max()) < 1e-30
+    assert rel_err(h(out[2]), ref[2], 1.0) < 1e-4 and rel_err(h(out[3]), ref[3], 1.0) < 1e-4
+    assert entropy_ok(h(out[4]), ref[4], ref[0], T)
+
+
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("params", [(-1.0, 1.0), (-0.4, 0.5), (-2.0, 2.0)], ids=lambda p: "g%g_T%g" % p)
 def test_linear_gpu_vs_oracle(shape, params):
