@@ -480,6 +480,42 @@ def test_long_pair_forward_backward_and_hvp():
     assert close_or_nearer_truth(h(hv), r, truth, tol2(r))
 
 
+def test_long_pair_gradients_vs_fp64_oracle_and_central_differences():
+    """The first pair of the 16 x 4096 x 4096 batch (N(0,1), seed 3, go=-11,
+    ge=-1, T=1): value, map on a central 64 x 64 crop and the parameter
+    gradients against the FP64 oracle of the whole pair; then that crop as a
+    pair of its own, its map and parameter gradients against FP64 central
+    finite differences of the oracle value (step 1e-3)."""
+    case("affine 1x4096x4096 seed 3 vs FP64 oracle; 64x64 crop vs FP64 central differences")
+    gen = torch.Generator().manual_seed(3)
+    s = torch.randn(1, 4096, 4096, generator=gen)
+    go, ge, T = -11.0, -1.0, 1.0
+    out = [h(x) for x in torch.ops.orihime.soft_sw_affine_with_grads(s.to(DEV), go, ge, T, None)]
+    sl = slice(2016, 2080)
+    f64 = O.sw_affine_forward(s.numpy().astype(np.float64), go, ge, T, prec="f64")
+    assert rel_err(out[0], f64[0], 1e-3) < 1e-5
+    assert abs_err(out[1][0, sl, sl], f64[1][0, sl, sl]) < 1e-4
+    for k in (2, 3, 4):
+        assert rel_err(out[k], f64[k], 1.0) < 1e-4, k
+    report("value (4096^2)", out[0], f64[0])
+    report("map crop (4096^2)", out[1][0, sl, sl], f64[1][0, sl, sl])
+
+    crop = s[:, sl, sl].contiguous()
+    got = [h(x) for x in torch.ops.orihime.soft_sw_affine_with_grads(crop.to(DEV), go, ge, T, None)]
+    x = crop.numpy().astype(np.float64)
+    step = 1e-3
+    bump = np.eye(64 * 64).reshape(64 * 64, 64, 64) * step
+    val = lambda sc, *p: O.sw_affine_forward(sc, *p, prec="f64")[0]
+    fd_map = ((val(x + bump, go, ge, T) - val(x - bump, go, ge, T)) / (2 * step)).reshape(64, 64)
+    assert abs_err(got[1][0], fd_map) < 1e-4
+    report("map crop vs central differences", got[1][0], fd_map)
+    for k, dp in ((2, (step, 0, 0)), (3, (0, step, 0)), (4, (0, 0, step))):
+        hi = val(x, go + dp[0], ge + dp[1], T + dp[2])
+        lo = val(x, go - dp[0], ge - dp[1], T - dp[2])
+        fd = (hi - lo) / (2 * step)
+        assert rel_err(got[k], fd, 1.0) < 1e-4, (k, got[k], fd)
+
+
 def test_c3_style_ragged_batch_sampled():
     """Lengths uniform in [64, 1024] (go=-8, ge=-0.5, T=0.5), packed, 96 pairs."""
     pairs = _ragged_pairs(96, 64, 1024, seed=2)
