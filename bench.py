@@ -338,7 +338,7 @@ def embed_workload(torch, args, dev):
         with torch.no_grad():
             cfg = tune.autotune(pair_scores(Q, K), -4.0, -0.5, 0.25, lengths=lnd)
         tuned = {"waves": cfg.waves, "groups_per_cu": cfg.groups_per_cu,
-                 "teams": {0: "auto", 1: "on", 2: "off"}[cfg.teams]}
+                 "teams": {0: "auto", 1: "on", 2: "off", 3: "grid"}[cfg.teams]}
         log(f"autotuned launch shape: {tuned}")
 
     for _ in range(max(1, args.warmup)):
@@ -449,7 +449,7 @@ def main():
         cfg = tune.autotune(main_launch.scores, *params, lengths=main_launch.lengths,
                             offsets=main_launch.offsets)
         tuned = {"waves": cfg.waves, "groups_per_cu": cfg.groups_per_cu,
-                 "teams": {0: "auto", 1: "on", 2: "off"}[cfg.teams],
+                 "teams": {0: "auto", 1: "on", 2: "off", 3: "grid"}[cfg.teams],
                  "tuning_s": round(time.perf_counter() - t0, 2)}
         log(f"autotuned launch shape: {tuned}")
 

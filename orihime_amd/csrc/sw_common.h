@@ -84,7 +84,7 @@ inline int64_t bnd_bound(int64_t N, int64_t B, int64_t L2max) {
 // and its strip-boundary block are contiguous inside one data region whose
 // offsets the plan kernel computes exactly; `data` bytes bound the region.
 struct WsLayout {
-    size_t claims, desc, order, stil, tgt, pst, data, total;
+    size_t claims, desc, order, stil, tgt, pst, grid, data, total;
 };
 
 // Per-pair state words of the scaled sweeps (4 int32 per pair).
@@ -94,11 +94,26 @@ constexpr int kPstZinv = 2;   // bits of 1 / zm
 // Slots of the claims block (64 ints, zeroed by the plan kernel): pair claim
 // counters of the launches, and the count of flagged pairs.
 constexpr int kClaimFlagged = 8;
+constexpr int kClaimGridFwd = 10;   // strip tickets of the grid-scheduled sweeps
+constexpr int kClaimGridRev = 11;
+constexpr int kClaimGridStall = 12;  // set when a grid hand-off wait exceeded its bound
+
+// Grid scheduling (strips of a pair on waves of any workgroup, hand-off
+// through global memory) for batches of at most kGridMaxPairs pairs.  Its
+// region: forward and reverse progress words [2][B][max_strips], per-pair
+// strip completion counts [2][B] (zeroed by the plan kernel), then per-strip
+// partial results [B][max_strips][4] floats.
+constexpr int kGridMaxPairs = 64;
+inline int64_t grid_zero_ints(int64_t B, int64_t max_strips) { return 2 * B * max_strips + 2 * B; }
+inline size_t grid_bytes(int64_t B, int64_t max_strips) {
+    if (B > kGridMaxPairs) return 0;
+    return (size_t)(grid_zero_ints(B, max_strips) + B * max_strips * 4) * 4;
+}
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-inline size_t ws_fixed(int64_t B, size_t* claims, size_t* desc, size_t* order, size_t* stil,
-                       size_t* tgt, size_t* pst) {
+inline size_t ws_fixed(int64_t B, int64_t max_strips, size_t* claims, size_t* desc, size_t* order,
+                       size_t* stil, size_t* tgt, size_t* pst, size_t* grid) {
     size_t o = 0;
     *claims = o; o += 256;
     *desc = o; o = align256(o + (size_t)B * sizeof(PairDesc));
@@ -106,6 +121,7 @@ inline size_t ws_fixed(int64_t B, size_t* claims, size_t* desc, size_t* order, s
     *stil = o; o = align256(o + (size_t)B * sizeof(float));
     *tgt = o; o = align256(o + (size_t)B * 4 * sizeof(float));
     *pst = o; o = align256(o + (size_t)B * 4 * sizeof(int32_t));
+    *grid = o; o = align256(o + grid_bytes(B, max_strips));
     return o;
 }
 
@@ -114,7 +130,9 @@ inline WsLayout ws_layout(int op, int64_t B, int64_t L1max, int64_t L2max, int64
                           TableSpec fwd, TableSpec tan) {
     if (N < 0) N = B * L1max * L2max;
     WsLayout w;
-    size_t o = ws_fixed(B, &w.claims, &w.desc, &w.order, &w.stil, &w.tgt, &w.pst);
+    const int64_t max_strips = L1max > 0 ? (L1max + kStrip - 1) / kStrip : 1;
+    size_t o = ws_fixed(B, max_strips, &w.claims, &w.desc, &w.order, &w.stil, &w.tgt, &w.pst,
+                        &w.grid);
     w.data = o;
     const TableSpec none = {0, 1};
     const TableSpec t = op == 0 ? none : (op == 1 ? fwd : tan);

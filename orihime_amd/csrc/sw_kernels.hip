@@ -292,6 +292,56 @@ DEV void publish(int* flag, int value, int lane) {
     if (lane == 0) __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Hand-off between waves of different workgroups (grid scheduling).  The
+// progress words and boundary entries are agent-scope atomic loads and stores
+// (global_load/store sc1: coherent per access across CUs and XCDs, without
+// the cache-wide write-back and invalidate of an agent-scope fence).  A
+// producer waits until its own stores have completed (vmcnt(0); stores count
+// in vmcnt on gfx9) before it publishes progress; a consumer reads boundary
+// entries only after it has seen the progress word.  The wait is bounded:
+// past kGridSpinCap polls it raises claims[kClaimGridStall] and returns (the
+// outputs are then NaN; by the ticket order every wait ends well before).
+constexpr int kGridSpinCap = 1 << 24;
+DEV float cload(const float* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+DEV void cstore(float* p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+DEV void wait_ready_grid(const int* flag, int need, int* stall) {
+    for (int n = 0; __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need; ++n) {
+        if (n > kGridSpinCap) {
+            __hip_atomic_store(stall, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    asm volatile("" ::: "memory");  // boundary loads stay after the wait
+}
+DEV void publish_grid(int* flag, int value, int lane) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if (lane == 0) __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Strip hand-off policy: progress words in LDS (one workgroup) or in global
+// memory (grid scheduling).
+template <bool GRID>
+DEV void hand_wait(int* flag, int need, int* stall) {
+    if constexpr (GRID) wait_ready_grid(flag, need, stall);
+    else wait_ready(flag, need);
+}
+template <bool GRID>
+DEV void hand_publish(int* flag, int value, int lane) {
+    if constexpr (GRID) publish_grid(flag, value, lane);
+    else publish(flag, value, lane);
+}
+template <bool GRID>
+DEV float hand_load(const float* p) {
+    if constexpr (GRID) return cload(p);
+    else return *p;
+}
+template <bool GRID>
+DEV void hand_store(float* p, float v) {
+    if constexpr (GRID) cstore(p, v);
+    else *p = v;
+}
+
 // Kahan add of a chunk partial into a running total.
 DEV void kadd(float& sum, float& comp, float v) {
     float y = v - comp;
@@ -318,13 +368,16 @@ constexpr int kSortMax = 2048;
 __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
     const int32_t* __restrict__ lengths, const int64_t* __restrict__ offsets, int B, int L1max,
     int L2max, int planes, int tstride, int64_t cap, PairDesc* __restrict__ desc,
-    int32_t* __restrict__ order, int* __restrict__ claims, float* __restrict__ value) {
+    int32_t* __restrict__ order, int* __restrict__ claims, float* __restrict__ value,
+    int* __restrict__ grid, int grid_ints) {
     __shared__ int64_t s_t[kPlanThreads];
     __shared__ int64_t s_carry[1];
     __shared__ float s_key[kSortMax];
     __shared__ int s_idx[kSortMax];
     const int tid = tid_x();
     if (tid < 64) claims[tid] = 0;
+    if (grid)
+        for (int q = tid; q < grid_ints; q += kPlanThreads) grid[q] = 0;
     if (tid == 0) s_carry[0] = 0;
     __syncthreads();
     const bool sort = B <= kSortMax;
@@ -456,6 +509,7 @@ struct SwArgs {
     float lgo, lge;          // 2^go2, 2^ge2: gap factors of the scaled sweeps
     int* pst;                // per pair: kPst* state words of the scaled sweeps
     int sel;                 // log-domain affine sweeps: 0 every pair, 1 only flagged pairs
+    int* grid;               // grid-scheduling region (null for more than kGridMaxPairs pairs)
 };
 
 // Claim the next pair for this workgroup (largest first).  Returns -1 when done.
@@ -875,6 +929,7 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
 }
 
 #include "sw_teams.hiph"
+#include "sw_grid.hiph"
 #include "sw_affine_scaled.hiph"
 
 // ==========================================================================
@@ -1209,9 +1264,11 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     L.stream = static_cast<hipStream_t>(stream);
     const TableSpec ts = op == 0 ? TableSpec{0, 1} : (op == 1 ? tf : tt);
     const int64_t cap = (int64_t)((ws_bytes - w.data) / sizeof(float));
+    if (B <= kGridMaxPairs) a.grid = reinterpret_cast<int*>(base + w.grid);
     hipLaunchKernelGGL(sw_plan_kernel, dim3(1), dim3(kPlanThreads), 0, L.stream, lengths, offsets,
                        B, L1max, L2max, ts.planes, ts.stride, cap, const_cast<PairDesc*>(a.desc),
-                       const_cast<int32_t*>(a.order), a.claims, (float*)nullptr);
+                       const_cast<int32_t*>(a.order), a.claims, (float*)nullptr, a.grid,
+                       a.grid ? (int)grid_zero_ints(B, a.max_strips) : 0);
     stage_mark(0, L.stream);
     return check_launch("plan kernel");
 }
@@ -1255,6 +1312,37 @@ ohm_status launch_team(K kernel, size_t lds, const Launch& L) {
     return check_launch("team kernel");
 }
 
+// Grid kernels (strips of every pair on the waves of all workgroups,
+// hand-off through global memory) for batches of few long pairs, which the
+// workgroup and team kernels would leave on a few CUs.  teams = 3 in
+// ohm_set_launch_config or OHM_GRID=0/1 force the choice.
+size_t grid_bwd_lds_bytes(int max_strips) {
+    return grid_cum_bytes(max_strips) + (size_t)kGridWaves * kRing * kWave * sizeof(float);
+}
+// Measured on 16 pairs of 4096 x 4096: grid 6.7 ms, team kernels 21 ms.
+bool use_grid(const SwArgs& a) {
+    if (!a.grid) return false;
+    if (grid_bwd_lds_bytes(a.max_strips) + sizeof(GridLds) + 1024 > kLdsLimit) return false;
+    const int cfg = g_cfg_teams.load();
+    if (cfg == 1 || cfg == 2) return false;
+    const int forced = cfg == 3 ? 1 : env_int("OHM_GRID", -1);
+    if (forced >= 0) return forced == 1;
+    if (env_int("OHM_TEAMS", -1) >= 0) return false;  // an explicit team / workgroup choice
+    return 4 * a.B < device_cus() && a.max_strips >= 4;
+}
+template <typename K>
+ohm_status launch_grid(K kernel, size_t lds, const Launch& L) {
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return fail(OHM_ELAUNCH, "cannot reserve LDS for the grid kernel");
+    // one wave per strip of the batch, at most 16 waves per CU
+    const long want = ((long)L.a.B * L.a.max_strips + kGridWaves - 1) / kGridWaves;
+    const long groups = std::max(1L, std::min(want, (long)device_cus() * (16 / kGridWaves)));
+    hipLaunchKernelGGL(kernel, dim3((unsigned)groups), dim3(64 * kGridWaves), lds, L.stream, L.a);
+    return check_launch("grid kernel");
+}
+
 // The scaled affine sweeps need representable gap factors 2^(g log2(e) / T)
 // with room for products of two of them and a neighbour ratio; outside that
 // (or with OHM_SCALED=0) the log-domain sweeps run alone.
@@ -1279,12 +1367,18 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     L.a.g1 = g1;
     L.a.g2 = g2;
     const bool scaled = affine && use_scaled(L.a);
-    const bool team = scaled && use_teams(B, L.a.max_strips);
+    const bool grid = scaled && use_grid(L.a);
+    const bool team = scaled && !grid && use_teams(B, L.a.max_strips);
     if (scaled) {
         // scaled sweep for every pair, then the log-domain sweep for the pairs
         // it flagged (a launch whose groups only walk the claim list when
         // none are)
-        if (team) {
+        if (grid) {
+            const size_t lds = grid_cum_bytes(L.a.max_strips);
+            if (need_bwd) st = launch_grid(sw_affine_scaled_forward_grid<true>, lds, L);
+            else st = launch_grid(sw_affine_scaled_forward_grid<false>, lds, L);
+            if (st != OHM_OK) return st;
+        } else if (team) {
             const size_t lds = team_lds_bytes(L.a.max_strips);
             if (need_bwd) st = launch_team(sw_affine_scaled_forward_team<true>, lds, L);
             else st = launch_team(sw_affine_scaled_forward_team<false>, lds, L);
@@ -1310,7 +1404,10 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     if (!need_bwd) return OHM_OK;
     if (scaled) {
         L.a.sel = 0;
-        if (team) {
+        if (grid) {
+            if ((st = launch_grid(sw_affine_scaled_backward_grid, grid_bwd_lds_bytes(L.a.max_strips), L)) != OHM_OK)
+                return st;
+        } else if (team) {
             if ((st = launch_team(sw_affine_scaled_backward_team, team_bwd_lds_bytes(L.a.max_strips), L)) != OHM_OK)
                 return st;
         } else {
@@ -1390,7 +1487,7 @@ extern "C" ohm_status ohm_set_launch_config(int32_t waves, int32_t groups_per_cu
         return fail(OHM_EINVAL, "launch config: waves must be 0, 1, 2, 4 or 8");
     if (groups_per_cu < 0 || groups_per_cu > 64)
         return fail(OHM_EINVAL, "launch config: groups_per_cu must be 0..64");
-    if (teams < 0 || teams > 2) return fail(OHM_EINVAL, "launch config: teams must be 0, 1 or 2");
+    if (teams < 0 || teams > 3) return fail(OHM_EINVAL, "launch config: teams must be 0, 1, 2 or 3");
     g_cfg_waves.store(waves);
     g_cfg_per_cu.store(groups_per_cu);
     g_cfg_teams.store(teams);

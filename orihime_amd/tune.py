@@ -24,14 +24,18 @@ import torch
 
 from . import _lib
 
-# candidate shapes: (waves, groups per CU, teams) with teams 1 = on, 2 = off
+# candidate shapes: (waves, groups per CU, teams) with teams 1 = team kernels,
+# 2 = per-pair workgroup kernels, 3 = grid kernels (batches of at most
+# GRID_MAX_PAIRS pairs)
 CANDIDATES = (
     (4, 0, 2),
     (4, 8, 2),
     (2, 0, 2),
     (8, 0, 2),
     (0, 0, 1),
+    (0, 0, 3),
 )
+GRID_MAX_PAIRS = 64
 
 
 @dataclass(frozen=True)
@@ -103,6 +107,8 @@ def autotune(scores, gap_open, gap_ext=None, temperature=1.0, *, lengths=None, o
         results = []
         try:
             for w, g, t in candidates:
+                if t == 3 and B > GRID_MAX_PAIRS:
+                    continue
                 _lib.set_launch_config(w, g, t)
                 results.append(LaunchConfig(w, g, t, _time(call, reps)))
         finally:

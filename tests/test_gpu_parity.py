@@ -242,6 +242,35 @@ def test_affine_team_kernels_vs_oracle(shape, monkeypatch):
     assert rel_err(h(a), ref[2], 1.0) < 1e-4 and rel_err(h(b), ref[3], 1.0) < 1e-4
 
 
+GRID_SHAPES = [(1, 1000, 700), (3, 300, 517), (5, 129, 64), (2, 64, 64), (4, 1, 200), (3, 70, 1)]
+
+
+@pytest.mark.parametrize("shape", GRID_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("params", PARAMS[:2], ids=lambda p: "go%g_ge%g_T%g" % p)
+def test_affine_grid_kernels_match_team_kernels_and_oracle(shape, params, monkeypatch):
+    """Grid kernels (the strips of every pair spread over all workgroups,
+    hand-offs through global memory) against the oracle.  Each strip runs the
+    same arithmetic as under the team kernels and the per-strip partials are
+    combined in strip order in both, so every output is bitwise identical to
+    the team kernels'."""
+    go, ge, T = params
+    s, _, _, ln = _inputs(*shape, seed=stable_seed(shape, params, "grid"))
+    st, lt = g(s), g(ln)
+    case(f"affine grid kernels shape={shape} params={params} lengths={ln.tolist()}")
+    monkeypatch.setenv("OHM_GRID", "1")
+    out = [t.clone() for t in torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)]
+    torch.cuda.synchronize()
+    monkeypatch.setenv("OHM_GRID", "0")
+    monkeypatch.setenv("OHM_TEAMS", "1")
+    team = torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)
+    for x, y in zip(out, team):
+        assert torch.equal(x, y)
+    ref = O.sw_affine_forward(s, go, ge, T, ln)
+    assert rel_err(h(out[0]), ref[0], 1e-3) < 1e-5
+    assert abs_err(h(out[1]), ref[1]) < 1e-4
+    assert rel_err(h(out[2]), ref[2], 1.0) < 1e-4 and rel_err(h(out[3]), ref[3], 1.0) < 1e-4
+
+
 def test_autotuned_launch_shape_changes_speed_not_results():
     """The autotuner (orihime_amd.tune) times candidate launch shapes on the
     batch and applies the fastest.  Maps are bitwise identical under every
