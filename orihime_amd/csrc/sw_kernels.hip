@@ -1447,7 +1447,13 @@ ohm_status tangent_impl(bool affine, const float* scores, const int32_t* lengths
     L.a.contract = contract;
     L.a.value = value;
     L.a.map = map;
-    if (affine) {
+    if (affine && use_grid(L.a)) {
+        const size_t lds = grid_cum_bytes(L.a.max_strips);
+        if ((st = launch_grid(sw_affine_tangent_forward_grid, lds, L)) != OHM_OK) return st;
+        stage_mark(1, L.stream);
+        stage_mark(2, L.stream);
+        if ((st = launch_grid(sw_affine_tangent_backward_grid, lds, L)) != OHM_OK) return st;
+    } else if (affine) {
         if (dir) {
             OHM_DISPATCH_W(L, (sw_affine_hvp_forward_kernel<W>));
             stage_mark(1, L.stream);

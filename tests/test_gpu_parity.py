@@ -271,6 +271,34 @@ def test_affine_grid_kernels_match_team_kernels_and_oracle(shape, params, monkey
     assert rel_err(h(out[2]), ref[2], 1.0) < 1e-4 and rel_err(h(out[3]), ref[3], 1.0) < 1e-4
 
 
+@pytest.mark.parametrize("shape", [(3, 300, 517), (1, 700, 400), (4, 129, 64)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("params", PARAMS[:2], ids=lambda p: "go%g_ge%g_T%g" % p)
+def test_affine_grid_tangent_sweeps_vs_oracle(shape, params, monkeypatch):
+    """HVP, parameter sensitivities and marginals backward through the
+    grid-scheduled tangent sweeps, against the oracle."""
+    go, ge, T = params
+    s, V, G, ln = _inputs(*shape, seed=stable_seed(shape, params, "grid tangent"))
+    st, lt = g(s), g(ln)
+    s64 = s.astype(np.float64)
+    case(f"affine grid tangent sweeps shape={shape} params={params} lengths={ln.tolist()}")
+    monkeypatch.setenv("OHM_GRID", "1")
+    hv = torch.ops.orihime.sw_affine_marginals_hvp(st, g(V), go, ge, T, lt)
+    r = O.sw_affine_hvp(s, V, go, ge, T, ln)
+    assert close_or_nearer_truth(h(hv), r, lambda: O.sw_affine_hvp(s64, V, go, ge, T, ln, prec="f64"), tol2(r))
+    for p, n in enumerate(("gap_open", "gap_ext", "temp")):
+        got = getattr(torch.ops.orihime, f"sw_affine_marginals_grad_{n}")(st, go, ge, T, lt)
+        r = O.sw_affine_param_jacobian(s, p, go, ge, T, ln)
+        truth = lambda p=p: O.sw_affine_param_jacobian(s64, p, go, ge, T, ln, prec="f64")
+        assert close_or_nearer_truth(h(got), r, truth, tol2(r)), n
+    gs, x, y, z = torch.ops.orihime.sw_affine_marginals_backward(st, g(G), go, ge, T, lt)
+    r = O.sw_affine_backward_full(s, G, go, ge, T, ln)
+    full = O.sw_affine_backward_full(s64, G, go, ge, T, ln, prec="f64")
+    assert close_or_nearer_truth(h(gs), r[0], lambda: full[0], tol2(r[0]))
+    for k, got in enumerate((x, y, z)):
+        assert close_or_nearer_truth(float(got), r[k + 1], lambda k=k: full[k + 1],
+                                     1e-3 * max(1.0, abs(r[k + 1])))
+
+
 def test_autotuned_launch_shape_changes_speed_not_results():
     """The autotuner (orihime_amd.tune) times candidate launch shapes on the
     batch and applies the fastest.  Maps are bitwise identical under every
