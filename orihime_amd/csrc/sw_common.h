@@ -100,11 +100,11 @@ constexpr int kClaimGridStall = 12;  // set when a grid hand-off wait exceeded i
 
 // Grid scheduling (strips of a pair on waves of any workgroup, hand-off
 // through global memory) for batches of at most kGridMaxPairs pairs.  Its
-// region: forward and reverse progress words [2][B][max_strips], per-pair
-// strip completion counts [2][B] (zeroed by the plan kernel), then per-strip
-// partial results [B][max_strips][4] floats.
+// region: per-pair strip completion counts of the forward and the reverse
+// launch [2][B] (zeroed by the plan kernel), then per-strip partial results
+// [B][max_strips][4] floats.
 constexpr int kGridMaxPairs = 64;
-inline int64_t grid_zero_ints(int64_t B, int64_t max_strips) { return 2 * B * max_strips + 2 * B; }
+inline int64_t grid_zero_ints(int64_t B, int64_t) { return 2 * B; }
 inline size_t grid_bytes(int64_t B, int64_t max_strips) {
     if (B > kGridMaxPairs) return 0;
     return (size_t)(grid_zero_ints(B, max_strips) + B * max_strips * 4) * 4;

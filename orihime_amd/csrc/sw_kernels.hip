@@ -292,50 +292,49 @@ DEV void publish(int* flag, int value, int lane) {
     if (lane == 0) __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Hand-off between waves of different workgroups (grid scheduling).  The
-// progress words and boundary entries are agent-scope atomic loads and stores
-// (global_load/store sc1: coherent per access across CUs and XCDs, without
-// the cache-wide write-back and invalidate of an agent-scope fence).  A
-// producer waits until its own stores have completed (vmcnt(0); stores count
-// in vmcnt on gfx9) before it publishes progress; a consumer reads boundary
-// entries only after it has seen the progress word.  The wait is bounded:
-// past kGridSpinCap polls it raises claims[kClaimGridStall] and returns (the
-// outputs are then NaN; by the ticket order every wait ends well before).
-constexpr int kGridSpinCap = 1 << 24;
+// Hand-off between waves of different workgroups (grid scheduling).
+//
+// Boundary entries are NaN-boxed: before a grid launch every word of the
+// pairs' boundary blocks is filled with kGridEmpty, a signalling-NaN pattern
+// that arithmetic never produces (invalid operations give the default quiet
+// NaN and NaN operands propagate quieted; scores are NaN-free).  A producer
+// stores its entries with agent-scope atomic stores (global_store sc1:
+// coherent per access across CUs and XCDs) and publishes nothing else; a
+// consumer takes a word as ready once an agent-scope load returns something
+// other than kGridEmpty (32-bit stores are single-copy atomic).  No progress
+// words, no fences and no store-completion waits: consumers prefetch the
+// next chunk's entries while they compute the current chunk and poll again
+// only for words still empty.  A poll is bounded: past kGridPollCap rounds,
+// or once another wave has given up, it raises claims[kClaimGridStall] and
+// returns (the outputs are then NaN; the ticket order makes every producer a
+// running wave, so no wait comes near the bound).
+constexpr uint32_t kGridEmpty = 0x7fbadbadu;
+constexpr int kGridPollCap = 1 << 20;
 DEV float cload(const float* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 DEV void cstore(float* p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-DEV void wait_ready_grid(const int* flag, int need, int* stall) {
-    for (int n = 0; __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need; ++n) {
-        if (n > kGridSpinCap) {
+DEV bool grid_empty(float v) { return __builtin_bit_cast(uint32_t, v) == kGridEmpty; }
+
+// Wait until the N words w of every lane with `need` are filled; fetch(w)
+// reloads a lane's words.
+template <int N, typename Fetch>
+DEV void grid_ready(float (&w)[N], bool need, Fetch&& fetch, int* stall) {
+    for (int n = 0;; ++n) {
+        bool empty = false;
+#pragma unroll
+        for (int i = 0; i < N; ++i) empty |= grid_empty(w[i]);
+        if (__builtin_amdgcn_ballot_w64(need && empty) == 0) break;
+        if (n >= kGridPollCap ||
+            ((n & 255) == 255 && __hip_atomic_load(stall, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
             __hip_atomic_store(stall, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
         }
         __builtin_amdgcn_s_sleep(1);
+        if (need && empty) fetch(w);
     }
-    asm volatile("" ::: "memory");  // boundary loads stay after the wait
-}
-DEV void publish_grid(int* flag, int value, int lane) {
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    if (lane == 0) __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Strip hand-off policy: progress words in LDS (one workgroup) or in global
-// memory (grid scheduling).
-template <bool GRID>
-DEV void hand_wait(int* flag, int need, int* stall) {
-    if constexpr (GRID) wait_ready_grid(flag, need, stall);
-    else wait_ready(flag, need);
-}
-template <bool GRID>
-DEV void hand_publish(int* flag, int value, int lane) {
-    if constexpr (GRID) publish_grid(flag, value, lane);
-    else publish(flag, value, lane);
-}
-template <bool GRID>
-DEV float hand_load(const float* p) {
-    if constexpr (GRID) return cload(p);
-    else return *p;
-}
+// Boundary entry store: plain (one workgroup, ordered by the LDS progress
+// words) or agent-scope atomic (grid scheduling).
 template <bool GRID>
 DEV void hand_store(float* p, float v) {
     if constexpr (GRID) cstore(p, v);
@@ -1330,6 +1329,10 @@ bool use_grid(const SwArgs& a) {
     if (env_int("OHM_TEAMS", -1) >= 0) return false;  // an explicit team / workgroup choice
     return 4 * a.B < device_cus() && a.max_strips >= 4;
 }
+ohm_status launch_grid_fill(const Launch& L) {
+    hipLaunchKernelGGL(grid_fill_kernel, dim3(L.a.B, 64), dim3(256), 0, L.stream, L.a);
+    return check_launch("grid fill kernel");
+}
 template <typename K>
 ohm_status launch_grid(K kernel, size_t lds, const Launch& L) {
     if (lds > 64 * 1024 &&
@@ -1374,6 +1377,9 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
         // it flagged (a launch whose groups only walk the claim list when
         // none are)
         if (grid) {
+            // one fill serves both sweeps (forward entries in words 0..2,
+            // reverse entries in words 4..6 of each block)
+            if ((st = launch_grid_fill(L)) != OHM_OK) return st;
             const size_t lds = grid_cum_bytes(L.a.max_strips);
             if (need_bwd) st = launch_grid(sw_affine_scaled_forward_grid<true>, lds, L);
             else st = launch_grid(sw_affine_scaled_forward_grid<false>, lds, L);
@@ -1448,10 +1454,13 @@ ohm_status tangent_impl(bool affine, const float* scores, const int32_t* lengths
     L.a.value = value;
     L.a.map = map;
     if (affine && use_grid(L.a)) {
+        // the tangent sweeps' entries share words 0..5: a fill before each
         const size_t lds = grid_cum_bytes(L.a.max_strips);
+        if ((st = launch_grid_fill(L)) != OHM_OK) return st;
         if ((st = launch_grid(sw_affine_tangent_forward_grid, lds, L)) != OHM_OK) return st;
         stage_mark(1, L.stream);
         stage_mark(2, L.stream);
+        if ((st = launch_grid_fill(L)) != OHM_OK) return st;
         if ((st = launch_grid(sw_affine_tangent_backward_grid, lds, L)) != OHM_OK) return st;
     } else if (affine) {
         if (dir) {
