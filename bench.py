@@ -1,22 +1,32 @@
 #!/usr/bin/env python3
 """Soft Smith-Waterman (affine gap) forward+backward throughput on one GPU.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ragged|fixed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload ragged|fixed|long|embed]
 
 Workload "ragged" (default): 2048 pairs with L1, L2 drawn independently and
 uniformly from [64, 1024], then each pair's N(0,1) FP32 scores in pair order
 (torch.Generator seed 2); gap_open=-8, gap_extend=-0.5, temperature=0.5.
 Workload "fixed": [256, 512, 512] scores N(0,1)*2 - 0.5 (seed 1), gap_open=-11,
 gap_extend=-1, temperature=1.
+Workload "long": [16, 4096, 4096] N(0,1) (seed 3), gap_open=-11, gap_extend=-1,
+temperature=1; also the HVP along an N(0,1) direction and the step's peak HBM
+against a reverse sweep that reads a full M/I/D table.
+Workload "embed": 512 pairs of BF16 embeddings Q [M, 128], K [N, 128] with M,
+N uniform in [128, 768] (seed 4); scores Q K^T by a BF16 GEMM with FP32
+accumulation, the ragged sweep, dQ and dK through the map.
 
 A step is one forward (value, log-partition) plus backward (marginal map and
 the gradients of gap_open, gap_extend and temperature) over the whole batch:
 one call of ohm_sw_affine_forward with every output requested, i.e. the
-planning kernel, the forward sweep and the reverse sweep.  The ragged batch
-runs as one packed persistent launch; it is compared with the padded launch
-([B, 1024, 1024] with the cells beyond each pair's lengths masked, i.e. set
-to the reference's mask sentinel), which is how the batch runs without ragged
-support.  GCUPS counts the useful cells sum_b L1_b * L2_b for both.
+planning kernel, the forward sweep and the reverse sweep (or one launch doing
+both per pair).  The launch shape is autotuned on the batch first
+(orihime_amd.tune; --no-tune keeps the automatic choice); the results do not
+depend on it.  The ragged batch runs as one packed persistent launch; it is
+compared with the padded launch ([B, 1024, 1024] with the cells beyond each
+pair's lengths masked, i.e. set to the reference's mask sentinel), which is
+how the batch runs without ragged support.  GCUPS counts the useful cells
+sum_b L1_b * L2_b for both.
 
 Timed steps replay a HIP graph of one step on inputs already in HBM.  Kernel
 durations come from HIP events recorded between the launches of the same
@@ -24,7 +34,11 @@ calls (ohm_set_stage_events) in a separate eager pass of the same K steps.
 
 Roofline per cell, affine (algorithmic, max-shifted FP32 log-sum-exp):
 forward 11 transcendentals and 4 bytes (read the score), reverse 11 and 4
-bytes (write the map); the binding roof is the transcendental issue rate.
+bytes (write the map), a fused forward+reverse launch 22 and 8.  `roofline`
+reports the dominant kernel against HBM (the contract's form) and, under
+`binding`, against the transcendental issue rate measured on the device
+(orihime_amd/csrc/peak_probe.hip); `traffic` is the dominant kernel's HBM
+bytes per launch from the committed byte-counter profile of the workload.
 
 The CPU baseline is the reference's CPU kernels (compiled from source into
 oracle/_ref) or, where that build is absent, the oracle's C restatement of
@@ -53,12 +67,15 @@ HBM_PEAK_GBS = 8000.0
 CUS, CLOCK_HZ = 256, 2.4e9
 VALU_LANE_OPS = CUS * 64 * CLOCK_HZ          # non-packed FP32 lane-ops/s (39.3 T)
 TRANS_PEAK = VALU_LANE_OPS / 4               # v_exp/v_log/v_rcp at quarter rate (9.83 T/s)
-TRANS_PER_CELL = {"forward": 11, "reverse": 11}
-BYTES_PER_CELL = {"forward": 4, "reverse": 4}
+TRANS_PER_CELL = {"forward": 11, "reverse": 11, "fused": 22}
+BYTES_PER_CELL = {"forward": 4, "reverse": 4, "fused": 8}
 KERNEL_NAMES = {
     "plan": "sw_plan_kernel",
     "forward": "sw_affine_scaled_forward_kernel<true, W>",
     "reverse": "sw_affine_scaled_backward_kernel<W>",
+    # one launch running each pair's forward and reverse sweep (the launch
+    # shape the autotuner picks on the ragged batch)
+    "fused": "sw_affine_scaled_fused_kernel<W>",
 }
 
 
@@ -338,7 +355,7 @@ def embed_workload(torch, args, dev):
         with torch.no_grad():
             cfg = tune.autotune(pair_scores(Q, K), -4.0, -0.5, 0.25, lengths=lnd)
         tuned = {"waves": cfg.waves, "groups_per_cu": cfg.groups_per_cu,
-                 "teams": {0: "auto", 1: "on", 2: "off", 3: "grid"}[cfg.teams]}
+                 "teams": {0: "auto", 1: "on", 2: "off", 3: "grid", 4: "fused"}[cfg.teams]}
         log(f"autotuned launch shape: {tuned}")
 
     for _ in range(max(1, args.warmup)):
@@ -449,7 +466,7 @@ def main():
         cfg = tune.autotune(main_launch.scores, *params, lengths=main_launch.lengths,
                             offsets=main_launch.offsets)
         tuned = {"waves": cfg.waves, "groups_per_cu": cfg.groups_per_cu,
-                 "teams": {0: "auto", 1: "on", 2: "off", 3: "grid"}[cfg.teams],
+                 "teams": {0: "auto", 1: "on", 2: "off", 3: "grid", 4: "fused"}[cfg.teams],
                  "tuning_s": round(time.perf_counter() - t0, 2)}
         log(f"autotuned launch shape: {tuned}")
 
@@ -485,7 +502,14 @@ def main():
     log("measured issue rates (T lane-ops/s): " + ", ".join(f"{k} {v}" for k, v in rates.items()))
     kd = kernel_durations(torch, _lib, hip, main_launch, args.steps)
     log("kernel durations (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in kd.items()))
-    dom = max(("forward", "reverse"), key=lambda k: kd[k])
+    fused = (tuned is not None and tuned["teams"] == "fused") or (
+        tuned is None and os.environ.get("OHM_FUSED") == "1")
+    if fused:
+        # the fused launch runs between the plan and forward marks
+        kd["fused"] = kd["forward"]
+        dom = "fused"
+    else:
+        dom = max(("forward", "reverse"), key=lambda k: kd[k])
     t_dom = kd[dom] * 1e-3
     ach_gbs = BYTES_PER_CELL[dom] * cells / t_dom / 1e9
     ach_trans = TRANS_PER_CELL[dom] * cells / t_dom

@@ -97,6 +97,7 @@ constexpr int kClaimFlagged = 8;
 constexpr int kClaimGridFwd = 10;   // strip tickets of the grid-scheduled sweeps
 constexpr int kClaimGridRev = 11;
 constexpr int kClaimGridStall = 12;  // set when a grid hand-off wait exceeded its bound
+constexpr int kClaimFused = 13;      // pair claims of the fused forward + reverse launch
 
 // Grid scheduling (strips of a pair on waves of any workgroup, hand-off
 // through global memory) for batches of at most kGridMaxPairs pairs.  Its

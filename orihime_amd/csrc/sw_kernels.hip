@@ -1346,6 +1346,15 @@ ohm_status launch_grid(K kernel, size_t lds, const Launch& L) {
     return check_launch("grid kernel");
 }
 
+// Fused forward + reverse per pair (sw_affine_scaled_fused_kernel) for the
+// per-pair workgroup path: teams = 4 in ohm_set_launch_config or OHM_FUSED=1.
+bool use_fused() {
+    const int cfg = g_cfg_teams.load();
+    if (cfg == 4) return true;
+    if (cfg != 0) return false;
+    return env_int("OHM_FUSED", 0) == 1;
+}
+
 // The scaled affine sweeps need representable gap factors 2^(g log2(e) / T)
 // with room for products of two of them and a neighbour ratio; outside that
 // (or with OHM_SCALED=0) the log-domain sweeps run alone.
@@ -1371,7 +1380,8 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     L.a.g2 = g2;
     const bool scaled = affine && use_scaled(L.a);
     const bool grid = scaled && use_grid(L.a);
-    const bool team = scaled && !grid && use_teams(B, L.a.max_strips);
+    const bool fused = scaled && !grid && need_bwd && use_fused();
+    const bool team = scaled && !grid && !fused && use_teams(B, L.a.max_strips);
     if (scaled) {
         // scaled sweep for every pair, then the log-domain sweep for the pairs
         // it flagged (a launch whose groups only walk the claim list when
@@ -1384,6 +1394,10 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
             if (need_bwd) st = launch_grid(sw_affine_scaled_forward_grid<true>, lds, L);
             else st = launch_grid(sw_affine_scaled_forward_grid<false>, lds, L);
             if (st != OHM_OK) return st;
+        } else if (fused) {
+            Launch F = L;
+            F.lds = 2 * (size_t)L.a.max_strips * sizeof(int);
+            OHM_DISPATCH_W(F, (sw_affine_scaled_fused_kernel<W>));
         } else if (team) {
             const size_t lds = team_lds_bytes(L.a.max_strips);
             if (need_bwd) st = launch_team(sw_affine_scaled_forward_team<true>, lds, L);
@@ -1416,7 +1430,7 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
         } else if (team) {
             if ((st = launch_team(sw_affine_scaled_backward_team, team_bwd_lds_bytes(L.a.max_strips), L)) != OHM_OK)
                 return st;
-        } else {
+        } else if (!fused) {
             OHM_DISPATCH_W(L, (sw_affine_scaled_backward_kernel<W>));
         }
         stage_mark(3, L.stream);
@@ -1502,7 +1516,7 @@ extern "C" ohm_status ohm_set_launch_config(int32_t waves, int32_t groups_per_cu
         return fail(OHM_EINVAL, "launch config: waves must be 0, 1, 2, 4 or 8");
     if (groups_per_cu < 0 || groups_per_cu > 64)
         return fail(OHM_EINVAL, "launch config: groups_per_cu must be 0..64");
-    if (teams < 0 || teams > 3) return fail(OHM_EINVAL, "launch config: teams must be 0, 1, 2 or 3");
+    if (teams < 0 || teams > 4) return fail(OHM_EINVAL, "launch config: teams must be 0..4");
     g_cfg_waves.store(waves);
     g_cfg_per_cu.store(groups_per_cu);
     g_cfg_teams.store(teams);

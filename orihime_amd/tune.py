@@ -26,7 +26,8 @@ from . import _lib
 
 # candidate shapes: (waves, groups per CU, teams) with teams 1 = team kernels,
 # 2 = per-pair workgroup kernels, 3 = grid kernels (batches of at most
-# GRID_MAX_PAIRS pairs)
+# GRID_MAX_PAIRS pairs), 4 = per-pair workgroups running each pair's forward
+# and reverse sweep in one launch
 CANDIDATES = (
     (4, 0, 2),
     (4, 8, 2),
@@ -34,6 +35,9 @@ CANDIDATES = (
     (8, 0, 2),
     (0, 0, 1),
     (0, 0, 3),
+    (4, 0, 4),
+    (4, 8, 4),
+    (8, 0, 4),
 )
 GRID_MAX_PAIRS = 64
 

@@ -242,6 +242,48 @@ def test_affine_team_kernels_vs_oracle(shape, monkeypatch):
     assert rel_err(h(a), ref[2], 1.0) < 1e-4 and rel_err(h(b), ref[3], 1.0) < 1e-4
 
 
+@pytest.mark.parametrize("shape", [(3, 63, 65), (2, 257, 31), (4, 191, 193), (2, 1, 70)], ids=lambda s: "x".join(map(str, s)))
+def test_affine_fused_forward_reverse_kernel_vs_oracle(shape, monkeypatch):
+    """One launch running each pair's forward and reverse sweep (teams = 4)
+    against the oracle; maps are bitwise those of the separate launches."""
+    go, ge, T = PARAMS[1]
+    s, _, _, ln = _inputs(*shape, seed=stable_seed(shape, "fused"))
+    st, lt = g(s), g(ln)
+    case(f"affine fused forward+reverse shape={shape} lengths={ln.tolist()}")
+    monkeypatch.setenv("OHM_TEAMS", "0")
+    sep = [t.clone() for t in torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)]
+    monkeypatch.setenv("OHM_FUSED", "1")
+    out = torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)
+    assert torch.equal(out[1], sep[1])
+    ref = O.sw_affine_forward(s, go, ge, T, ln)
+    assert rel_err(h(out[0]), ref[0], 1e-3) < 1e-5
+    assert abs_err(h(out[1]), ref[1]) < 1e-4
+    assert rel_err(h(out[2]), ref[2], 1.0) < 1e-4 and rel_err(h(out[3]), ref[3], 1.0) < 1e-4
+
+
+def test_affine_fused_kernel_leaves_flagged_pairs_to_the_log_domain(monkeypatch):
+    """Under the fused launch a pair that leaves the scaled range skips the
+    fused reverse and is redone by the log-domain sweeps, as with separate
+    launches."""
+    go, ge, T = -11.0, -1.0, 1.0
+    rng = np.random.default_rng(7)
+    s = rng.normal(size=(2, 40, 50)).astype(np.float32)
+    s[0] = -80.0
+    s[0, 20, 25] = 80.0
+    s[0, 21, 26] = 79.0
+    ln = np.array([[40, 50], [40, 50]], np.int32)
+    case("fused launch, range fallback")
+    monkeypatch.setenv("OHM_FUSED", "1")
+    out = [t.clone() for t in torch.ops.orihime.soft_sw_affine_with_grads(g(s), go, ge, T, g(ln))]
+    monkeypatch.setenv("OHM_SCALED", "0")
+    logd = torch.ops.orihime.soft_sw_affine_with_grads(g(s), go, ge, T, g(ln))
+    for x, y in zip(out, logd):
+        assert torch.equal(x[0], y[0])
+    ref = O.sw_affine_forward(s, go, ge, T, ln)
+    assert rel_err(h(out[0]), ref[0], 1e-3) < 1e-5
+    assert abs_err(h(out[1]), ref[1]) < 1e-4
+
+
 GRID_SHAPES = [(1, 1000, 700), (3, 300, 517), (5, 129, 64), (2, 64, 64), (4, 1, 200), (3, 70, 1)]
 
 
