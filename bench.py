@@ -516,8 +516,9 @@ def main():
     # transcendental peak: measured on this device (v_exp_f32 issue rate),
     # the quarter-rate assumption only when the probe is unavailable
     trans_peak = rates["exp"] * 1e12 if rates else TRANS_PEAK
-    step_roof = min(HBM_PEAK_GBS * 1e9 / sum(BYTES_PER_CELL.values()),
-                    trans_peak / sum(TRANS_PER_CELL.values())) / 1e9
+    sweeps = ("forward", "reverse")  # one step = one forward and one reverse sweep per cell
+    step_roof = min(HBM_PEAK_GBS * 1e9 / sum(BYTES_PER_CELL[k] for k in sweeps),
+                    trans_peak / sum(TRANS_PER_CELL[k] for k in sweeps)) / 1e9
     traffic, traffic_src = profiled_traffic("ragged2048" if args.workload == "ragged" else args.workload, dom)
     roofline = {
         "bound": "hbm",
