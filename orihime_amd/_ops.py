@@ -126,11 +126,20 @@ def sw_affine_forward_impl(scores, gap_open: float, gap_ext: float, temp: float,
     return [value, mp]
 
 
+# The tensor-parameter forwards also request the parameter gradients (the
+# same sweeps produce them) and attach them to the returned value tensor, so
+# the autograd backward reuses them instead of running the sweeps again for
+# value_grad_params (orihime/ops/sw_affine.py:133-146 recomputes).  Where the
+# attribute does not survive (fake or traced tensors) the backward recomputes.
+_PARAM_GRADS = "_ohm_param_grads"
+
+
 def sw_affine_forward_t_impl(scores, gap_open, gap_ext, temp, lengths):
     go = _scalar(gap_open, "gap_open")
     ge = _scalar(gap_ext, "gap_ext")
     T = _scalar(temp, "temperature")
-    value, mp, _ = _aff_forward(scores, go, ge, T, lengths)
+    value, mp, g = _aff_forward(scores, go, ge, T, lengths, want_grads=True)
+    setattr(value, _PARAM_GRADS, tuple(g))
     return [value, mp]
 
 
@@ -213,7 +222,9 @@ def sw_forward_impl(scores, gap: float, temp: float, lengths=None):
 
 
 def sw_forward_t_impl(scores, gap, temp, lengths):
-    value, mp, _ = _lin_forward(scores, _scalar(gap, "gap"), _scalar(temp, "temperature"), lengths)
+    value, mp, g = _lin_forward(scores, _scalar(gap, "gap"), _scalar(temp, "temperature"), lengths,
+                                want_grads=True)
+    setattr(value, _PARAM_GRADS, tuple(g))
     return [value, mp]
 
 
@@ -382,6 +393,7 @@ def _forward_setup(ctx, inputs, output):
     scores = inputs[0]
     ctx.save_for_backward(scores, output[1])
     ctx.inputs = inputs
+    ctx.param_grads = getattr(output[0], _PARAM_GRADS, None)
 
 
 def _make_forward_backward(affine: bool, tensor_params: bool):
@@ -400,8 +412,10 @@ def _make_forward_backward(affine: bool, tensor_params: bool):
         if gv is not None:
             g_scores = g_scores + gv.reshape(-1, 1, 1) * mp
             if tensor_params:
-                pg = (orihime.sw_affine_value_grad_params if affine else orihime.sw_value_grad_params)(
-                    scores.detach(), *fparams, lengths)
+                pg = ctx.param_grads
+                if pg is None:
+                    pg = (orihime.sw_affine_value_grad_params if affine else orihime.sw_value_grad_params)(
+                        scores.detach(), *fparams, lengths)
                 for k in range(nparam):
                     g_params[k] = g_params[k] + (gv * pg[k]).sum().reshape(1)
         if gm is not None:

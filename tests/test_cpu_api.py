@@ -445,3 +445,37 @@ def test_embedding_scored_alignment_cpu_matches_fp64_oracle():
     assert rel_err(v.detach().numpy(), val, 1e-3) < 1e-5
     assert abs_err(Q.grad.numpy(), dQ) < 1e-4 and abs_err(K.grad.numpy(), dK) < 1e-4
     assert float(Q.grad[1, 31:].abs().max()) == 0.0 and float(K.grad[1, 22:].abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("affine", [True, False], ids=["affine", "linear"])
+def test_tensor_parameter_gradients_reuse_the_forward_sweeps(affine, monkeypatch):
+    """The tensor-parameter forwards keep the parameter gradients of their own
+    sweeps, so backward runs no second forward for them (the reference
+    recomputes value_grad_params, orihime/ops/sw_affine.py:133-146); the
+    result equals the explicit value_grad_params."""
+    g = torch.Generator().manual_seed(11)
+    x = torch.randn(3, 20, 30, generator=g)
+    ln = torch.tensor([[20, 30], [7, 30], [20, 4]], dtype=torch.int32)
+    w = torch.tensor([1.0, -2.0, 0.5])
+    if affine:
+        params = [torch.tensor([-2.0], requires_grad=True), torch.tensor([-0.5], requires_grad=True),
+                  torch.tensor([0.7], requires_grad=True)]
+        op, vgp, name = torch.ops.orihime.sw_affine_forward_t, torch.ops.orihime.sw_affine_value_grad_params, "affine_forward"
+    else:
+        params = [torch.tensor([-1.0], requires_grad=True), torch.tensor([0.7], requires_grad=True)]
+        op, vgp, name = torch.ops.orihime.sw_forward_t, torch.ops.orihime.sw_value_grad_params, "linear_forward"
+    calls = []
+    orig = getattr(_lib, name)
+
+    def counted(*a, **k):
+        calls.append(1)
+        return orig(*a, **k)
+
+    monkeypatch.setattr(_lib, name, counted)
+    v, _ = op(x, *params, ln)
+    n = len(calls)
+    (v * w).sum().backward()
+    assert len(calls) == n
+    ref = vgp(x, *[float(p.detach()) for p in params], ln)
+    for p, r in zip(params, ref):
+        assert torch.allclose(p.grad, (w * r).sum().reshape(1), rtol=1e-6, atol=1e-6)
