@@ -240,6 +240,22 @@ def profiled_traffic(workload: str, kernel: str):
     return None, None
 
 
+def profiled_valu(workload: str, kernel: str):
+    """Wave-level VALU instructions per launch of `kernel` from the committed
+    SQ-counter profile of this workload (rocprofv3 SQ_INSTS_VALU), or None."""
+    path = os.path.join(ROOT, "profiles", f"sw_affine_{workload}_instructions.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("valu_instructions_per_launch:"):
+            table = json.loads(line.split(":", 1)[1])
+            key = KERNEL_NAMES[kernel].split("<")[0] + "<" + ("true" if kernel == "forward" else "")
+            for k, v in table.items():
+                if k.startswith(key):
+                    return int(v), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def timed(torch, fn, steps):
     """Mean ms per call over `steps` calls, bracketed by device syncs."""
     torch.cuda.synchronize()
@@ -519,7 +535,11 @@ def main():
     sweeps = ("forward", "reverse")  # one step = one forward and one reverse sweep per cell
     step_roof = min(HBM_PEAK_GBS * 1e9 / sum(BYTES_PER_CELL[k] for k in sweeps),
                     trans_peak / sum(TRANS_PER_CELL[k] for k in sweeps)) / 1e9
-    traffic, traffic_src = profiled_traffic("ragged2048" if args.workload == "ragged" else args.workload, dom)
+    wl_name = "ragged2048" if args.workload == "ragged" else args.workload
+    traffic, traffic_src = profiled_traffic(wl_name, dom)
+    valu, valu_src = profiled_valu(wl_name, dom)
+    # full-rate VALU issue measured on this device (wave-level instructions/s)
+    issue_peak = max(rates["fma"], rates["dpp"]) * 1e12 / 64 if rates else VALU_LANE_OPS / 64
     roofline = {
         "bound": "hbm",
         "achieved": round(ach_gbs, 2),
@@ -540,6 +560,19 @@ def main():
             "unit": "Top/s",
             "frac": round(ach_trans / trans_peak, 4),
         },
+        # what binds these kernels: VALU issue (profiled instruction count of
+        # the dominant kernel against the measured full-rate issue) and the
+        # measured HBM traffic against 8 TB/s
+        "issue": None if valu is None else {
+            "bound": "valu",
+            "instructions": valu,
+            "instructions_source": valu_src,
+            "achieved": round(valu / t_dom / 1e9, 2),
+            "peak": round(issue_peak / 1e9, 2),
+            "unit": "G wave-instructions/s",
+            "frac": round(valu / t_dom / issue_peak, 4),
+        },
+        "traffic_frac": None if traffic is None else round(traffic / t_dom / (HBM_PEAK_GBS * 1e9), 4),
         "measured_issue_rates_tops": rates,
         "step_roof_gcups": round(step_roof, 1),
         "step_frac": round(gcups / step_roof, 4),
