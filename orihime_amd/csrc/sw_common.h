@@ -62,7 +62,8 @@ constexpr int kCkpt = 8;                // steps between affine checkpoints
 constexpr int kPlanesAffineCkpt = 6;    // six state rows (log-domain sweep: M, I, D, A of the
                                         // previous step and A of the diagonal; scaled sweep:
                                         // g, h, a, c of the previous step, a and c of the diagonal)
-constexpr int kPlanesAffineTan = 6;     // M, I, D and their tangents
+constexpr int kPlanesAffineTan = 7;     // M, I, D and their tangents (log-domain tangent sweeps),
+                                        // plus the evaluation exponent (exponent-tracked HVP)
 constexpr int kPlanesLinearFwd = 2;     // A, X
 constexpr int kPlanesLinearTan = 4;     // A, X and their tangents
 

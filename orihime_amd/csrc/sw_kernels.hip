@@ -1130,6 +1130,7 @@ __global__ __launch_bounds__(W * 64) void sw_backward_kernel(SwArgs a) {
 }
 
 #include "sw_tangent.hiph"
+#include "sw_tangent_scaled.hiph"
 
 
 // ==========================================================================
@@ -1323,7 +1324,7 @@ bool use_grid(const SwArgs& a) {
     if (!a.grid) return false;
     if (grid_bwd_lds_bytes(a.max_strips) + sizeof(GridLds) + 1024 > kLdsLimit) return false;
     const int cfg = g_cfg_teams.load();
-    if (cfg == 1 || cfg == 2) return false;
+    if (cfg == 1 || cfg == 2 || cfg == 4) return false;  // an explicit other shape
     const int forced = cfg == 3 ? 1 : env_int("OHM_GRID", -1);
     if (forced >= 0) return forced == 1;
     if (env_int("OHM_TEAMS", -1) >= 0) return false;  // an explicit team / workgroup choice
@@ -1467,7 +1468,37 @@ ohm_status tangent_impl(bool affine, const float* scores, const int32_t* lengths
     L.a.contract = contract;
     L.a.value = value;
     L.a.map = map;
-    if (affine && use_grid(L.a)) {
+    const bool hvp_scaled = affine && dir && dgo == 0.0f && dge == 0.0f && dT == 0.0f && !contract &&
+                            use_scaled(L.a);
+    if (hvp_scaled) {
+        // exponent-tracked HVP sweeps for every pair, then the log-domain
+        // tangent sweeps for the pairs they flagged
+        const bool grid = use_grid(L.a);
+        const size_t lds = grid_cum_bytes(L.a.max_strips);
+        if (grid) {
+            if ((st = launch_grid_fill(L)) != OHM_OK) return st;
+            if ((st = launch_grid(sw_affine_scaled_hvp_forward_grid, lds, L)) != OHM_OK) return st;
+        } else {
+            OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_forward_kernel<W>));
+        }
+        stage_mark(1, L.stream);
+        L.a.sel = 1;
+        OHM_DISPATCH_W(L, (sw_affine_hvp_forward_kernel<W>));
+        stage_mark(2, L.stream);
+        if ((st = check_launch("tangent forward")) != OHM_OK) return st;
+        L.a.sel = 0;
+        if (grid) {
+            if ((st = launch_grid_fill(L)) != OHM_OK) return st;
+            if ((st = launch_grid(sw_affine_scaled_hvp_backward_grid, lds, L)) != OHM_OK) return st;
+        } else {
+            OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_backward_kernel<W>));
+        }
+        stage_mark(3, L.stream);
+        L.a.sel = 1;
+        OHM_DISPATCH_W(L, (sw_affine_hvp_backward_kernel<W>));
+        stage_mark(4, L.stream);
+        return check_launch("tangent backward");
+    } else if (affine && use_grid(L.a)) {
         // the tangent sweeps' entries share words 0..5: a fill before each
         const size_t lds = grid_cum_bytes(L.a.max_strips);
         if ((st = launch_grid_fill(L)) != OHM_OK) return st;

@@ -389,6 +389,28 @@ def test_scaled_sweep_out_of_range_pair_is_redone_in_log_domain(monkeypatch):
     assert rel_err(h(out[2]), ref[2], 1.0) < 1e-4 and rel_err(h(out[3]), ref[3], 1.0) < 1e-4
 
 
+def test_scaled_hvp_out_of_range_pair_is_redone_in_log_domain(monkeypatch):
+    """The exponent-tracked HVP sweeps flag the pair whose adjacent scores
+    jump from -80 T to +80 T; the log-domain tangent sweeps redo it, so its
+    HVP is the log-domain one bit for bit, and both pairs match the oracle."""
+    go, ge, T = -11.0, -1.0, 1.0
+    rng = np.random.default_rng(7)
+    s = rng.normal(size=(2, 40, 50)).astype(np.float32)
+    s[0] = -80.0
+    s[0, 20, 25] = 80.0
+    s[0, 21, 26] = 79.0
+    V = rng.normal(size=s.shape).astype(np.float32)
+    ln = np.array([[40, 50], [40, 50]], np.int32)
+    case("scaled HVP range fallback")
+    hv = h(torch.ops.orihime.sw_affine_marginals_hvp(g(s), g(V), go, ge, T, g(ln)))
+    monkeypatch.setenv("OHM_SCALED", "0")
+    logd = h(torch.ops.orihime.sw_affine_marginals_hvp(g(s), g(V), go, ge, T, g(ln)))
+    assert np.array_equal(hv[0], logd[0])
+    r = O.sw_affine_hvp(s, V, go, ge, T, ln)
+    truth = lambda: O.sw_affine_hvp(s.astype(np.float64), V, go, ge, T, ln, prec="f64")
+    assert close_or_nearer_truth(hv, r, truth, tol2(r))
+
+
 def test_masked_padding_stays_in_the_scaled_sweep():
     """Padded storage masked with -inf (replaced by the sentinel far below the
     scores, orihime/operator.py:334-386): masked cells have E = 0 and, hundreds
