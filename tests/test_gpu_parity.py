@@ -765,3 +765,7 @@ def test_embedding_scored_alignment_bf16_matches_fp64_oracle():
     assert abs_err(h(Q.grad.float()), dQ) < 4e-3 * max(1.0, float(np.abs(dQ).max()))
     assert abs_err(h(K.grad.float()), dK) < 4e-3 * max(1.0, float(np.abs(dK).max()))
     report("embed value", h(v), ref[0])
+    for name, got, want in (("dQ", Q.grad, dQ), ("dK", K.grad, dK)):
+        err = abs_err(h(got.float()), want)
+        print(f"  embed {name}: max abs err vs FP64 oracle {err:.3e} "
+              f"({err / float(np.abs(want).max()):.2e} of its largest entry)", flush=True)
