@@ -1468,31 +1468,40 @@ ohm_status tangent_impl(bool affine, const float* scores, const int32_t* lengths
     L.a.contract = contract;
     L.a.value = value;
     L.a.map = map;
-    const bool hvp_scaled = affine && dir && dgo == 0.0f && dge == 0.0f && dT == 0.0f && !contract &&
-                            use_scaled(L.a);
-    if (hvp_scaled) {
-        // exponent-tracked HVP sweeps for every pair, then the log-domain
+    if (affine && use_scaled(L.a)) {
+        // exponent-tracked tangent sweeps for every pair, then the log-domain
         // tangent sweeps for the pairs they flagged
+        const int mode = contract ? kTanContract
+                                  : (dgo != 0.0f || dge != 0.0f || dT != 0.0f) ? kTanParam : kTanScore;
         const bool grid = use_grid(L.a);
         const size_t lds = grid_cum_bytes(L.a.max_strips);
-        if (grid) {
-            if ((st = launch_grid_fill(L)) != OHM_OK) return st;
-            if ((st = launch_grid(sw_affine_scaled_hvp_forward_grid, lds, L)) != OHM_OK) return st;
+        if (grid && (st = launch_grid_fill(L)) != OHM_OK) return st;
+        if (mode == kTanParam) {
+            if (grid) st = launch_grid(sw_affine_scaled_hvp_forward_grid<kTanParam>, lds, L);
+            else OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_forward_kernel<W, kTanParam>));
         } else {
-            OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_forward_kernel<W>));
+            if (grid) st = launch_grid(sw_affine_scaled_hvp_forward_grid<kTanScore>, lds, L);
+            else OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_forward_kernel<W, kTanScore>));
         }
+        if (st != OHM_OK) return st;
         stage_mark(1, L.stream);
         L.a.sel = 1;
         OHM_DISPATCH_W(L, (sw_affine_hvp_forward_kernel<W>));
         stage_mark(2, L.stream);
         if ((st = check_launch("tangent forward")) != OHM_OK) return st;
         L.a.sel = 0;
-        if (grid) {
-            if ((st = launch_grid_fill(L)) != OHM_OK) return st;
-            if ((st = launch_grid(sw_affine_scaled_hvp_backward_grid, lds, L)) != OHM_OK) return st;
+        if (grid && (st = launch_grid_fill(L)) != OHM_OK) return st;
+        if (mode == kTanParam) {
+            if (grid) st = launch_grid(sw_affine_scaled_hvp_backward_grid<kTanParam>, lds, L);
+            else OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_backward_kernel<W, kTanParam>));
+        } else if (mode == kTanContract) {
+            if (grid) st = launch_grid(sw_affine_scaled_hvp_backward_grid<kTanContract>, lds, L);
+            else OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_backward_kernel<W, kTanContract>));
         } else {
-            OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_backward_kernel<W>));
+            if (grid) st = launch_grid(sw_affine_scaled_hvp_backward_grid<kTanScore>, lds, L);
+            else OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_backward_kernel<W, kTanScore>));
         }
+        if (st != OHM_OK) return st;
         stage_mark(3, L.stream);
         L.a.sel = 1;
         OHM_DISPATCH_W(L, (sw_affine_hvp_backward_kernel<W>));
