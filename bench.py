@@ -2,7 +2,7 @@
 """Soft Smith-Waterman (affine gap) forward+backward throughput on one GPU.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload ragged|fixed|long|embed]
+                    [--workload ragged|fixed|long|embed|nw]
 
 Workload "ragged" (default): 2048 pairs with L1, L2 drawn independently and
 uniformly from [64, 1024], then each pair's N(0,1) FP32 scores in pair order
@@ -15,6 +15,8 @@ against a reverse sweep that reads a full M/I/D table.
 Workload "embed": 512 pairs of BF16 embeddings Q [M, 128], K [N, 128] with M,
 N uniform in [128, 768] (seed 4); scores Q K^T by a BF16 GEMM with FP32
 accumulation, the ragged sweep, dQ and dK through the map.
+Workload "nw": the ragged batch of "ragged" through the Needleman-Wunsch
+(global) affine sweeps: value, map and parameter gradients.
 
 A step is one forward (value, log-partition) plus backward (marginal map and
 the gradients of gap_open, gap_extend and temperature) over the whole batch:
@@ -397,12 +399,70 @@ def embed_workload(torch, args, dev):
     print(json.dumps(line), flush=True)
 
 
+def nw_workload(torch, args, dev, hip):
+    """Needleman-Wunsch affine (global alignment) fwd+bwd on the ragged batch
+    of the headline workload (2048 pairs, L1, L2 ~ U[64, 1024], seed 2,
+    go=-8 ge=-0.5 T=0.5, packed layout).  Roofline: the forward streams the
+    scores in and M, I, D out (16 B per cell), the reverse streams M, I, D
+    and the scores in and the map out (20 B per cell)."""
+    from orihime_amd import _lib, ragged
+    params = (-8.0, -0.5, 0.5)
+    pairs = ragged_pairs(torch)
+    flat, lengths, offsets = ragged.pack(pairs)
+    flat, lengths, offsets = flat.to(dev), lengths.to(dev), offsets.to(dev)
+    cells = int(sum(p.numel() for p in pairs))
+    B, L1max, L2max = len(pairs), 1024, 1024
+    ws = torch.empty(int(_lib._ws_nw_affine(1, B, L1max, L2max, flat.numel())), dtype=torch.uint8,
+                     device=dev)
+    value = torch.empty(B, device=dev)
+    mp = torch.empty_like(flat)
+    g = [torch.empty(B, device=dev) for _ in range(3)]
+
+    class _Nw:
+        def step(self):
+            _lib.nw_affine_forward(flat, lengths, offsets, *params, value=value, map=mp,
+                                   grad_open=g[0], grad_ext=g[1], grad_temp=g[2], L1max=L1max,
+                                   L2max=L2max, workspace=ws)
+
+    launch = _Nw()
+    for _ in range(max(1, args.warmup)):
+        launch.step()
+    torch.cuda.synchronize()
+    ms, wall = timed(torch, launch.step, args.steps)
+    kd = kernel_durations(torch, _lib, hip, launch, max(3, min(args.steps, 10)))
+    fwd_ms, rev_ms = kd["forward"], kd["reverse"]
+    gcups = cells / (ms * 1e-3) / 1e9
+    byts = {"forward": 16.0 * cells, "reverse": 20.0 * cells}
+    dom = "reverse" if rev_ms >= fwd_ms else "forward"
+    dom_ms = rev_ms if dom == "reverse" else fwd_ms
+    achieved = byts[dom] / (dom_ms * 1e-3) / 1e9
+    log(f"nw affine fwd+bwd: {ms:.3f} ms/step, {gcups:.1f} GCUPS; forward {fwd_ms:.3f} ms "
+        f"({byts['forward'] / fwd_ms / 1e6:.0f} GB/s), reverse {rev_ms:.3f} ms "
+        f"({byts['reverse'] / rev_ms / 1e6:.0f} GB/s)")
+    line = {
+        "metric": "GCUPS soft-NW-affine fwd+bwd (value, map, parameter grads), 1 GPU",
+        "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "ragged affine soft-NW (global) fwd+bwd, 2048 pairs, L1,L2 ~ "
+                               "U[64,1024] (seed 2), gap_open=-8 gap_extend=-0.5 T=0.5, packed",
+                   "cells": cells, "wall_ms_per_step": round(wall, 4),
+                   "kernel_ms": {k: round(v, 4) for k, v in kd.items()}},
+        "roofline": {"bound": "hbm", "kernel": f"nw_affine_{dom}", "achieved": round(achieved, 1),
+                     "peak": 8000.0, "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
+                     "traffic": None, "bytes_per_cell": 20 if dom == "reverse" else 16},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("ragged", "fixed", "long", "embed"), default="ragged")
+    ap.add_argument("--workload", choices=("ragged", "fixed", "long", "embed", "nw"),
+                    default="ragged")
     ap.add_argument("--no-padded", action="store_true", help="skip the padded comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -461,6 +521,8 @@ def main():
         cpu_pairs = [x[i].numpy() for i in range(x.shape[0])]
     elif args.workload == "embed":
         return embed_workload(torch, args, dev)
+    elif args.workload == "nw":
+        return nw_workload(torch, args, dev, Hip())
     else:
         params = (-11.0, -1.0, 1.0)
         gen = torch.Generator().manual_seed(3)

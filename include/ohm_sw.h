@@ -168,6 +168,31 @@ ohm_status ohm_sw_tangent(const float* scores, const int32_t* lengths, const int
                           void* workspace, size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Needleman-Wunsch, affine gap (global alignment, three states M, I, D):  */
+/* boundary states M(0,0) = 0, I(i,0) = go + (i-1) ge, D(0,j) = go + (j-1) */
+/* ge, no restart, value LSE(M, I, D) at (L1, L2).                          */
+/* ------------------------------------------------------------------------ */
+
+/* Workspace bytes; op is OHM_OP_VALUE or OHM_OP_FORWARD (0 otherwise). */
+size_t ohm_nw_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
+                               int64_t total_cells);
+
+/* Value, map and scalar-parameter gradients, arguments as
+ * ohm_sw_affine_forward (the gradients need the map buffer).  Pairs with an
+ * empty side take the closed form: value go + (n-1) ge over the n > 0
+ * characters of the other side (0 for two empty sides), zero map.
+ * Replaces: nw_affine_forward, nw_affine_value_grad_params,
+ * soft_nw_affine_with_grads (src/nw_affine/registry.cpp:38-46) and the
+ * launchers nw_affine_forward_hip / nw_affine_backward_hip
+ * (src/nw_affine/kernels_gpu.hip). */
+ohm_status ohm_nw_affine_forward(const float* scores, const int32_t* lengths,
+                                 const int64_t* offsets, int32_t B, int32_t L1max,
+                                 int32_t L2max, float gap_open, float gap_ext, float temp,
+                                 float* value, float* map, float* grad_open, float* grad_ext,
+                                 float* grad_temp, void* workspace, size_t workspace_bytes,
+                                 void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Host (CPU) twins: same arguments on host memory, no workspace or stream. */
 /* `threads` <= 0 uses every hardware thread.  Replace the reference CPU     */
 /* dispatch (src/sw_affine/torch_cpu.cpp:593-707, src/sw/torch_cpu.cpp:522-591) */
@@ -187,6 +212,14 @@ ohm_status ohm_sw_affine_tangent_cpu(const float* scores, const int32_t* lengths
                                      float d_gap_ext, float d_temp, float* map_tangent,
                                      float* contract, float* value, float* map,
                                      int32_t threads);
+
+/* Replaces nw_affine_forward_cpu / nw_affine_backward_cpu
+ * (src/nw_affine/kernels_cpu.cpp:100-335). */
+ohm_status ohm_nw_affine_forward_cpu(const float* scores, const int32_t* lengths,
+                                     const int64_t* offsets, int32_t B, int32_t L1max,
+                                     int32_t L2max, float gap_open, float gap_ext, float temp,
+                                     float* value, float* map, float* grad_open,
+                                     float* grad_ext, float* grad_temp, int32_t threads);
 
 ohm_status ohm_sw_forward_cpu(const float* scores, const int32_t* lengths,
                               const int64_t* offsets, int32_t B, int32_t L1max, int32_t L2max,

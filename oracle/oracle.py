@@ -16,6 +16,8 @@ runs and how it contracts them:
 * ``backward_full``      -> ``soft_sw*_backward_full_cpu`` (forward + backward +
                             HVP + one sensitivity pass per parameter, each
                             contracted with the cotangent and summed)
+* ``nw_affine_forward``  -> src/nw_affine/torch_cpu.cpp ``soft_nw_affine_cpu_with_grads``
+                            (global alignment: forward + backward)
 
 All arrays are numpy; ``lengths`` defaults to the full shape.
 """
@@ -59,6 +61,10 @@ def ref_available() -> bool:
         os.path.exists(os.path.join(_HERE, "_ref", f"lib{n}_ref.so"))
         for n in ("sw", "sw_affine")
     )
+
+
+def nw_ref_available() -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", "libnw_affine_ref.so"))
 
 
 def _ref_lib(name: str):
@@ -291,3 +297,36 @@ def sw_backward_full(scores, G, gap, temp, lengths=None, impl="oracle", prec="f3
         dP = _lin_pj_from(k, s, ln, B, L1, L2, alpha, part, g[p], p, gap, temp)
         out.append(np.sum(Gc.astype(np.float64) * dP.astype(np.float64)))
     return tuple(out)
+
+
+# --------------------------------------------------------------------------
+# Needleman-Wunsch, affine gap (src/nw_affine/torch_cpu.cpp)
+# --------------------------------------------------------------------------
+
+def nw_affine_forward(scores, gap_open, gap_ext, temp, lengths=None, impl="oracle", prec="f32"):
+    """(value[B], map[B,L1,L2], grad_open[B], grad_ext[B], grad_temp[B]) of
+    the global (Needleman-Wunsch) affine alignment."""
+    dtype = np.float32 if prec == "f32" else np.float64
+    C = ctypes.c_float if prec == "f32" else ctypes.c_double
+    if impl == "ref":
+        if prec != "f32":
+            raise ValueError("the reference kernels are FP32 only")
+        lib = _ref_lib("nw_affine")
+        fwd, bwd = lib.nw_affine_forward_cpu, lib.nw_affine_backward_cpu
+    else:
+        lib = _lib()
+        fwd = getattr(lib, f"oracle_{prec}_nwa_forward")
+        bwd = getattr(lib, f"oracle_{prec}_nwa_backward")
+    s = _arr(scores, dtype)
+    ln = _lens(s, lengths)
+    B, L1, L2 = s.shape
+    asz = 3 * (L1 + 1) * (L2 + 1)
+    alpha = np.zeros((B, asz), dtype)
+    part = np.zeros(B, dtype)
+    fwd(_ptr(s), _ptr(alpha), _ptr(part), _ptr(ln), B, L1, L2, C(gap_open), C(gap_ext), C(temp))
+    beta = np.zeros((B, asz), dtype)
+    post = np.zeros((B, L1, L2), dtype)
+    g = [np.zeros(B, dtype) for _ in range(3)]
+    bwd(_ptr(alpha), _ptr(s), _ptr(part), _ptr(beta), _ptr(post), _ptr(g[0]), _ptr(g[1]),
+        _ptr(g[2]), _ptr(ln), B, L1, L2, C(gap_open), C(gap_ext), C(temp))
+    return part, post, g[0], g[1], g[2]

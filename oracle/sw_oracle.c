@@ -803,3 +803,141 @@ void OFN(lin_param_grad)(const real* alpha, const real* scores, const real* part
         }
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Needleman-Wunsch, affine gap (global alignment), three states M, I, D.    */
+/* ------------------------------------------------------------------------ */
+
+/* softmax3 of src/nw_affine/kernels_cpu.cpp:58-70: Kahan sum over the
+ * entries above NINF only, NINF when the sum is not positive. */
+static real nw_lse3(real a, real b, real c, real T) {
+    real m = rmax(rmax(a, b), c);
+    if (m <= NINF_) return NINF_;
+    kahan acc = {0, 0};
+    if (a > NINF_) kadd(&acc, sexp((a - m) / T));
+    if (b > NINF_) kadd(&acc, sexp((b - m) / T));
+    if (c > NINF_) kadd(&acc, sexp((c - m) / T));
+    if (acc.s <= 0) return NINF_;
+    return m + T * OLOG(acc.s);
+}
+
+/* softmax3_weights, src/nw_affine/kernels_cpu.cpp:73-94: plain sum. */
+static void nw_w3(real a, real b, real c, real T, real* w) {
+    real m = rmax(rmax(a, b), c);
+    if (m <= NINF_) { w[0] = w[1] = w[2] = 0; return; }
+    real ea = (a > NINF_) ? sexp((a - m) / T) : 0;
+    real eb = (b > NINF_) ? sexp((b - m) / T) : 0;
+    real ec = (c > NINF_) ? sexp((c - m) / T) : 0;
+    real tot = ea + eb + ec;
+    if (tot > 0) { w[0] = ea / tot; w[1] = eb / tot; w[2] = ec / tot; }
+    else w[0] = w[1] = w[2] = 0;
+}
+
+/* nw_affine_forward_cpu, src/nw_affine/kernels_cpu.cpp:100-193: boundary
+ * states M(0,0) = 0, I(i,0) = go + (i-1) ge, D(0,j) = go + (j-1) ge, no
+ * restart, value = LSE(M, I, D) at (L1, L2). */
+void OFN(nwa_forward)(const real* scores, real* alpha, real* part, const int* len,
+                      int B, int L1m, int L2m, real go, real ge, real T) {
+    geom g = mk_geom(L1m, L2m);
+    for (int b = 0; b < B; ++b) {
+        const real* s = scores + (size_t)b * g.ss;
+        real* M = alpha + (size_t)b * 3 * g.cs;
+        real* I = M + g.cs;
+        real* D = I + g.cs;
+        int L1 = len[2 * b], L2 = len[2 * b + 1];
+        for (size_t t = 0; t < 3 * g.cs; ++t) M[t] = NINF_;
+        M[0] = 0;
+        for (int i = 1; i <= L1; ++i) I[CELL(g, i, 0)] = go + (real)(i - 1) * ge;
+        for (int j = 1; j <= L2; ++j) D[CELL(g, 0, j)] = go + (real)(j - 1) * ge;
+        for (int k = 2; k <= L1 + L2; ++k) {
+            int lo = k - L2 > 1 ? k - L2 : 1, hi = k - 1 < L1 ? k - 1 : L1;
+            for (int i = lo; i <= hi; ++i) {
+                int j = k - i;
+                size_t c = CELL(g, i, j), dg = CELL(g, i - 1, j - 1);
+                size_t up = CELL(g, i - 1, j), lf = CELL(g, i, j - 1);
+                M[c] = s[SIDX(g, i, j)] + nw_lse3(M[dg], I[dg], D[dg], T);
+                I[c] = nw_lse3(M[up] + go, I[up] + ge, D[up] + go, T);
+                D[c] = nw_lse3(M[lf] + go, I[lf] + go, D[lf] + ge, T);
+            }
+        }
+        size_t f = CELL(g, L1, L2);
+        part[b] = nw_lse3(M[f], I[f], D[f], T);
+    }
+}
+
+/* nw_affine_backward_cpu, src/nw_affine/kernels_cpu.cpp:199-335: push-form
+ * adjoint seeded with the terminal LSE weights; adjoints at or below 1e-20
+ * are not propagated; the leading-gap boundary states add one open and
+ * (length - 1) extends each; grad_T = (S - E[path score]) / T. */
+void OFN(nwa_backward)(const real* alpha, const real* scores, const real* part, real* beta,
+                       real* post, real* gopen, real* gext, real* gT, const int* len,
+                       int B, int L1m, int L2m, real go, real ge, real T) {
+    geom g = mk_geom(L1m, L2m);
+    const real tiny = (real)1e-20;
+    for (int b = 0; b < B; ++b) {
+        const real* M = alpha + (size_t)b * 3 * g.cs;
+        const real* I = M + g.cs;
+        const real* D = I + g.cs;
+        const real* s = scores + (size_t)b * g.ss;
+        real* bM = beta + (size_t)b * 3 * g.cs;
+        real* bI = bM + g.cs;
+        real* bD = bI + g.cs;
+        real* P = post + (size_t)b * g.ss;
+        int L1 = len[2 * b], L2 = len[2 * b + 1];
+        memset(bM, 0, 3 * g.cs * sizeof(real));
+        memset(P, 0, g.ss * sizeof(real));
+        real w[3];
+        size_t f = CELL(g, L1, L2);
+        nw_w3(M[f], I[f], D[f], T, w);
+        bM[f] = w[0]; bI[f] = w[1]; bD[f] = w[2];
+        real so = 0, se = 0;
+        kahan ms = {0, 0};
+        for (int k = L1 + L2; k >= 2; --k) {
+            int lo = k - L2 > 1 ? k - L2 : 1, hi = k - 1 < L1 ? k - 1 : L1;
+            for (int i = lo; i <= hi; ++i) {
+                int j = k - i;
+                size_t c = CELL(g, i, j), dg = CELL(g, i - 1, j - 1);
+                size_t up = CELL(g, i - 1, j), lf = CELL(g, i, j - 1);
+                real sc = s[SIDX(g, i, j)];
+                real bm = bM[c], bi = bI[c], bd = bD[c];
+                if (bm > tiny && nw_lse3(M[dg], I[dg], D[dg], T) > NINF_) {
+                    nw_w3(M[dg], I[dg], D[dg], T, w);
+                    if (M[dg] > NINF_) bM[dg] += bm * w[0];
+                    if (I[dg] > NINF_) bI[dg] += bm * w[1];
+                    if (D[dg] > NINF_) bD[dg] += bm * w[2];
+                    P[SIDX(g, i, j)] += bm;
+                    kadd(&ms, bm * sc);
+                }
+                if (bi > tiny) {
+                    real x0 = M[up] + go, x1 = I[up] + ge, x2 = D[up] + go;
+                    if (nw_lse3(x0, x1, x2, T) > NINF_) {
+                        nw_w3(x0, x1, x2, T, w);
+                        bM[up] += bi * w[0]; bI[up] += bi * w[1]; bD[up] += bi * w[2];
+                        so += bi * (w[0] + w[2]);
+                        se += bi * w[1];
+                    }
+                }
+                if (bd > tiny) {
+                    real x0 = M[lf] + go, x1 = I[lf] + go, x2 = D[lf] + ge;
+                    if (nw_lse3(x0, x1, x2, T) > NINF_) {
+                        nw_w3(x0, x1, x2, T, w);
+                        bM[lf] += bd * w[0]; bI[lf] += bd * w[1]; bD[lf] += bd * w[2];
+                        so += bd * (w[0] + w[1]);
+                        se += bd * w[2];
+                    }
+                }
+            }
+        }
+        for (int i = 1; i <= L1; ++i) {
+            real v = bI[CELL(g, i, 0)];
+            if (v > tiny) { so += v; se += v * (real)(i - 1); }
+        }
+        for (int j = 1; j <= L2; ++j) {
+            real v = bD[CELL(g, 0, j)];
+            if (v > tiny) { so += v; se += v * (real)(j - 1); }
+        }
+        gopen[b] = so;
+        gext[b] = se;
+        gT[b] = (part[b] - (ms.s + so * go + se * ge)) / T;
+    }
+}

@@ -2,7 +2,8 @@
 
 Drop-in for the soft Smith-Waterman path of orihime: the same public
 functions (``sw``, ``sw_value``, ``sw_entropy``, ``sw_affine``,
-``sw_affine_value``, ``sw_affine_entropy``), ``nn`` modules, ``ops``
+``sw_affine_value``, ``sw_affine_entropy``; the global-alignment
+``nw_affine``, ``nw_affine_value``, ``nw_affine_entropy``), ``nn`` modules, ``ops``
 named operations, ``raw`` low-level tier and ``torch.ops.orihime.*``
 primitives, computed by hand-written gfx950 HIP kernels (GPU tensors) or
 the library's host implementation (CPU tensors), both behind the C ABI in
@@ -13,6 +14,7 @@ __version__ = "0.1.0"
 
 from . import _ops as _ops  # registers torch.ops.orihime.* (loads libohm_sw.so)
 from .sw import sw, sw_affine, sw_affine_entropy, sw_affine_value, sw_entropy, sw_value
+from .nw import nw_affine, nw_affine_entropy, nw_affine_value
 from . import nn as nn
 from . import raw as raw
 from . import ops as ops
@@ -29,6 +31,9 @@ __all__ = [
     "sw_affine",
     "sw_affine_value",
     "sw_affine_entropy",
+    "nw_affine",
+    "nw_affine_value",
+    "nw_affine_entropy",
     "nn",
     "ops",
     "raw",

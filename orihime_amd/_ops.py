@@ -297,6 +297,67 @@ def soft_sw_param_jacobian_impl(scores, param_type: int, gap: float, temperature
 
 
 # --------------------------------------------------------------------------
+# Needleman-Wunsch, affine gap (src/nw_affine/registry.cpp:33-46)
+# --------------------------------------------------------------------------
+
+
+def _nwa_forward(scores, go, ge, temp, lengths, want_map=True, want_grads=False):
+    _check_scores(scores)
+    ln = _check_lengths(lengths, scores)
+    B = scores.shape[0]
+    value = _vec(scores, B)
+    # the gradients come out of the reverse sweep that writes the map
+    mp = torch.empty_like(scores) if (want_map or want_grads) else None
+    g = [_vec(scores, B) for _ in range(3)] if want_grads else [None] * 3
+    _lib.nw_affine_forward(scores, ln, None, go, ge, temp, value=value, map=mp,
+                           grad_open=g[0], grad_ext=g[1], grad_temp=g[2])
+    return value, mp, g
+
+
+def nw_affine_forward_impl(scores, gap_open: float, gap_ext: float, temp: float, lengths=None):
+    value, mp, _ = _nwa_forward(scores, gap_open, gap_ext, temp, lengths)
+    return [value, mp]
+
+
+def nw_affine_forward_t_impl(scores, gap_open, gap_ext, temp, lengths):
+    go = _scalar(gap_open, "gap_open")
+    ge = _scalar(gap_ext, "gap_ext")
+    T = _scalar(temp, "temperature")
+    value, mp, g = _nwa_forward(scores, go, ge, T, lengths, want_grads=True)
+    setattr(value, _PARAM_GRADS, tuple(g))
+    return [value, mp]
+
+
+def nw_affine_value_grad_params_impl(scores, gap_open: float, gap_ext: float, temp: float,
+                                     lengths=None):
+    """(value, dS/dgap_open, dS/dgap_ext, dS/dT): src/nw_affine/torch_cpu.cpp:596-610."""
+    value, _, g = _nwa_forward(scores, gap_open, gap_ext, temp, lengths, want_map=False,
+                               want_grads=True)
+    return value, g[0], g[1], g[2]
+
+
+def soft_nw_affine_with_grads_impl(scores, gap_open: float, gap_ext: float, temperature: float,
+                                   lengths=None):
+    value, mp, g = _nwa_forward(scores, gap_open, gap_ext, temperature, lengths,
+                                want_grads=True)
+    return value, mp, g[0], g[1], g[2]
+
+
+def nw_affine_value_only_impl(scores, gap_open: float, gap_ext: float, temp: float, lengths=None):
+    value, _, _ = _nwa_forward(scores, gap_open, gap_ext, temp, lengths, want_map=False)
+    return value
+
+
+def _nwa_second_order(what: str):
+    def impl(scores, *args):
+        raise NotImplementedError(
+            f"nw_affine {what} (second-order derivatives of the Needleman-Wunsch map) has no "
+            "kernel in orihime_amd yet; the value, map and parameter gradients are available")
+
+    return impl
+
+
+# --------------------------------------------------------------------------
 # Registration
 # --------------------------------------------------------------------------
 
@@ -336,6 +397,23 @@ _SCHEMAS = {
     "soft_sw_hvp": ("(Tensor scores, Tensor tangent, float gap, float temperature, Tensor? lengths) -> Tensor", sw_marginals_hvp_impl),
     "soft_sw_param_jacobian": ("(Tensor scores, int param_type, float gap, float temperature, Tensor? lengths) -> Tensor", soft_sw_param_jacobian_impl),
     "soft_sw_backward_full": ("(Tensor scores, Tensor grad_alignment, float gap, float temperature, Tensor? lengths) -> (Tensor, Tensor, Tensor)", sw_marginals_backward_impl),
+    # Needleman-Wunsch affine — namespaced API
+    "nw_affine_forward": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor[]", nw_affine_forward_impl),
+    "nw_affine_forward_t": ("(Tensor scores, Tensor gap_open, Tensor gap_ext, Tensor temp, Tensor lengths) -> Tensor[]", nw_affine_forward_t_impl),
+    "nw_affine_value_grad_params": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor)", nw_affine_value_grad_params_impl),
+    "nw_affine_marginals_backward": ("(Tensor scores, Tensor grad_marginals, float gap_open, float gap_ext, float temp, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor)", _nwa_second_order("marginals_backward")),
+    "nw_affine_marginals_hvp": ("(Tensor scores, Tensor v, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", _nwa_second_order("marginals_hvp")),
+    "nw_affine_marginals_grad_gap_open": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", _nwa_second_order("marginals_grad_gap_open")),
+    "nw_affine_marginals_grad_gap_ext": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", _nwa_second_order("marginals_grad_gap_ext")),
+    "nw_affine_marginals_grad_temp": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", _nwa_second_order("marginals_grad_temp")),
+    "nw_affine_value_only": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", nw_affine_value_only_impl),
+    # Needleman-Wunsch affine — legacy
+    "soft_nw_affine": ("(Tensor scores, Tensor gap_open, Tensor gap_ext, Tensor temperature, Tensor lengths) -> Tensor[]", nw_affine_forward_t_impl),
+    "soft_nw_affine_float": ("(Tensor scores, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> Tensor[]", nw_affine_forward_impl),
+    "soft_nw_affine_with_grads": ("(Tensor scores, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor, Tensor)", soft_nw_affine_with_grads_impl),
+    "soft_nw_affine_hvp": ("(Tensor scores, Tensor tangent, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> Tensor", _nwa_second_order("hvp")),
+    "soft_nw_affine_param_jacobian": ("(Tensor scores, int param_type, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> Tensor", _nwa_second_order("param_jacobian")),
+    "soft_nw_affine_backward_full": ("(Tensor scores, Tensor grad_alignment, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor)", _nwa_second_order("backward_full")),
 }
 
 
@@ -356,7 +434,7 @@ def _fake_for(name: str, schema: str):
         if ret == "Tensor[]":
             return [vec(scores), like(scores)]
         if name.endswith("value_grad_params"):
-            n = 3 if "affine" in name else 2
+            n = 4 if name.startswith("nw_") else (3 if "affine" in name else 2)
             return tuple(vec(scores) for _ in range(n))
         if name.endswith("_with_grads"):
             n = 3 if "affine" in name else 2
@@ -396,8 +474,9 @@ def _forward_setup(ctx, inputs, output):
     ctx.param_grads = getattr(output[0], _PARAM_GRADS, None)
 
 
-def _make_forward_backward(affine: bool, tensor_params: bool):
+def _make_forward_backward(affine: bool, tensor_params: bool, family: str | None = None):
     nparam = 3 if affine else 2
+    family = family or ("sw_affine" if affine else "sw")
 
     def backward(ctx, grads):
         scores, mp = ctx.saved_tensors
@@ -414,12 +493,13 @@ def _make_forward_backward(affine: bool, tensor_params: bool):
             if tensor_params:
                 pg = ctx.param_grads
                 if pg is None:
-                    pg = (orihime.sw_affine_value_grad_params if affine else orihime.sw_value_grad_params)(
-                        scores.detach(), *fparams, lengths)
+                    pg = getattr(orihime, f"{family}_value_grad_params")(scores.detach(), *fparams, lengths)
+                    if family == "nw_affine":
+                        pg = pg[1:]  # (value, grads...)
                 for k in range(nparam):
                     g_params[k] = g_params[k] + (gv * pg[k]).sum().reshape(1)
         if gm is not None:
-            mb = (orihime.sw_affine_marginals_backward if affine else orihime.sw_marginals_backward)(
+            mb = getattr(orihime, f"{family}_marginals_backward")(
                 scores.detach(), gm.contiguous().to(torch.float32), *fparams, lengths)
             g_scores = g_scores + mb[0]
             for k in range(nparam):
@@ -442,6 +522,12 @@ for _name, _aff, _tp in (
     torch.library.register_autograd(
         f"orihime::{_name}", _make_forward_backward(_aff, _tp), setup_context=_forward_setup,
         lib=_LIB,
+    )
+for _name, _tp in (("nw_affine_forward", False), ("nw_affine_forward_t", True),
+                   ("soft_nw_affine_float", False), ("soft_nw_affine", True)):
+    torch.library.register_autograd(
+        f"orihime::{_name}", _make_forward_backward(True, _tp, "nw_affine"),
+        setup_context=_forward_setup, lib=_LIB,
     )
 
 # --------------------------------------------------------------------------
@@ -535,6 +621,20 @@ sw_affine_marginals_grad_gap_open = orihime.sw_affine_marginals_grad_gap_open
 sw_affine_marginals_grad_gap_ext = orihime.sw_affine_marginals_grad_gap_ext
 sw_affine_marginals_grad_temp = orihime.sw_affine_marginals_grad_temp
 sw_affine_value_only = orihime.sw_affine_value_only
+
+for _op in ("nw_affine_forward", "nw_affine_forward_t", "nw_affine_value_grad_params",
+            "nw_affine_value_only"):
+    torch.library.register_vmap(f"orihime::{_op}", _make_batch_folding_vmap_rule(_op), lib=_LIB)
+
+nw_affine_forward = orihime.nw_affine_forward
+nw_affine_forward_t = orihime.nw_affine_forward_t
+nw_affine_value_grad_params = orihime.nw_affine_value_grad_params
+nw_affine_marginals_backward = orihime.nw_affine_marginals_backward
+nw_affine_marginals_hvp = orihime.nw_affine_marginals_hvp
+nw_affine_marginals_grad_gap_open = orihime.nw_affine_marginals_grad_gap_open
+nw_affine_marginals_grad_gap_ext = orihime.nw_affine_marginals_grad_gap_ext
+nw_affine_marginals_grad_temp = orihime.nw_affine_marginals_grad_temp
+nw_affine_value_only = orihime.nw_affine_value_only
 
 
 def _has_cuda_dispatch() -> bool:

@@ -149,6 +149,8 @@ def install_public_operations(namespace: ModuleType) -> None:
     swmod = import_module(".sw", __package__)
     k_sw = import_module(".ops.sw", __package__)
     k_aff = import_module(".ops.sw_affine", __package__)
+    nwmod = import_module(".nw", __package__)
+    k_nwa = import_module(".ops.nw_affine", __package__)
 
     ops = {
         "sw": PublicOperation(name="sw", operator=swmod._sw_operator, map_function=swmod.sw,
@@ -157,8 +159,12 @@ def install_public_operations(namespace: ModuleType) -> None:
                                      map_function=swmod.sw_affine,
                                      value_function=swmod.sw_affine_value,
                                      entropy_function=swmod.sw_affine_entropy),
+        "nw_affine": PublicOperation(name="nw_affine", operator=nwmod._nw_affine_operator,
+                                     map_function=nwmod.nw_affine,
+                                     value_function=nwmod.nw_affine_value,
+                                     entropy_function=nwmod.nw_affine_entropy),
     }
-    namespace._kernels = MappingProxyType({"sw": k_sw, "sw_affine": k_aff})
+    namespace._kernels = MappingProxyType({"sw": k_sw, "sw_affine": k_aff, "nw_affine": k_nwa})
     for name, op in ops.items():
         setattr(namespace, name, op)
     namespace.__all__ = list(ops)

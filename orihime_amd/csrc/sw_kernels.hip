@@ -927,6 +927,7 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
     }
 }
 
+#include "nw_affine.hiph"
 #include "sw_teams.hiph"
 #include "sw_grid.hiph"
 #include "sw_affine_scaled.hiph"
@@ -1226,9 +1227,11 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     if (offsets && !lengths) return fail(OHM_EINVAL, "the packed layout needs lengths");
     if ((int64_t)((L1max + kStrip - 1) / kStrip) * 4 > 60 * 1024)
         return fail(OHM_EINVAL, "L1 too long for one launch (max 983040 rows)");
-    const TableSpec tf = affine ? TableSpec{kPlanesAffineCkpt, kCkpt} : TableSpec{kPlanesLinearFwd, 1};
+    // op 3: the Needleman-Wunsch forward + reverse (full M, I, D table)
+    const TableSpec tf = op == 3 ? TableSpec{kPlanesNw, 1}
+                         : affine ? TableSpec{kPlanesAffineCkpt, kCkpt} : TableSpec{kPlanesLinearFwd, 1};
     const TableSpec tt = {affine ? kPlanesAffineTan : kPlanesLinearTan, 1};
-    WsLayout w = ws_layout(op, B, L1max, L2max, -1, tf, tt);
+    WsLayout w = ws_layout(op == 3 ? 1 : op, B, L1max, L2max, -1, tf, tt);
     if (!ws) return fail(OHM_EWORKSPACE, "workspace is null");
     if (ws_bytes < w.data + 256) return fail(OHM_EWORKSPACE, "workspace too small");
     if (!offsets && ws_bytes < w.total)
@@ -1262,7 +1265,7 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     L.geo = choose_geometry(B, L1max);
     L.lds = (size_t)a.max_strips * sizeof(int);
     L.stream = static_cast<hipStream_t>(stream);
-    const TableSpec ts = op == 0 ? TableSpec{0, 1} : (op == 1 ? tf : tt);
+    const TableSpec ts = op == 0 ? TableSpec{0, 1} : (op == 2 ? tt : tf);
     const int64_t cap = (int64_t)((ws_bytes - w.data) / sizeof(float));
     if (B <= kGridMaxPairs) a.grid = reinterpret_cast<int*>(base + w.grid);
     hipLaunchKernelGGL(sw_plan_kernel, dim3(1), dim3(kPlanThreads), 0, L.stream, lengths, offsets,
@@ -1447,6 +1450,35 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     return check_launch("backward kernel");
 }
 
+// Needleman-Wunsch affine: value, then (when any backward output is asked
+// for) map and parameter gradients.
+ohm_status nw_forward_impl(const float* scores, const int32_t* lengths, const int64_t* offsets,
+                           int B, int L1max, int L2max, float go, float ge, float temp,
+                           float* value, float* map, float* g0, float* g1, float* g2, void* ws,
+                           size_t ws_bytes, void* stream) {
+    if (B > 0 && !value) return fail(OHM_EINVAL, "value is null");
+    const bool need_bwd = map || g0 || g1 || g2;
+    if (need_bwd && !map) return fail(OHM_EINVAL, "the gradients need the map buffer");
+    Launch L;
+    ohm_status st = setup(need_bwd ? 3 : 0, true, scores, lengths, offsets, B, L1max, L2max, go,
+                          ge, temp, ws, ws_bytes, stream, L);
+    if (st != OHM_OK || B == 0) return st;
+    L.a.value = value;
+    L.a.map = map;
+    L.a.g0 = g0;
+    L.a.g1 = g1;
+    L.a.g2 = g2;
+    if (need_bwd) OHM_DISPATCH_W(L, (nw_affine_forward_kernel<true, W>));
+    else OHM_DISPATCH_W(L, (nw_affine_forward_kernel<false, W>));
+    stage_mark(1, L.stream);
+    stage_mark(2, L.stream);
+    if ((st = check_launch("nw forward kernel")) != OHM_OK || !need_bwd) return st;
+    OHM_DISPATCH_W(L, (nw_affine_backward_kernel<W>));
+    stage_mark(3, L.stream);
+    stage_mark(4, L.stream);
+    return check_launch("nw backward kernel");
+}
+
 ohm_status tangent_impl(bool affine, const float* scores, const int32_t* lengths,
                         const int64_t* offsets, int B, int L1max, int L2max, float go, float ge,
                         float temp, const float* dir, float dgo, float dge, float dT,
@@ -1568,6 +1600,25 @@ extern "C" size_t ohm_sw_affine_workspace(int32_t op, int32_t B, int32_t L1max, 
     if (B < 0 || L1max < 0 || L2max < 0 || op < 0 || op > 2) return 0;
     return ws_layout(op, B, L1max, L2max, total_cells, TableSpec{kPlanesAffineCkpt, kCkpt},
                      TableSpec{kPlanesAffineTan, 1}).total;
+}
+
+extern "C" size_t ohm_nw_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
+                                          int64_t total_cells) {
+    if (B < 0 || L1max < 0 || L2max < 0 || op < 0 || op > 1) return 0;
+    return ws_layout(op, B, L1max, L2max, total_cells, TableSpec{kPlanesNw, 1},
+                     TableSpec{kPlanesNw, 1}).total;
+}
+
+extern "C" ohm_status ohm_nw_affine_forward(const float* scores, const int32_t* lengths,
+                                            const int64_t* offsets, int32_t B, int32_t L1max,
+                                            int32_t L2max, float gap_open, float gap_ext,
+                                            float temp, float* value, float* map,
+                                            float* grad_open, float* grad_ext, float* grad_temp,
+                                            void* workspace, size_t workspace_bytes,
+                                            void* stream) {
+    return nw_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
+                           value, map, grad_open, grad_ext, grad_temp, workspace,
+                           workspace_bytes, stream);
 }
 
 extern "C" size_t ohm_sw_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,

@@ -1,7 +1,8 @@
 """Write the golden fixtures in tests/golden/*.npz.
 
 Every expected output comes from the reference's own CPU kernels
-(src/sw/kernels_cpu.cpp, src/sw_affine/kernels_cpu.cpp) compiled from their
+(src/sw/kernels_cpu.cpp, src/sw_affine/kernels_cpu.cpp,
+src/nw_affine/kernels_cpu.cpp) compiled from their
 sources into oracle/_ref/ by ``make -C oracle ref``; the composites follow
 src/sw*/torch_cpu.cpp (which passes each primitive runs).  The fixtures hold
 inputs and expected outputs only.  Regenerate with
@@ -89,6 +90,18 @@ def linear_case(name, s, V, G, gap, T, lengths=None, full=True):
     return d
 
 
+def nw_affine_case(name, s, go, ge, T, lengths=None):
+    B, L1, L2 = s.shape
+    if lengths is None:
+        lengths = np.tile(np.array([L1, L2], np.int32), (B, 1))
+    d = dict(kind="nw_affine", scores=s, lengths=lengths, gap_open=np.float32(go),
+             gap_ext=np.float32(ge), temp=np.float32(T))
+    v, m, go_, ge_, gt = O.nw_affine_forward(s, go, ge, T, lengths, impl="ref")
+    d.update(value=v, map=m, grad_open=go_, grad_ext=ge_, grad_temp=gt)
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **d)
+    return d
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("build the reference kernels first: make -C oracle ref")
@@ -131,6 +144,31 @@ def main():
     ln = np.array([[0, 11], [9, 0], [4, 7]], np.int32)
     affine_case("affine_zero_lengths", s, V, G, -1.0, -0.3, 0.9, ln)
     linear_case("linear_zero_lengths", s, V, G, -0.7, 0.9, ln)
+    # Needleman-Wunsch affine (global): the reference test grid's shapes and
+    # gaps (tests/test_soft_nw_affine.py:143-160), a multi-strip pair, ragged
+    # lengths with empty sides, boundary-heavy scores (:248-258).
+    if not O.nw_ref_available():
+        raise SystemExit("build the reference kernels first: make -C oracle ref")
+    for i, (shape, go, ge, T) in enumerate([((4, 8, 10), -2.0, -0.5, 1.0),
+                                            ((4, 16, 16), -1.0, -0.3, 0.1),
+                                            ((4, 5, 20), -3.0, -1.0, 2.0)]):
+        s = _draw(70 + i, shape, extra=0)[0]
+        nw_affine_case(f"nw_affine_b{shape[0]}_{shape[1]}x{shape[2]}", s, go, ge, T)
+    s = _draw(80, (2, 150, 97), extra=0)[0]
+    nw_affine_case("nw_affine_b2_150x97", s, -2.0, -0.5, 0.5)
+    s = _draw(81, (5, 40, 36), extra=0)[0]
+    ln = np.array([[40, 36], [17, 29], [0, 6], [33, 0], [1, 1]], np.int32)
+    nw_affine_case("nw_affine_ragged_b5_40x36", s, -2.0, -0.5, 1.0, ln)
+    s = _draw(7, (3, 5, 7), extra=0)[0]
+    s[:, 0, :] -= 2.5
+    s[:, :, 0] -= 2.5
+    nw_affine_case("nw_affine_boundary_b3_5x7", s, -2.0, -0.5, 1.0)
+    for i, (tag, shape, go, ge, T) in enumerate([("1x1", (1, 1, 1), -2.0, -0.5, 1.0),
+                                                 ("1x3", (1, 1, 3), -2.0, -0.5, 1.0),
+                                                 ("3x1", (1, 3, 1), -2.0, -0.5, 1.0),
+                                                 ("gaps_pos", (2, 8, 10), 0.5, 0.2, 1.0)]):
+        s = _draw(90 + i, shape, extra=0)[0]
+        nw_affine_case(f"nw_affine_edge_{tag}", s, go, ge, T)
     print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
 
 

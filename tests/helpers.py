@@ -80,6 +80,29 @@ def brute_affine(s, go, ge, T):
     return _lse(totals, T)
 
 
+def brute_nw_affine(s, go, ge, T):
+    """Global affine alignment: every path from the start (0, 0) in M to the
+    cell (L1, L2) in any state; moves diag into M (+ the new cell's score),
+    down into I (+ge from I, +go otherwise), right into D (+ge from D, +go
+    otherwise).  An empty side leaves one gap run (or the empty path)."""
+    L1, L2 = s.shape
+    totals = []
+
+    def walk(i, j, st, acc):
+        if i == L1 and j == L2:
+            totals.append(acc)
+            return
+        if i < L1 and j < L2:
+            walk(i + 1, j + 1, "M", acc + s[i, j])
+        if i < L1:
+            walk(i + 1, j, "I", acc + (ge if st == "I" else go))
+        if j < L2:
+            walk(i, j + 1, "D", acc + (ge if st == "D" else go))
+
+    walk(0, 0, "M", 0.0)
+    return _lse(totals, T)
+
+
 def brute_linear(s, g, T):
     """Linear gap: start at any cell with its score; moves diag (+s of the new
     cell), down (+g), right (+g); every prefix counts; empty alignment 0."""
