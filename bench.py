@@ -399,12 +399,29 @@ def embed_workload(torch, args, dev):
     print(json.dumps(line), flush=True)
 
 
+def nw_traffic(kernel: str):
+    """HBM bytes per launch of the NW forward or reverse sweep from the
+    committed FETCH_SIZE / WRITE_SIZE profile of the nw workload, or None."""
+    path = os.path.join(ROOT, "profiles", "nw_affine_ragged2048_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            table = json.loads(line.split(":", 1)[1])
+            key = "nw_affine_forward_kernel<true" if kernel == "forward" else "nw_affine_backward_kernel"
+            for k, v in table.items():
+                if k.startswith(key):
+                    return int(v)
+    return None
+
+
 def nw_workload(torch, args, dev, hip):
     """Needleman-Wunsch affine (global alignment) fwd+bwd on the ragged batch
     of the headline workload (2048 pairs, L1, L2 ~ U[64, 1024], seed 2,
     go=-8 ge=-0.5 T=0.5, packed layout).  Roofline: the forward streams the
-    scores in and M, I, D out (16 B per cell), the reverse streams M, I, D
-    and the scores in and the map out (20 B per cell)."""
+    scores in and the two table planes (M - I, D - I) out, 12 B per cell; the
+    reverse streams the table and the scores in and the map out, 16 B per
+    cell."""
     from orihime_amd import _lib, ragged
     params = (-8.0, -0.5, 0.5)
     pairs = ragged_pairs(torch)
@@ -432,7 +449,7 @@ def nw_workload(torch, args, dev, hip):
     kd = kernel_durations(torch, _lib, hip, launch, max(3, min(args.steps, 10)))
     fwd_ms, rev_ms = kd["forward"], kd["reverse"]
     gcups = cells / (ms * 1e-3) / 1e9
-    byts = {"forward": 16.0 * cells, "reverse": 20.0 * cells}
+    byts = {"forward": 12.0 * cells, "reverse": 16.0 * cells}
     dom = "reverse" if rev_ms >= fwd_ms else "forward"
     dom_ms = rev_ms if dom == "reverse" else fwd_ms
     achieved = byts[dom] / (dom_ms * 1e-3) / 1e9
@@ -450,7 +467,7 @@ def nw_workload(torch, args, dev, hip):
                    "kernel_ms": {k: round(v, 4) for k, v in kd.items()}},
         "roofline": {"bound": "hbm", "kernel": f"nw_affine_{dom}", "achieved": round(achieved, 1),
                      "peak": 8000.0, "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
-                     "traffic": None, "bytes_per_cell": 20 if dom == "reverse" else 16},
+                     "traffic": nw_traffic(dom), "bytes_per_cell": 16 if dom == "reverse" else 12},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
