@@ -2,7 +2,7 @@
 """Soft Smith-Waterman (affine gap) forward+backward throughput on one GPU.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload ragged|fixed|long|embed|nw]
+                    [--workload ragged|fixed|long|embed|nw|sv]
 
 Workload "ragged" (default): 2048 pairs with L1, L2 drawn independently and
 uniformly from [64, 1024], then each pair's N(0,1) FP32 scores in pair order
@@ -15,8 +15,9 @@ against a reverse sweep that reads a full M/I/D table.
 Workload "embed": 512 pairs of BF16 embeddings Q [M, 128], K [N, 128] with M,
 N uniform in [128, 768] (seed 4); scores Q K^T by a BF16 GEMM with FP32
 accumulation, the ragged sweep, dQ and dK through the map.
-Workload "nw": the ragged batch of "ragged" through the Needleman-Wunsch
-(global) affine sweeps: value, map and parameter gradients.
+Workload "nw" / "sv": the ragged batch of "ragged" through the
+Needleman-Wunsch (global) / Saigo-Vert affine sweeps: value, map and
+parameter gradients.
 
 A step is one forward (value, log-partition) plus backward (marginal map and
 the gradients of gap_open, gap_extend and temperature) over the whole batch:
@@ -399,45 +400,48 @@ def embed_workload(torch, args, dev):
     print(json.dumps(line), flush=True)
 
 
-def nw_traffic(kernel: str):
-    """HBM bytes per launch of the NW forward or reverse sweep from the
-    committed FETCH_SIZE / WRITE_SIZE profile of the nw workload, or None."""
-    path = os.path.join(ROOT, "profiles", "nw_affine_ragged2048_hbm_traffic.md")
+def nw_traffic(kernel: str, family: str = "nw"):
+    """HBM bytes per launch of the NW (SV) forward or reverse sweep from the
+    committed FETCH_SIZE / WRITE_SIZE profile of the workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"{family}_affine_ragged2048_hbm_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
             table = json.loads(line.split(":", 1)[1])
-            key = "nw_affine_forward_kernel<true" if kernel == "forward" else "nw_affine_backward_kernel"
+            key = (f"{family}_affine_forward_kernel<true" if kernel == "forward"
+                   else f"{family}_affine_backward_kernel")
             for k, v in table.items():
                 if k.startswith(key):
                     return int(v)
     return None
 
 
-def nw_workload(torch, args, dev, hip):
-    """Needleman-Wunsch affine (global alignment) fwd+bwd on the ragged batch
-    of the headline workload (2048 pairs, L1, L2 ~ U[64, 1024], seed 2,
-    go=-8 ge=-0.5 T=0.5, packed layout).  Roofline: the forward streams the
-    scores in and the two table planes (M - I, D - I) out, 12 B per cell; the
-    reverse streams the table and the scores in and the map out, 16 B per
-    cell."""
+def nw_workload(torch, args, dev, hip, family="nw"):
+    """Needleman-Wunsch (family "nw") or Saigo-Vert ("sv") affine fwd+bwd on
+    the ragged batch of the headline workload (2048 pairs, L1, L2 ~ U[64,
+    1024], seed 2, go=-8 ge=-0.5 T=0.5, packed layout).  Roofline: the forward
+    streams the scores in and the table out (NW: two planes, M - I and D - I;
+    SV: three, M, I - M, D - M), the reverse streams the table and the scores
+    in and the map out: 12 / 16 B per cell (NW), 16 / 20 (SV)."""
     from orihime_amd import _lib, ragged
+    planes = 2 if family == "nw" else 3
+    call = _lib.nw_affine_forward if family == "nw" else _lib.sv_affine_forward
+    ws_fn = _lib._ws_nw_affine if family == "nw" else _lib._ws_sv_affine
     params = (-8.0, -0.5, 0.5)
     pairs = ragged_pairs(torch)
     flat, lengths, offsets = ragged.pack(pairs)
     flat, lengths, offsets = flat.to(dev), lengths.to(dev), offsets.to(dev)
     cells = int(sum(p.numel() for p in pairs))
     B, L1max, L2max = len(pairs), 1024, 1024
-    ws = torch.empty(int(_lib._ws_nw_affine(1, B, L1max, L2max, flat.numel())), dtype=torch.uint8,
-                     device=dev)
+    ws = torch.empty(int(ws_fn(1, B, L1max, L2max, flat.numel())), dtype=torch.uint8, device=dev)
     value = torch.empty(B, device=dev)
     mp = torch.empty_like(flat)
     g = [torch.empty(B, device=dev) for _ in range(3)]
 
     class _Nw:
         def step(self):
-            _lib.nw_affine_forward(flat, lengths, offsets, *params, value=value, map=mp,
+            call(flat, lengths, offsets, *params, value=value, map=mp,
                                    grad_open=g[0], grad_ext=g[1], grad_temp=g[2], L1max=L1max,
                                    L2max=L2max, workspace=ws)
 
@@ -449,25 +453,26 @@ def nw_workload(torch, args, dev, hip):
     kd = kernel_durations(torch, _lib, hip, launch, max(3, min(args.steps, 10)))
     fwd_ms, rev_ms = kd["forward"], kd["reverse"]
     gcups = cells / (ms * 1e-3) / 1e9
-    byts = {"forward": 12.0 * cells, "reverse": 16.0 * cells}
+    byts = {"forward": (4.0 + 4 * planes) * cells, "reverse": (8.0 + 4 * planes) * cells}
     dom = "reverse" if rev_ms >= fwd_ms else "forward"
     dom_ms = rev_ms if dom == "reverse" else fwd_ms
     achieved = byts[dom] / (dom_ms * 1e-3) / 1e9
-    log(f"nw affine fwd+bwd: {ms:.3f} ms/step, {gcups:.1f} GCUPS; forward {fwd_ms:.3f} ms "
+    log(f"{family} affine fwd+bwd: {ms:.3f} ms/step, {gcups:.1f} GCUPS; forward {fwd_ms:.3f} ms "
         f"({byts['forward'] / fwd_ms / 1e6:.0f} GB/s), reverse {rev_ms:.3f} ms "
         f"({byts['reverse'] / rev_ms / 1e6:.0f} GB/s)")
     line = {
-        "metric": "GCUPS soft-NW-affine fwd+bwd (value, map, parameter grads), 1 GPU",
+        "metric": f"GCUPS soft-{family.upper()}-affine fwd+bwd (value, map, parameter grads), 1 GPU",
         "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "ragged affine soft-NW (global) fwd+bwd, 2048 pairs, L1,L2 ~ "
+        "config": {"workload": f"ragged affine soft-{family.upper()} fwd+bwd, 2048 pairs, L1,L2 ~ "
                                "U[64,1024] (seed 2), gap_open=-8 gap_extend=-0.5 T=0.5, packed",
                    "cells": cells, "wall_ms_per_step": round(wall, 4),
                    "kernel_ms": {k: round(v, 4) for k, v in kd.items()}},
-        "roofline": {"bound": "hbm", "kernel": f"nw_affine_{dom}", "achieved": round(achieved, 1),
-                     "peak": 8000.0, "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
-                     "traffic": nw_traffic(dom), "bytes_per_cell": 16 if dom == "reverse" else 12},
+        "roofline": {"bound": "hbm", "kernel": f"{family}_affine_{dom}",
+                     "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "traffic": nw_traffic(dom, family),
+                     "bytes_per_cell": int(byts[dom] / cells)},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -478,7 +483,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("ragged", "fixed", "long", "embed", "nw"),
+    ap.add_argument("--workload", choices=("ragged", "fixed", "long", "embed", "nw", "sv"),
                     default="ragged")
     ap.add_argument("--no-padded", action="store_true", help="skip the padded comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -538,8 +543,8 @@ def main():
         cpu_pairs = [x[i].numpy() for i in range(x.shape[0])]
     elif args.workload == "embed":
         return embed_workload(torch, args, dev)
-    elif args.workload == "nw":
-        return nw_workload(torch, args, dev, Hip())
+    elif args.workload in ("nw", "sv"):
+        return nw_workload(torch, args, dev, Hip(), args.workload)
     else:
         params = (-11.0, -1.0, 1.0)
         gen = torch.Generator().manual_seed(3)

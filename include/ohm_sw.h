@@ -193,6 +193,29 @@ ohm_status ohm_nw_affine_forward(const float* scores, const int32_t* lengths,
                                  void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Saigo-Vert, affine gap (local alignment counting every monotone matched- */
+/* pair skeleton once): D also opens from I (no D -> I edge), paths end in */
+/* M only, S = LSE(0, {M(i,j)}).                                           */
+/* ------------------------------------------------------------------------ */
+
+/* Workspace bytes; op is OHM_OP_VALUE or OHM_OP_FORWARD (0 otherwise). */
+size_t ohm_sv_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
+                               int64_t total_cells);
+
+/* Value, map and scalar-parameter gradients, arguments as
+ * ohm_sw_affine_forward (the gradients need the map buffer); a pair with an
+ * empty side has value 0 (the empty alignment).  Replaces: sv_affine_forward,
+ * sv_affine_value_grad_params, soft_sv_affine_with_grads
+ * (src/sv_affine/registry.cpp) and the launchers sv_affine_forward_hip /
+ * sv_affine_backward_hip (src/sv_affine/kernels_gpu.hip). */
+ohm_status ohm_sv_affine_forward(const float* scores, const int32_t* lengths,
+                                 const int64_t* offsets, int32_t B, int32_t L1max,
+                                 int32_t L2max, float gap_open, float gap_ext, float temp,
+                                 float* value, float* map, float* grad_open, float* grad_ext,
+                                 float* grad_temp, void* workspace, size_t workspace_bytes,
+                                 void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Host (CPU) twins: same arguments on host memory, no workspace or stream. */
 /* `threads` <= 0 uses every hardware thread.  Replace the reference CPU     */
 /* dispatch (src/sw_affine/torch_cpu.cpp:593-707, src/sw/torch_cpu.cpp:522-591) */
@@ -216,6 +239,14 @@ ohm_status ohm_sw_affine_tangent_cpu(const float* scores, const int32_t* lengths
 /* Replaces nw_affine_forward_cpu / nw_affine_backward_cpu
  * (src/nw_affine/kernels_cpu.cpp:100-335). */
 ohm_status ohm_nw_affine_forward_cpu(const float* scores, const int32_t* lengths,
+                                     const int64_t* offsets, int32_t B, int32_t L1max,
+                                     int32_t L2max, float gap_open, float gap_ext, float temp,
+                                     float* value, float* map, float* grad_open,
+                                     float* grad_ext, float* grad_temp, int32_t threads);
+
+/* Replaces sv_affine_forward_cpu / sv_affine_backward_cpu
+ * (src/sv_affine/kernels_cpu.cpp:96-356). */
+ohm_status ohm_sv_affine_forward_cpu(const float* scores, const int32_t* lengths,
                                      const int64_t* offsets, int32_t B, int32_t L1max,
                                      int32_t L2max, float gap_open, float gap_ext, float temp,
                                      float* value, float* map, float* grad_open,

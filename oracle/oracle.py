@@ -18,6 +18,8 @@ runs and how it contracts them:
                             contracted with the cotangent and summed)
 * ``nw_affine_forward``  -> src/nw_affine/torch_cpu.cpp ``soft_nw_affine_cpu_with_grads``
                             (global alignment: forward + backward)
+* ``sv_affine_forward``  -> src/sv_affine/torch_cpu.cpp ``soft_sv_affine_cpu_with_grads``
+                            (Saigo-Vert: forward + backward)
 
 All arrays are numpy; ``lengths`` defaults to the full shape.
 """
@@ -65,6 +67,10 @@ def ref_available() -> bool:
 
 def nw_ref_available() -> bool:
     return os.path.exists(os.path.join(_HERE, "_ref", "libnw_affine_ref.so"))
+
+
+def sv_ref_available() -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", "libsv_affine_ref.so"))
 
 
 def _ref_lib(name: str):
@@ -303,20 +309,19 @@ def sw_backward_full(scores, G, gap, temp, lengths=None, impl="oracle", prec="f3
 # Needleman-Wunsch, affine gap (src/nw_affine/torch_cpu.cpp)
 # --------------------------------------------------------------------------
 
-def nw_affine_forward(scores, gap_open, gap_ext, temp, lengths=None, impl="oracle", prec="f32"):
-    """(value[B], map[B,L1,L2], grad_open[B], grad_ext[B], grad_temp[B]) of
-    the global (Needleman-Wunsch) affine alignment."""
+def _three_state_forward(family, scores, gap_open, gap_ext, temp, lengths=None, impl="oracle", prec="f32"):
     dtype = np.float32 if prec == "f32" else np.float64
     C = ctypes.c_float if prec == "f32" else ctypes.c_double
     if impl == "ref":
         if prec != "f32":
             raise ValueError("the reference kernels are FP32 only")
-        lib = _ref_lib("nw_affine")
-        fwd, bwd = lib.nw_affine_forward_cpu, lib.nw_affine_backward_cpu
+        lib = _ref_lib(f"{family}_affine")
+        fwd = getattr(lib, f"{family}_affine_forward_cpu")
+        bwd = getattr(lib, f"{family}_affine_backward_cpu")
     else:
         lib = _lib()
-        fwd = getattr(lib, f"oracle_{prec}_nwa_forward")
-        bwd = getattr(lib, f"oracle_{prec}_nwa_backward")
+        fwd = getattr(lib, f"oracle_{prec}_{family}a_forward")
+        bwd = getattr(lib, f"oracle_{prec}_{family}a_backward")
     s = _arr(scores, dtype)
     ln = _lens(s, lengths)
     B, L1, L2 = s.shape
@@ -330,3 +335,15 @@ def nw_affine_forward(scores, gap_open, gap_ext, temp, lengths=None, impl="oracl
     bwd(_ptr(alpha), _ptr(s), _ptr(part), _ptr(beta), _ptr(post), _ptr(g[0]), _ptr(g[1]),
         _ptr(g[2]), _ptr(ln), B, L1, L2, C(gap_open), C(gap_ext), C(temp))
     return part, post, g[0], g[1], g[2]
+
+
+def nw_affine_forward(scores, gap_open, gap_ext, temp, lengths=None, impl="oracle", prec="f32"):
+    """(value[B], map[B,L1,L2], grad_open[B], grad_ext[B], grad_temp[B]) of
+    the global (Needleman-Wunsch) affine alignment."""
+    return _three_state_forward("nw", scores, gap_open, gap_ext, temp, lengths, impl, prec)
+
+
+def sv_affine_forward(scores, gap_open, gap_ext, temp, lengths=None, impl="oracle", prec="f32"):
+    """(value[B], map[B,L1,L2], grad_open[B], grad_ext[B], grad_temp[B]) of
+    the Saigo-Vert affine alignment."""
+    return _three_state_forward("sv", scores, gap_open, gap_ext, temp, lengths, impl, prec)

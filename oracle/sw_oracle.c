@@ -941,3 +941,127 @@ void OFN(nwa_backward)(const real* alpha, const real* scores, const real* part, 
         gT[b] = (part[b] - (ms.s + so * go + se * ge)) / T;
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Saigo-Vert, affine gap: SW affine's states plus the I -> D open edge, no  */
+/* D -> I edge, paths end in M only.                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Candidates at cell (i, j), src/sv_affine/kernels_cpu.cpp:138-167 */
+static void sva_cands(const real* a, geom g, int i, int j, real sc, real go, real ge,
+                      real* cm, real* ci, real* cd) {
+    size_t dg = CELL(g, i - 1, j - 1), up = CELL(g, i - 1, j), lf = CELL(g, i, j - 1);
+    int in = (i > 1 && j > 1);
+    cm[0] = in ? a[0 * g.cs + dg] + sc : NINF_;
+    cm[1] = in ? a[1 * g.cs + dg] + sc : NINF_;
+    cm[2] = in ? a[2 * g.cs + dg] + sc : NINF_;
+    cm[3] = sc;
+    ci[0] = (i > 1) ? a[0 * g.cs + up] + go : NINF_;
+    ci[1] = (i > 1) ? a[1 * g.cs + up] + ge : NINF_;
+    cd[0] = (j > 1) ? a[0 * g.cs + lf] + go : NINF_;
+    cd[1] = (j > 1) ? a[1 * g.cs + lf] + go : NINF_;
+    cd[2] = (j > 1) ? a[2 * g.cs + lf] + ge : NINF_;
+}
+
+/* sv_affine_forward_cpu, src/sv_affine/kernels_cpu.cpp:96-196: partition
+ * LSE(0, {M}), Kahan sum of the empty alignment first, then the M states in
+ * row-major order. */
+void OFN(sva_forward)(const real* scores, real* alpha, real* part, const int* len,
+                      int B, int L1m, int L2m, real go, real ge, real T) {
+    geom g = mk_geom(L1m, L2m);
+    for (int b = 0; b < B; ++b) {
+        const real* s = scores + (size_t)b * g.ss;
+        real* a = alpha + (size_t)b * 3 * g.cs;
+        int L1 = len[2 * b], L2 = len[2 * b + 1];
+        for (size_t t = 0; t < 3 * g.cs; ++t) a[t] = NINF_;
+        a[0] = 0;
+        for (int k = 2; k <= L1 + L2; ++k) {
+            int lo = k - L2 > 1 ? k - L2 : 1, hi = k - 1 < L1 ? k - 1 : L1;
+            for (int i = lo; i <= hi; ++i) {
+                int j = k - i;
+                real cm[4], ci[2], cd[3];
+                sva_cands(a, g, i, j, s[SIDX(g, i, j)], go, ge, cm, ci, cd);
+                size_t c = CELL(g, i, j);
+                a[0 * g.cs + c] = lse_n(cm, 4, T);
+                a[1 * g.cs + c] = lse_n(ci, 2, T);
+                a[2 * g.cs + c] = lse_n(cd, 3, T);
+            }
+        }
+        real m = 0;
+        for (int i = 1; i <= L1; ++i)
+            for (int j = 1; j <= L2; ++j) m = rmax(m, a[CELL(g, i, j)]);
+        kahan z = {0, 0};
+        kadd(&z, sexp((0 - m) / T));
+        for (int i = 1; i <= L1; ++i)
+            for (int j = 1; j <= L2; ++j) {
+                real v = a[CELL(g, i, j)];
+                if (v > NINF_) kadd(&z, sexp((v - m) / T));
+            }
+        part[b] = m + T * OLOG(z.s);
+    }
+}
+
+/* sv_affine_backward_cpu, src/sv_affine/kernels_cpu.cpp:203-356: terminal
+ * weights on the M states only, push-form adjoint, I -> D counted as an
+ * open. */
+void OFN(sva_backward)(const real* alpha, const real* scores, const real* part, real* beta,
+                       real* post, real* gopen, real* gext, real* gT, const int* len,
+                       int B, int L1m, int L2m, real go, real ge, real T) {
+    geom g = mk_geom(L1m, L2m);
+    for (int b = 0; b < B; ++b) {
+        const real* a = alpha + (size_t)b * 3 * g.cs;
+        const real* s = scores + (size_t)b * g.ss;
+        real* be = beta + (size_t)b * 3 * g.cs;
+        real* P = post + (size_t)b * g.ss;
+        real S = part[b];
+        int L1 = len[2 * b], L2 = len[2 * b + 1];
+        memset(P, 0, g.ss * sizeof(real));
+        memset(be, 0, 3 * g.cs * sizeof(real));
+        for (int i = 1; i <= L1; ++i)
+            for (int j = 1; j <= L2; ++j) be[CELL(g, i, j)] = post_w(a[CELL(g, i, j)], S, T);
+        real so = 0, se = 0;
+        for (int k = L1 + L2; k >= 2; --k) {
+            int lo = k - L2 > 1 ? k - L2 : 1, hi = k - 1 < L1 ? k - 1 : L1;
+            for (int i = lo; i <= hi; ++i) {
+                int j = k - i;
+                size_t c = CELL(g, i, j), dg = CELL(g, i - 1, j - 1);
+                size_t up = CELL(g, i - 1, j), lf = CELL(g, i, j - 1);
+                real cm[4], ci[2], cd[3], w[4];
+                sva_cands(a, g, i, j, s[SIDX(g, i, j)], go, ge, cm, ci, cd);
+                real bm = be[0 * g.cs + c];
+                if (!is_bad(bm)) {
+                    wsoft(cm, 4, T, w);
+                    P[SIDX(g, i, j)] += bm * (w[0] + w[1] + w[2] + w[3]);
+                    if (i > 1 && j > 1) {
+                        if (w[0] > 0) be[0 * g.cs + dg] += bm * w[0];
+                        if (w[1] > 0) be[1 * g.cs + dg] += bm * w[1];
+                        if (w[2] > 0) be[2 * g.cs + dg] += bm * w[2];
+                    }
+                }
+                real bi = be[1 * g.cs + c];
+                if (!is_bad(bi)) {
+                    wsoft(ci, 2, T, w);
+                    if (i > 1) {
+                        if (w[0] > 0) { be[0 * g.cs + up] += bi * w[0]; so += bi * w[0]; }
+                        if (w[1] > 0) { be[1 * g.cs + up] += bi * w[1]; se += bi * w[1]; }
+                    }
+                }
+                real bd = be[2 * g.cs + c];
+                if (!is_bad(bd)) {
+                    wsoft(cd, 3, T, w);
+                    if (j > 1) {
+                        if (w[0] > 0) { be[0 * g.cs + lf] += bd * w[0]; so += bd * w[0]; }
+                        if (w[1] > 0) { be[1 * g.cs + lf] += bd * w[1]; so += bd * w[1]; }
+                        if (w[2] > 0) { be[2 * g.cs + lf] += bd * w[2]; se += bd * w[2]; }
+                    }
+                }
+            }
+        }
+        kahan ms = {0, 0};
+        for (size_t x = 0; x < g.ss; ++x) kadd(&ms, P[x] * s[x]);
+        real gap_cost = so * go + se * ge;  /* kernels_cpu.cpp:348-349 */
+        gopen[b] = so;
+        gext[b] = se;
+        gT[b] = (S - (ms.s + gap_cost)) / T;
+    }
+}

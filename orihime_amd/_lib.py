@@ -50,6 +50,9 @@ EXPORTS = (
     "ohm_nw_affine_workspace",
     "ohm_nw_affine_forward",
     "ohm_nw_affine_forward_cpu",
+    "ohm_sv_affine_workspace",
+    "ohm_sv_affine_forward",
+    "ohm_sv_affine_forward_cpu",
 )
 
 
@@ -67,6 +70,7 @@ _launch_config = _sig("ohm_set_launch_config", ctypes.c_int, _i32, _i32, _i32)
 _ws_affine = _sig("ohm_sw_affine_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 _ws_linear = _sig("ohm_sw_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 _ws_nw_affine = _sig("ohm_nw_affine_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
+_ws_sv_affine = _sig("ohm_sv_affine_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 
 _PROBLEM = (_vp, _vp, _vp, _i32, _i32, _i32)  # scores, lengths, offsets, B, L1max, L2max
 _aff_fwd = _sig("ohm_sw_affine_forward", _i32, *_PROBLEM, _f, _f, _f,
@@ -88,6 +92,10 @@ _lin_tan_cpu = _sig("ohm_sw_tangent_cpu", _i32, *_PROBLEM, _f, _f,
 _nwa_fwd = _sig("ohm_nw_affine_forward", _i32, *_PROBLEM, _f, _f, _f,
                 _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp)
 _nwa_fwd_cpu = _sig("ohm_nw_affine_forward_cpu", _i32, *_PROBLEM, _f, _f, _f,
+                    _vp, _vp, _vp, _vp, _vp, _i32)
+_sva_fwd = _sig("ohm_sv_affine_forward", _i32, *_PROBLEM, _f, _f, _f,
+                _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp)
+_sva_fwd_cpu = _sig("ohm_sv_affine_forward_cpu", _i32, *_PROBLEM, _f, _f, _f,
                     _vp, _vp, _vp, _vp, _vp, _i32)
 
 OP_VALUE, OP_FORWARD, OP_TANGENT = 0, 1, 2
@@ -255,3 +263,23 @@ def nw_affine_forward(scores, lengths, offsets, gap_open, gap_ext, temp, *, valu
         _check(_nwa_fwd(*args, ws.data_ptr(), ws.numel(), stream), "ohm_nw_affine_forward")
     else:
         _check(_nwa_fwd_cpu(*args, _cpu_threads()), "ohm_nw_affine_forward_cpu")
+
+
+def sv_affine_forward(scores, lengths, offsets, gap_open, gap_ext, temp, *, value, map=None,
+                      grad_open=None, grad_ext=None, grad_temp=None, L1max=None, L2max=None,
+                      workspace=None):
+    """Saigo-Vert affine: value, and map + parameter gradients when ``map`` is
+    given."""
+    B, L1, L2 = _dims(scores, lengths, offsets, L1max, L2max)
+    args = (_ptr(scores), _ptr(lengths), _ptr(offsets), B, L1, L2,
+            float(gap_open), float(gap_ext), float(temp),
+            _ptr(value), _ptr(map), _ptr(grad_open), _ptr(grad_ext), _ptr(grad_temp))
+    if scores.is_cuda:
+        op = OP_FORWARD if map is not None else OP_VALUE
+        total = -1 if offsets is None else scores.numel()
+        need = int(_ws_sv_affine(op, B, L1, L2, total))
+        stream = torch.cuda.current_stream(scores.device).cuda_stream
+        ws = workspace if workspace is not None else _workspace(scores.device, need, stream)
+        _check(_sva_fwd(*args, ws.data_ptr(), ws.numel(), stream), "ohm_sv_affine_forward")
+    else:
+        _check(_sva_fwd_cpu(*args, _cpu_threads()), "ohm_sv_affine_forward_cpu")

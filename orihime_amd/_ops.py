@@ -297,64 +297,89 @@ def soft_sw_param_jacobian_impl(scores, param_type: int, gap: float, temperature
 
 
 # --------------------------------------------------------------------------
-# Needleman-Wunsch, affine gap (src/nw_affine/registry.cpp:33-46)
+# Needleman-Wunsch and Saigo-Vert, affine gap (src/nw_affine/registry.cpp:33-46,
+# src/sv_affine/registry.cpp:29-44): forward + reverse sweeps for value, map
+# and parameter gradients; the second-order schemas have no kernel yet.
 # --------------------------------------------------------------------------
 
 
-def _nwa_forward(scores, go, ge, temp, lengths, want_map=True, want_grads=False):
-    _check_scores(scores)
-    ln = _check_lengths(lengths, scores)
-    B = scores.shape[0]
-    value = _vec(scores, B)
-    # the gradients come out of the reverse sweep that writes the map
-    mp = torch.empty_like(scores) if (want_map or want_grads) else None
-    g = [_vec(scores, B) for _ in range(3)] if want_grads else [None] * 3
-    _lib.nw_affine_forward(scores, ln, None, go, ge, temp, value=value, map=mp,
-                           grad_open=g[0], grad_ext=g[1], grad_temp=g[2])
-    return value, mp, g
+def _three_state_family(family: str, call, value_first: bool):
+    """Implementations of one family's first-order schemas.  ``call`` is the
+    ``_lib`` entry point; ``value_first``: ``*_value_grad_params`` returns the
+    value before the three gradients (NW, src/nw_affine/torch_cpu.cpp:596-610)
+    or the gradients only (SV, src/sv_affine/torch_cpu.cpp:615-624)."""
+
+    def run(scores, go, ge, temp, lengths, want_map=True, want_grads=False):
+        _check_scores(scores)
+        ln = _check_lengths(lengths, scores)
+        B = scores.shape[0]
+        value = _vec(scores, B)
+        # the gradients come out of the reverse sweep that writes the map
+        mp = torch.empty_like(scores) if (want_map or want_grads) else None
+        g = [_vec(scores, B) for _ in range(3)] if want_grads else [None] * 3
+        call(scores, ln, None, go, ge, temp, value=value, map=mp, grad_open=g[0],
+             grad_ext=g[1], grad_temp=g[2])
+        return value, mp, g
+
+    def forward(scores, gap_open: float, gap_ext: float, temp: float, lengths=None):
+        value, mp, _ = run(scores, gap_open, gap_ext, temp, lengths)
+        return [value, mp]
+
+    def forward_t(scores, gap_open, gap_ext, temp, lengths):
+        go = _scalar(gap_open, "gap_open")
+        ge = _scalar(gap_ext, "gap_ext")
+        T = _scalar(temp, "temperature")
+        value, mp, g = run(scores, go, ge, T, lengths, want_grads=True)
+        setattr(value, _PARAM_GRADS, tuple(g))
+        return [value, mp]
+
+    def value_grad_params(scores, gap_open: float, gap_ext: float, temp: float, lengths=None):
+        value, _, g = run(scores, gap_open, gap_ext, temp, lengths, want_map=False,
+                          want_grads=True)
+        return (value, g[0], g[1], g[2]) if value_first else (g[0], g[1], g[2])
+
+    def with_grads(scores, gap_open: float, gap_ext: float, temperature: float, lengths=None):
+        value, mp, g = run(scores, gap_open, gap_ext, temperature, lengths, want_grads=True)
+        return value, mp, g[0], g[1], g[2]
+
+    def value_only(scores, gap_open: float, gap_ext: float, temp: float, lengths=None):
+        value, _, _ = run(scores, gap_open, gap_ext, temp, lengths, want_map=False)
+        return value
+
+    def second_order(what: str):
+        def impl(scores, *args):
+            raise NotImplementedError(
+                f"{family}_affine {what} (second-order derivatives of the map) has no kernel in "
+                "orihime_amd yet; the value, map and parameter gradients are available")
+
+        return impl
+
+    return forward, forward_t, value_grad_params, with_grads, value_only, second_order
 
 
-def nw_affine_forward_impl(scores, gap_open: float, gap_ext: float, temp: float, lengths=None):
-    value, mp, _ = _nwa_forward(scores, gap_open, gap_ext, temp, lengths)
-    return [value, mp]
-
-
-def nw_affine_forward_t_impl(scores, gap_open, gap_ext, temp, lengths):
-    go = _scalar(gap_open, "gap_open")
-    ge = _scalar(gap_ext, "gap_ext")
-    T = _scalar(temp, "temperature")
-    value, mp, g = _nwa_forward(scores, go, ge, T, lengths, want_grads=True)
-    setattr(value, _PARAM_GRADS, tuple(g))
-    return [value, mp]
-
-
-def nw_affine_value_grad_params_impl(scores, gap_open: float, gap_ext: float, temp: float,
-                                     lengths=None):
-    """(value, dS/dgap_open, dS/dgap_ext, dS/dT): src/nw_affine/torch_cpu.cpp:596-610."""
-    value, _, g = _nwa_forward(scores, gap_open, gap_ext, temp, lengths, want_map=False,
-                               want_grads=True)
-    return value, g[0], g[1], g[2]
-
-
-def soft_nw_affine_with_grads_impl(scores, gap_open: float, gap_ext: float, temperature: float,
-                                   lengths=None):
-    value, mp, g = _nwa_forward(scores, gap_open, gap_ext, temperature, lengths,
-                                want_grads=True)
-    return value, mp, g[0], g[1], g[2]
-
-
-def nw_affine_value_only_impl(scores, gap_open: float, gap_ext: float, temp: float, lengths=None):
-    value, _, _ = _nwa_forward(scores, gap_open, gap_ext, temp, lengths, want_map=False)
-    return value
-
-
-def _nwa_second_order(what: str):
-    def impl(scores, *args):
-        raise NotImplementedError(
-            f"nw_affine {what} (second-order derivatives of the Needleman-Wunsch map) has no "
-            "kernel in orihime_amd yet; the value, map and parameter gradients are available")
-
-    return impl
+def _three_state_schemas(family: str, call, value_first: bool) -> dict:
+    fwd, fwd_t, vgp, wg, vo, so = _three_state_family(family, call, value_first)
+    f = family
+    S = "(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths)"
+    L = "(Tensor scores, float gap_open, float gap_ext, float temperature, Tensor? lengths)"
+    grads = "(Tensor, Tensor, Tensor, Tensor)" if value_first else "(Tensor, Tensor, Tensor)"
+    return {
+        f"{f}_affine_forward": (f"{S} -> Tensor[]", fwd),
+        f"{f}_affine_forward_t": ("(Tensor scores, Tensor gap_open, Tensor gap_ext, Tensor temp, Tensor lengths) -> Tensor[]", fwd_t),
+        f"{f}_affine_value_grad_params": (f"{S} -> {grads}", vgp),
+        f"{f}_affine_marginals_backward": ("(Tensor scores, Tensor grad_marginals, float gap_open, float gap_ext, float temp, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor)", so("marginals_backward")),
+        f"{f}_affine_marginals_hvp": ("(Tensor scores, Tensor v, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", so("marginals_hvp")),
+        f"{f}_affine_marginals_grad_gap_open": (f"{S} -> Tensor", so("marginals_grad_gap_open")),
+        f"{f}_affine_marginals_grad_gap_ext": (f"{S} -> Tensor", so("marginals_grad_gap_ext")),
+        f"{f}_affine_marginals_grad_temp": (f"{S} -> Tensor", so("marginals_grad_temp")),
+        f"{f}_affine_value_only": (f"{S} -> Tensor", vo),
+        f"soft_{f}_affine": ("(Tensor scores, Tensor gap_open, Tensor gap_ext, Tensor temperature, Tensor lengths) -> Tensor[]", fwd_t),
+        f"soft_{f}_affine_float": (f"{L} -> Tensor[]", fwd),
+        f"soft_{f}_affine_with_grads": (f"{L} -> (Tensor, Tensor, Tensor, Tensor, Tensor)", wg),
+        f"soft_{f}_affine_hvp": ("(Tensor scores, Tensor tangent, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> Tensor", so("hvp")),
+        f"soft_{f}_affine_param_jacobian": ("(Tensor scores, int param_type, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> Tensor", so("param_jacobian")),
+        f"soft_{f}_affine_backward_full": ("(Tensor scores, Tensor grad_alignment, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor)", so("backward_full")),
+    }
 
 
 # --------------------------------------------------------------------------
@@ -397,24 +422,11 @@ _SCHEMAS = {
     "soft_sw_hvp": ("(Tensor scores, Tensor tangent, float gap, float temperature, Tensor? lengths) -> Tensor", sw_marginals_hvp_impl),
     "soft_sw_param_jacobian": ("(Tensor scores, int param_type, float gap, float temperature, Tensor? lengths) -> Tensor", soft_sw_param_jacobian_impl),
     "soft_sw_backward_full": ("(Tensor scores, Tensor grad_alignment, float gap, float temperature, Tensor? lengths) -> (Tensor, Tensor, Tensor)", sw_marginals_backward_impl),
-    # Needleman-Wunsch affine — namespaced API
-    "nw_affine_forward": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor[]", nw_affine_forward_impl),
-    "nw_affine_forward_t": ("(Tensor scores, Tensor gap_open, Tensor gap_ext, Tensor temp, Tensor lengths) -> Tensor[]", nw_affine_forward_t_impl),
-    "nw_affine_value_grad_params": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor)", nw_affine_value_grad_params_impl),
-    "nw_affine_marginals_backward": ("(Tensor scores, Tensor grad_marginals, float gap_open, float gap_ext, float temp, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor)", _nwa_second_order("marginals_backward")),
-    "nw_affine_marginals_hvp": ("(Tensor scores, Tensor v, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", _nwa_second_order("marginals_hvp")),
-    "nw_affine_marginals_grad_gap_open": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", _nwa_second_order("marginals_grad_gap_open")),
-    "nw_affine_marginals_grad_gap_ext": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", _nwa_second_order("marginals_grad_gap_ext")),
-    "nw_affine_marginals_grad_temp": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", _nwa_second_order("marginals_grad_temp")),
-    "nw_affine_value_only": ("(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths) -> Tensor", nw_affine_value_only_impl),
-    # Needleman-Wunsch affine — legacy
-    "soft_nw_affine": ("(Tensor scores, Tensor gap_open, Tensor gap_ext, Tensor temperature, Tensor lengths) -> Tensor[]", nw_affine_forward_t_impl),
-    "soft_nw_affine_float": ("(Tensor scores, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> Tensor[]", nw_affine_forward_impl),
-    "soft_nw_affine_with_grads": ("(Tensor scores, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor, Tensor)", soft_nw_affine_with_grads_impl),
-    "soft_nw_affine_hvp": ("(Tensor scores, Tensor tangent, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> Tensor", _nwa_second_order("hvp")),
-    "soft_nw_affine_param_jacobian": ("(Tensor scores, int param_type, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> Tensor", _nwa_second_order("param_jacobian")),
-    "soft_nw_affine_backward_full": ("(Tensor scores, Tensor grad_alignment, float gap_open, float gap_ext, float temperature, Tensor? lengths) -> (Tensor, Tensor, Tensor, Tensor)", _nwa_second_order("backward_full")),
 }
+
+
+_SCHEMAS.update(_three_state_schemas("nw", _lib.nw_affine_forward, True))
+_SCHEMAS.update(_three_state_schemas("sv", _lib.sv_affine_forward, False))
 
 
 def _fake_for(name: str, schema: str):
@@ -523,12 +535,13 @@ for _name, _aff, _tp in (
         f"orihime::{_name}", _make_forward_backward(_aff, _tp), setup_context=_forward_setup,
         lib=_LIB,
     )
-for _name, _tp in (("nw_affine_forward", False), ("nw_affine_forward_t", True),
-                   ("soft_nw_affine_float", False), ("soft_nw_affine", True)):
-    torch.library.register_autograd(
-        f"orihime::{_name}", _make_forward_backward(True, _tp, "nw_affine"),
-        setup_context=_forward_setup, lib=_LIB,
-    )
+for _fam in ("nw", "sv"):
+    for _name, _tp in ((f"{_fam}_affine_forward", False), (f"{_fam}_affine_forward_t", True),
+                       (f"soft_{_fam}_affine_float", False), (f"soft_{_fam}_affine", True)):
+        torch.library.register_autograd(
+            f"orihime::{_name}", _make_forward_backward(True, _tp, f"{_fam}_affine"),
+            setup_context=_forward_setup, lib=_LIB,
+        )
 
 # --------------------------------------------------------------------------
 # vmap: fold the mapped dimension into the native batch (orihime/_ops.py:237-406)
@@ -622,9 +635,10 @@ sw_affine_marginals_grad_gap_ext = orihime.sw_affine_marginals_grad_gap_ext
 sw_affine_marginals_grad_temp = orihime.sw_affine_marginals_grad_temp
 sw_affine_value_only = orihime.sw_affine_value_only
 
-for _op in ("nw_affine_forward", "nw_affine_forward_t", "nw_affine_value_grad_params",
-            "nw_affine_value_only"):
-    torch.library.register_vmap(f"orihime::{_op}", _make_batch_folding_vmap_rule(_op), lib=_LIB)
+for _fam in ("nw", "sv"):
+    for _op in (f"{_fam}_affine_forward", f"{_fam}_affine_forward_t",
+                f"{_fam}_affine_value_grad_params", f"{_fam}_affine_value_only"):
+        torch.library.register_vmap(f"orihime::{_op}", _make_batch_folding_vmap_rule(_op), lib=_LIB)
 
 nw_affine_forward = orihime.nw_affine_forward
 nw_affine_forward_t = orihime.nw_affine_forward_t
@@ -635,6 +649,16 @@ nw_affine_marginals_grad_gap_open = orihime.nw_affine_marginals_grad_gap_open
 nw_affine_marginals_grad_gap_ext = orihime.nw_affine_marginals_grad_gap_ext
 nw_affine_marginals_grad_temp = orihime.nw_affine_marginals_grad_temp
 nw_affine_value_only = orihime.nw_affine_value_only
+
+sv_affine_forward = orihime.sv_affine_forward
+sv_affine_forward_t = orihime.sv_affine_forward_t
+sv_affine_value_grad_params = orihime.sv_affine_value_grad_params
+sv_affine_marginals_backward = orihime.sv_affine_marginals_backward
+sv_affine_marginals_hvp = orihime.sv_affine_marginals_hvp
+sv_affine_marginals_grad_gap_open = orihime.sv_affine_marginals_grad_gap_open
+sv_affine_marginals_grad_gap_ext = orihime.sv_affine_marginals_grad_gap_ext
+sv_affine_marginals_grad_temp = orihime.sv_affine_marginals_grad_temp
+sv_affine_value_only = orihime.sv_affine_value_only
 
 
 def _has_cuda_dispatch() -> bool:

@@ -459,6 +459,83 @@ void nw_pair(const Problem& P, int b, bool want_bwd) {
     zero_pad(P, r, P.map);
 }
 
+// --------------------------------------------------------------------------
+// Saigo-Vert, affine gap, one pair: src/sv_affine/kernels_cpu.cpp:96-356.
+// Full M, I, D tables in log2 units, S = LSE(0, {M}); the pull-form adjoint
+// seeds every M state with its terminal weight 2^(M - S).
+// --------------------------------------------------------------------------
+
+void sv_pair(const Problem& P, int b, bool want_bwd) {
+    const PairRef r = pair_ref(P, b);
+    const int L1 = r.L1, L2 = r.L2;
+    if (L1 == 0 || L2 == 0) {  // the empty alignment only
+        P.value[b] = 0.0f;
+        if (P.g0) P.g0[b] = 0.0f;
+        if (P.g1) P.g1[b] = 0.0f;
+        if (P.g2) P.g2[b] = 0.0f;
+        if (want_bwd) zero_pad(P, r, P.map);
+        return;
+    }
+    const float* s = P.scores + r.off;
+    const int C = L2 + 1;
+    const size_t n = (size_t)(L1 + 1) * C;
+    auto at = [C](int i, int j) { return (size_t)i * C + j; };
+    auto l2 = [](float a, float c) {
+        float hi = std::max(a, c), lo = std::min(a, c);
+        return hi + lg2(1.0f + ex2(lo - hi));
+    };
+    std::vector<float> M(n, kNinf), I(n, kNinf), D(n, kNinf), A(n, kNinf);
+    double m = 0.0;  // the empty alignment
+    for (int i = 1; i <= L1; ++i)
+        for (int j = 1; j <= L2; ++j) {
+            const size_t c = at(i, j), dg = at(i - 1, j - 1), up = at(i - 1, j), lf = at(i, j - 1);
+            M[c] = s[(int64_t)(i - 1) * r.ld + j - 1] * P.kap + l2(A[dg], 0.0f);
+            I[c] = l2(M[up] + P.go2, I[up] + P.ge2);
+            D[c] = l2(l2(M[lf], I[lf]) + P.go2, D[lf] + P.ge2);
+            A[c] = l2(l2(M[c], I[c]), D[c]);
+            m = std::max(m, (double)M[c]);
+        }
+    double z = std::exp2(-m);
+    for (int i = 1; i <= L1; ++i)
+        for (int j = 1; j <= L2; ++j) z += std::exp2((double)M[at(i, j)] - m);
+    const double S2 = m + std::log2(z);
+    const double S = S2 / P.kap;
+    P.value[b] = (float)S;
+    if (!want_bwd) return;
+    auto softmax = [](float a, float c, float d, float* w) {
+        float hi = std::max(std::max(a, c), d);
+        float ea = ex2(a - hi), ec = ex2(c - hi), ed = ex2(d - hi), q = 1.0f / (ea + ec + ed);
+        w[0] = ea * q; w[1] = ec * q; w[2] = ed * q;
+    };
+    std::vector<float> PM(n, 0.0f), PI(n, 0.0f), PD(n, 0.0f);
+    double so = 0.0, se = 0.0, ss = 0.0;
+    for (int i = L1; i >= 1; --i)
+        for (int j = L2; j >= 1; --j) {
+            const size_t c = at(i, j);
+            float wd[3], wi[3], wx[3];
+            softmax(M[c], I[c], D[c], wd);
+            softmax(M[c] + P.go2, I[c] + P.ge2, kNinf, wi);
+            softmax(M[c] + P.go2, I[c] + P.go2, D[c] + P.ge2, wx);
+            const float sg = 1.0f / (1.0f + ex2(-A[c]));  // continuing share of M(i+1, j+1)
+            const float Pd = (i < L1 && j < L2) ? PM[at(i + 1, j + 1)] * sg : 0.0f;
+            const float Pi = i < L1 ? PI[at(i + 1, j)] : 0.0f;
+            const float Px = j < L2 ? PD[at(i, j + 1)] : 0.0f;
+            const float seed = (float)std::exp2(std::min((double)M[c] - S2, 0.0));
+            PM[c] = wd[0] * Pd + wi[0] * Pi + wx[0] * Px + seed;
+            PI[c] = wd[1] * Pd + wi[1] * Pi + wx[1] * Px;
+            PD[c] = wd[2] * Pd + wx[2] * Px;
+            so += (double)Pi * wi[0] + (double)Px * (wx[0] + wx[1]);
+            se += (double)Pi * wi[1] + (double)Px * wx[2];
+            const int64_t q = (int64_t)(i - 1) * r.ld + j - 1;
+            P.map[r.off + q] = PM[c];
+            ss += (double)PM[c] * s[q];
+        }
+    if (P.g0) P.g0[b] = (float)so;
+    if (P.g1) P.g1[b] = (float)se;
+    if (P.g2) P.g2[b] = (float)((S - (ss + (double)P.go * so + (double)P.ge * se)) / P.T);
+    zero_pad(P, r, P.map);
+}
+
 template <typename F>
 void parallel_pairs(int B, int threads, F&& fn) {
     int nt = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
@@ -542,6 +619,24 @@ extern "C" ohm_status ohm_nw_affine_forward_cpu(const float* scores, const int32
     const bool bwd = map || grad_open || grad_ext || grad_temp;
     if (bwd && !map) return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
     parallel_pairs(B, threads, [&](int b) { nw_pair(P, b, bwd); });
+    return OHM_OK;
+}
+
+extern "C" ohm_status ohm_sv_affine_forward_cpu(const float* scores, const int32_t* lengths,
+                                                const int64_t* offsets, int32_t B, int32_t L1max,
+                                                int32_t L2max, float gap_open, float gap_ext,
+                                                float temp, float* value, float* map,
+                                                float* grad_open, float* grad_ext,
+                                                float* grad_temp, int32_t threads) {
+    Problem P = make(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp);
+    P.value = value; P.map = map; P.g0 = grad_open; P.g1 = grad_ext; P.g2 = grad_temp;
+    ohm_status st = check(P);
+    if (st != OHM_OK) return st;
+    if (!std::isfinite(gap_open) || !std::isfinite(gap_ext))
+        return cpu_fail(OHM_EINVAL, "gap scores must be finite");
+    const bool bwd = map || grad_open || grad_ext || grad_temp;
+    if (bwd && !map) return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
+    parallel_pairs(B, threads, [&](int b) { sv_pair(P, b, bwd); });
     return OHM_OK;
 }
 

@@ -928,6 +928,7 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
 }
 
 #include "nw_affine.hiph"
+#include "sv_affine.hiph"
 #include "sw_teams.hiph"
 #include "sw_grid.hiph"
 #include "sw_affine_scaled.hiph"
@@ -1227,11 +1228,12 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     if (offsets && !lengths) return fail(OHM_EINVAL, "the packed layout needs lengths");
     if ((int64_t)((L1max + kStrip - 1) / kStrip) * 4 > 60 * 1024)
         return fail(OHM_EINVAL, "L1 too long for one launch (max 983040 rows)");
-    // op 3: the Needleman-Wunsch forward + reverse (full M, I, D table)
+    // op 3 / 4: the Needleman-Wunsch / Saigo-Vert forward + reverse (full tables)
     const TableSpec tf = op == 3 ? TableSpec{kPlanesNw, 1}
+                         : op == 4 ? TableSpec{kPlanesSv, 1}
                          : affine ? TableSpec{kPlanesAffineCkpt, kCkpt} : TableSpec{kPlanesLinearFwd, 1};
     const TableSpec tt = {affine ? kPlanesAffineTan : kPlanesLinearTan, 1};
-    WsLayout w = ws_layout(op == 3 ? 1 : op, B, L1max, L2max, -1, tf, tt);
+    WsLayout w = ws_layout(op >= 3 ? 1 : op, B, L1max, L2max, -1, tf, tt);
     if (!ws) return fail(OHM_EWORKSPACE, "workspace is null");
     if (ws_bytes < w.data + 256) return fail(OHM_EWORKSPACE, "workspace too small");
     if (!offsets && ws_bytes < w.total)
@@ -1479,6 +1481,35 @@ ohm_status nw_forward_impl(const float* scores, const int32_t* lengths, const in
     return check_launch("nw backward kernel");
 }
 
+// Saigo-Vert affine: value, then (when any backward output is asked for) map
+// and parameter gradients.
+ohm_status sv_forward_impl(const float* scores, const int32_t* lengths, const int64_t* offsets,
+                           int B, int L1max, int L2max, float go, float ge, float temp,
+                           float* value, float* map, float* g0, float* g1, float* g2, void* ws,
+                           size_t ws_bytes, void* stream) {
+    if (B > 0 && !value) return fail(OHM_EINVAL, "value is null");
+    const bool need_bwd = map || g0 || g1 || g2;
+    if (need_bwd && !map) return fail(OHM_EINVAL, "the gradients need the map buffer");
+    Launch L;
+    ohm_status st = setup(need_bwd ? 4 : 0, true, scores, lengths, offsets, B, L1max, L2max, go,
+                          ge, temp, ws, ws_bytes, stream, L);
+    if (st != OHM_OK || B == 0) return st;
+    L.a.value = value;
+    L.a.map = map;
+    L.a.g0 = g0;
+    L.a.g1 = g1;
+    L.a.g2 = g2;
+    if (need_bwd) OHM_DISPATCH_W(L, (sv_affine_forward_kernel<true, W>));
+    else OHM_DISPATCH_W(L, (sv_affine_forward_kernel<false, W>));
+    stage_mark(1, L.stream);
+    stage_mark(2, L.stream);
+    if ((st = check_launch("sv forward kernel")) != OHM_OK || !need_bwd) return st;
+    OHM_DISPATCH_W(L, (sv_affine_backward_kernel<W>));
+    stage_mark(3, L.stream);
+    stage_mark(4, L.stream);
+    return check_launch("sv backward kernel");
+}
+
 ohm_status tangent_impl(bool affine, const float* scores, const int32_t* lengths,
                         const int64_t* offsets, int B, int L1max, int L2max, float go, float ge,
                         float temp, const float* dir, float dgo, float dge, float dT,
@@ -1617,6 +1648,25 @@ extern "C" ohm_status ohm_nw_affine_forward(const float* scores, const int32_t* 
                                             void* workspace, size_t workspace_bytes,
                                             void* stream) {
     return nw_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
+                           value, map, grad_open, grad_ext, grad_temp, workspace,
+                           workspace_bytes, stream);
+}
+
+extern "C" size_t ohm_sv_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
+                                          int64_t total_cells) {
+    if (B < 0 || L1max < 0 || L2max < 0 || op < 0 || op > 1) return 0;
+    return ws_layout(op, B, L1max, L2max, total_cells, TableSpec{kPlanesSv, 1},
+                     TableSpec{kPlanesSv, 1}).total;
+}
+
+extern "C" ohm_status ohm_sv_affine_forward(const float* scores, const int32_t* lengths,
+                                            const int64_t* offsets, int32_t B, int32_t L1max,
+                                            int32_t L2max, float gap_open, float gap_ext,
+                                            float temp, float* value, float* map,
+                                            float* grad_open, float* grad_ext, float* grad_temp,
+                                            void* workspace, size_t workspace_bytes,
+                                            void* stream) {
+    return sv_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
                            value, map, grad_open, grad_ext, grad_temp, workspace,
                            workspace_bytes, stream);
 }

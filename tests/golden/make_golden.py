@@ -2,7 +2,7 @@
 
 Every expected output comes from the reference's own CPU kernels
 (src/sw/kernels_cpu.cpp, src/sw_affine/kernels_cpu.cpp,
-src/nw_affine/kernels_cpu.cpp) compiled from their
+src/nw_affine/kernels_cpu.cpp, src/sv_affine/kernels_cpu.cpp) compiled from their
 sources into oracle/_ref/ by ``make -C oracle ref``; the composites follow
 src/sw*/torch_cpu.cpp (which passes each primitive runs).  The fixtures hold
 inputs and expected outputs only.  Regenerate with
@@ -90,13 +90,14 @@ def linear_case(name, s, V, G, gap, T, lengths=None, full=True):
     return d
 
 
-def nw_affine_case(name, s, go, ge, T, lengths=None):
+def nw_affine_case(name, s, go, ge, T, lengths=None, family="nw"):
     B, L1, L2 = s.shape
     if lengths is None:
         lengths = np.tile(np.array([L1, L2], np.int32), (B, 1))
-    d = dict(kind="nw_affine", scores=s, lengths=lengths, gap_open=np.float32(go),
+    d = dict(kind=f"{family}_affine", scores=s, lengths=lengths, gap_open=np.float32(go),
              gap_ext=np.float32(ge), temp=np.float32(T))
-    v, m, go_, ge_, gt = O.nw_affine_forward(s, go, ge, T, lengths, impl="ref")
+    fwd = O.nw_affine_forward if family == "nw" else O.sv_affine_forward
+    v, m, go_, ge_, gt = fwd(s, go, ge, T, lengths, impl="ref")
     d.update(value=v, map=m, grad_open=go_, grad_ext=ge_, grad_temp=gt)
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **d)
     return d
@@ -169,6 +170,24 @@ def main():
                                                  ("gaps_pos", (2, 8, 10), 0.5, 0.2, 1.0)]):
         s = _draw(90 + i, shape, extra=0)[0]
         nw_affine_case(f"nw_affine_edge_{tag}", s, go, ge, T)
+    # Saigo-Vert affine: the same shapes and edge cases (values are LSE(0, {M})).
+    if not O.sv_ref_available():
+        raise SystemExit("build the reference kernels first: make -C oracle ref")
+    for i, (shape, go, ge, T) in enumerate([((4, 8, 10), -2.0, -0.5, 1.0),
+                                            ((4, 16, 16), -1.0, -0.3, 0.1),
+                                            ((4, 5, 20), -3.0, -1.0, 2.0)]):
+        s = _draw(100 + i, shape, extra=0)[0]
+        nw_affine_case(f"sv_affine_b{shape[0]}_{shape[1]}x{shape[2]}", s, go, ge, T, family="sv")
+    s = _draw(110, (2, 150, 97), extra=0)[0]
+    nw_affine_case("sv_affine_b2_150x97", s, -2.0, -0.5, 0.5, family="sv")
+    s = _draw(111, (5, 40, 36), extra=0)[0]
+    ln = np.array([[40, 36], [17, 29], [0, 6], [33, 0], [1, 1]], np.int32)
+    nw_affine_case("sv_affine_ragged_b5_40x36", s, -8.0, -0.5, 0.5, ln, family="sv")
+    for i, (tag, shape, go, ge, T) in enumerate([("1x1", (1, 1, 1), -2.0, -0.5, 1.0),
+                                                 ("3x1", (1, 3, 1), -2.0, -0.5, 1.0),
+                                                 ("gaps_pos", (2, 8, 10), 0.5, 0.2, 1.0)]):
+        s = _draw(120 + i, shape, extra=0)[0]
+        nw_affine_case(f"sv_affine_edge_{tag}", s, go, ge, T, family="sv")
     print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
 
 

@@ -103,6 +103,30 @@ def brute_nw_affine(s, go, ge, T):
     return _lse(totals, T)
 
 
+def brute_sv_affine(s, go, ge, T):
+    """Saigo-Vert: paths start in M at any cell with that cell's score and end
+    in M; from M(i,j) they go to M(i+1,j+1) (+s), I(i+1,j) (+go) or D(i,j+1)
+    (+go); from I to M (+s), I(i+1,j) (+ge) or D(i,j+1) (+go); from D to M
+    (+s) or D(i,j+1) (+ge).  The empty alignment scores 0."""
+    L1, L2 = s.shape
+    totals = [0.0]
+
+    def walk(i, j, st, acc):
+        if st == "M":
+            totals.append(acc)
+        if i + 1 < L1 and j + 1 < L2:
+            walk(i + 1, j + 1, "M", acc + s[i + 1, j + 1])
+        if st in "MI" and i + 1 < L1:
+            walk(i + 1, j, "I", acc + (go if st == "M" else ge))
+        if j + 1 < L2:
+            walk(i, j + 1, "D", acc + (ge if st == "D" else go))
+
+    for i in range(L1):
+        for j in range(L2):
+            walk(i, j, "M", s[i, j])
+    return _lse(totals, T)
+
+
 def brute_linear(s, g, T):
     """Linear gap: start at any cell with its score; moves diag (+s of the new
     cell), down (+g), right (+g); every prefix counts; empty alignment 0."""

@@ -50,6 +50,30 @@ def _temp_ok(got, ref, value, T):
     return bool(np.all(np.abs(got - ref) <= tol * np.maximum(1.0, np.abs(ref))))
 
 
+def _temp_close(got, ref, truth, value, T):
+    """Within the cancellation bar of the FP32 reference's dS/dT, or at most
+    twice as far from the FP64 result (truth(), computed when needed) as the
+    FP32 reference is itself (it reads 4e-4 off FP64 at T = 0.1 on 16 x 16
+    pairs)."""
+    if _temp_ok(got, ref, value, T):
+        return True
+    t = np.asarray(truth(), np.float64)
+    got = np.asarray(got, np.float64)
+    bar = np.maximum(1e-4 * np.maximum(1.0, np.abs(t)), 2.0 * np.abs(np.asarray(ref, np.float64) - t))
+    return bool(np.all(np.abs(got - t) <= bar))
+
+
+def _temp_vs_fp64(got, f32, f64, value, T):
+    """dS/dT against the FP64 oracle: within the cancellation bar, or the
+    batch's worst error at most twice the FP32 reference's worst error on the
+    same inputs (its seeds 2^(M - S) carry ulp(|S| / T) of rounding, so at
+    T = 0.1 it reads up to 5e-3 of max(|g|, 1) off FP64)."""
+    if _temp_ok(got, f64, value, T):
+        return True
+    got, f32, f64 = (np.asarray(x, np.float64) for x in (got, f32, f64))
+    return float(np.max(np.abs(got - f64))) <= 2.0 * float(np.max(np.abs(f32 - f64)))
+
+
 def _library(scores, lengths, go, ge, T, device="cpu", offsets=None, L1max=None, L2max=None):
     s = torch.as_tensor(scores, dtype=torch.float32).to(device).contiguous()
     ln = torch.as_tensor(lengths, dtype=torch.int32).to(device).contiguous()
@@ -152,7 +176,8 @@ def test_host_path_matches_golden(path):
     assert abs_err(mp, d["map"]) < 1e-4
     for got, key in ((gop, "grad_open"), (gex, "grad_ext")):
         assert _grad_err(got, d[key]) < 1e-4, key
-    assert _temp_ok(gT, d["grad_temp"], d["value"], T)
+    truth = lambda: O.nw_affine_forward(d["scores"], go, ge, T, d["lengths"], prec="f64")[4]  # noqa: E731
+    assert _temp_close(gT, d["grad_temp"], truth, d["value"], T)
 
 
 def test_public_functions_follow_the_oracle():
@@ -234,7 +259,8 @@ def test_gpu_matches_golden(path):
     assert abs_err(mp, d["map"]) < 1e-4
     for got, key in ((gop, "grad_open"), (gex, "grad_ext")):
         assert _grad_err(got, d[key]) < 1e-4, key
-    assert _temp_ok(gT, d["grad_temp"], d["value"], T)
+    truth = lambda: O.nw_affine_forward(d["scores"], go, ge, T, d["lengths"], prec="f64")[4]  # noqa: E731
+    assert _temp_close(gT, d["grad_temp"], truth, d["value"], T)
 
 
 @pytest.mark.gpu
@@ -254,7 +280,8 @@ def test_gpu_matches_fp64_oracle_multi_strip(B, L1, L2, go, ge, T):
     assert abs_err(mp, r[1]) < 1e-4
     for got, ref in zip((gop, gex), r[2:4]):
         assert _grad_err(got, ref) < 1e-4
-    assert _temp_ok(gT, r[4], r[0], T)
+    r32 = O.nw_affine_forward(s, go, ge, T, ln)
+    assert _temp_vs_fp64(gT, r32[4], r[4], r[0], T)
 
 
 @pytest.mark.gpu

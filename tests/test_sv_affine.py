@@ -1,0 +1,342 @@
+"""Saigo-Vert affine: oracle pins, host path, API, and the HIP kernels
+against the oracle.
+
+The oracle restatement (oracle/sw_oracle.c, sva_*) is pinned bit for bit
+against the reference's own CPU kernels (the golden fixtures were written by
+src/sv_affine/kernels_cpu.cpp built into oracle/_ref), against brute-force
+enumeration of every Saigo-Vert path, and against finite differences of its
+value.  The library (host and GPU) is then checked against the oracle:
+values to 1e-5 relative (of max(|S|, 1)), maps to
+1e-4 absolute, parameter gradients (sums over the whole path) to 1e-4
+relative of max(|g|, 1).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import abs_err, brute_sv_affine, golden_files, load
+from oracle import oracle as O
+
+import orihime_amd as ohm
+from orihime_amd import _lib
+
+SV = golden_files("sv_affine")
+IDS = [p.split("/")[-1] for p in SV]
+
+
+def _params(d):
+    return float(d["gap_open"]), float(d["gap_ext"]), float(d["temp"])
+
+
+def _grad_err(got, ref):
+    """Error relative to max(|ref|, 1): parameter gradients are expected
+    counts of opens / extends."""
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)))
+
+
+def _temp_ok(got, ref, value, T):
+    """dS/dT = (S - E[path score]) / T cancels two values of size |S|: its
+    FP32 noise grows with |S| / T (the FP32 reference carries the same: 1e-4
+    of max(|g|, 1) at T = 0.1 on 64 x 64 pairs), hence the bar
+    1e-4 + 8 (|S| + 1) 2^-23 / T relative to max(|g|, 1)."""
+    tol = 1e-4 + 8.0 * (np.abs(np.asarray(value, np.float64)) + 1.0) * 2.0 ** -23 / T
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return bool(np.all(np.abs(got - ref) <= tol * np.maximum(1.0, np.abs(ref))))
+
+
+def _temp_close(got, ref, truth, value, T):
+    """Within the cancellation bar of the FP32 reference's dS/dT, or at most
+    twice as far from the FP64 result (truth(), computed when needed) as the
+    FP32 reference is itself (it reads 4e-4 off FP64 at T = 0.1 on 16 x 16
+    pairs)."""
+    if _temp_ok(got, ref, value, T):
+        return True
+    t = np.asarray(truth(), np.float64)
+    got = np.asarray(got, np.float64)
+    bar = np.maximum(1e-4 * np.maximum(1.0, np.abs(t)), 2.0 * np.abs(np.asarray(ref, np.float64) - t))
+    return bool(np.all(np.abs(got - t) <= bar))
+
+
+def _temp_vs_fp64(got, f32, f64, value, T):
+    """dS/dT against the FP64 oracle: within the cancellation bar, or the
+    batch's worst error at most twice the FP32 reference's worst error on the
+    same inputs (its seeds 2^(M - S) carry ulp(|S| / T) of rounding, so at
+    T = 0.1 it reads up to 5e-3 of max(|g|, 1) off FP64)."""
+    if _temp_ok(got, f64, value, T):
+        return True
+    got, f32, f64 = (np.asarray(x, np.float64) for x in (got, f32, f64))
+    return float(np.max(np.abs(got - f64))) <= 2.0 * float(np.max(np.abs(f32 - f64)))
+
+
+def _library(scores, lengths, go, ge, T, device="cpu", offsets=None, L1max=None, L2max=None):
+    s = torch.as_tensor(scores, dtype=torch.float32).to(device).contiguous()
+    ln = torch.as_tensor(lengths, dtype=torch.int32).to(device).contiguous()
+    B = ln.shape[0]
+    value = torch.empty(B, dtype=torch.float32, device=device)
+    mp = torch.empty_like(s)
+    g = [torch.empty(B, dtype=torch.float32, device=device) for _ in range(3)]
+    off = None if offsets is None else torch.as_tensor(offsets, dtype=torch.int64).to(device)
+    _lib.sv_affine_forward(s, ln, off, go, ge, T, value=value, map=mp, grad_open=g[0],
+                           grad_ext=g[1], grad_temp=g[2], L1max=L1max, L2max=L2max)
+    return [t.cpu().numpy() for t in (value, mp, *g)]
+
+
+# --------------------------------------------------------------------------
+# Oracle pins
+# --------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("path", SV, ids=IDS)
+def test_oracle_matches_reference_golden_bitwise(path):
+    d = load(path)
+    go, ge, T = _params(d)
+    out = O.sv_affine_forward(d["scores"], go, ge, T, d["lengths"])
+    for got, key in zip(out, ("value", "map", "grad_open", "grad_ext", "grad_temp")):
+        assert np.array_equal(got, d[key]), key
+
+
+@pytest.mark.skipif(not O.sv_ref_available(), reason="reference kernels not built here")
+def test_oracle_matches_reference_build_on_fresh_inputs():
+    rng = np.random.default_rng(5)
+    for B, L1, L2, go, ge, T in [(3, 9, 13, -1.5, -0.4, 0.3), (2, 70, 65, -4.0, -1.0, 1.5)]:
+        s = (rng.standard_normal((B, L1, L2)) * 1.5).astype(np.float32)
+        ln = np.array([[L1, L2], [L1 - 5, L2 - 2], [1, L2]][:B], np.int32)
+        a = O.sv_affine_forward(s, go, ge, T, ln)
+        r = O.sv_affine_forward(s, go, ge, T, ln, impl="ref")
+        for x, y in zip(a, r):
+            assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("shape,go,ge,T", [((2, 2), -2.0, -0.5, 1.0), ((2, 3), -1.0, -0.3, 0.5),
+                                           ((3, 3), -3.0, -1.0, 2.0), ((1, 4), -2.0, -0.5, 1.0),
+                                           ((4, 1), -2.0, -0.5, 0.7), ((3, 4), 0.5, 0.2, 1.0)])
+def test_oracle_value_is_the_log_sum_over_all_paths(shape, go, ge, T):
+    rng = np.random.default_rng(sum(shape))
+    s = rng.standard_normal(shape)
+    v = O.sv_affine_forward(s[None], go, ge, T, prec="f64")[0][0]
+    assert abs(v - brute_sv_affine(s, go, ge, T)) < 1e-10
+
+
+def test_oracle_empty_sides_leave_the_empty_alignment():
+    s = np.zeros((3, 4, 5), np.float64)
+    ln = np.array([[0, 5], [4, 0], [0, 0]], np.int32)
+    v, mp, gop, gex, gT = O.sv_affine_forward(s, -2.0, -0.5, 1.0, ln, prec="f64")
+    assert np.allclose(v, 0) and np.allclose(gop, 0) and np.allclose(gex, 0)
+    assert np.allclose(gT, 0) and not mp.any()
+
+
+def test_oracle_differs_from_smith_waterman_by_the_i_to_d_edge():
+    """The I -> D edge and the M-only ends make the value differ from the
+    Smith-Waterman affine one on the same inputs (a plain SW restatement
+    would fail the enumeration check above)."""
+    rng = np.random.default_rng(4)
+    s = rng.standard_normal((1, 3, 4))
+    sv = O.sv_affine_forward(s, -1.0, -0.3, 1.0, prec="f64")[0][0]
+    sw = O.sw_affine_forward(s, -1.0, -0.3, 1.0, prec="f64")[0][0]
+    assert abs(sv - sw) > 1e-6
+
+
+def test_oracle_derivatives_match_finite_differences():
+    rng = np.random.default_rng(11)
+    s = rng.standard_normal((1, 6, 8))
+    go, ge, T = -1.5, -0.4, 0.8
+    v, mp, gop, gex, gT = O.sv_affine_forward(s, go, ge, T, prec="f64")
+    h = 1e-6
+
+    def val(ss=s, a=go, b=ge, t=T):
+        return O.sv_affine_forward(ss, a, b, t, prec="f64")[0][0]
+
+    fd = np.zeros_like(s)
+    for i in range(6):
+        for j in range(8):
+            sp, sm = s.copy(), s.copy()
+            sp[0, i, j] += h
+            sm[0, i, j] -= h
+            fd[0, i, j] = (val(sp) - val(sm)) / (2 * h)
+    assert abs_err(mp, fd) < 1e-7
+    assert abs(gop[0] - (val(a=go + h) - val(a=go - h)) / (2 * h)) < 1e-7
+    assert abs(gex[0] - (val(b=ge + h) - val(b=ge - h)) / (2 * h)) < 1e-7
+    assert abs(gT[0] - (val(t=T + h) - val(t=T - h)) / (2 * h)) < 1e-7
+
+
+# --------------------------------------------------------------------------
+# Library on the host (CPU tensors through the C ABI)
+# --------------------------------------------------------------------------
+
+
+def test_library_exports_the_sv_entry_points():
+    for name in ("ohm_sv_affine_workspace", "ohm_sv_affine_forward", "ohm_sv_affine_forward_cpu"):
+        assert hasattr(_lib._lib, name)
+    assert _lib._ws_sv_affine(1, 4, 100, 100, -1) > _lib._ws_sv_affine(0, 4, 100, 100, -1) > 0
+    assert _lib._ws_sv_affine(2, 4, 100, 100, -1) == 0
+
+
+@pytest.mark.parametrize("path", SV, ids=IDS)
+def test_host_path_matches_golden(path):
+    d = load(path)
+    go, ge, T = _params(d)
+    v, mp, gop, gex, gT = _library(d["scores"], d["lengths"], go, ge, T)
+    assert _grad_err(v, d["value"]) < 1e-5
+    assert abs_err(mp, d["map"]) < 1e-4
+    for got, key in ((gop, "grad_open"), (gex, "grad_ext")):
+        assert _grad_err(got, d[key]) < 1e-4, key
+    truth = lambda: O.sv_affine_forward(d["scores"], go, ge, T, d["lengths"], prec="f64")[4]  # noqa: E731
+    assert _temp_close(gT, d["grad_temp"], truth, d["value"], T)
+
+
+def test_public_functions_follow_the_oracle():
+    d = load([p for p in SV if "ragged" in p][0])
+    go, ge, T = _params(d)
+    x = torch.from_numpy(d["scores"])
+    ln = torch.from_numpy(d["lengths"])
+    kw = dict(gap_open_score=go, gap_extend_score=ge, temperature=T, lengths=ln)
+    assert abs_err(ohm.sv_affine(x, **kw).numpy(), d["map"]) < 1e-4
+    assert _grad_err(ohm.sv_affine_value(x, **kw).numpy(), d["value"]) < 1e-5
+    assert _temp_ok(ohm.sv_affine_entropy(x, **kw).numpy(), d["grad_temp"], d["value"], T)
+    out = ohm.ops.sv_affine.forward(x, **kw)
+    assert set(out) == {"map", "value", "entropy"}
+    assert abs_err(out["map"].numpy(), d["map"]) < 1e-4
+    assert _grad_err(out["value"].numpy(), d["value"]) < 1e-5
+
+
+def test_value_gradients_reach_scores_and_tensor_parameters():
+    d = load([p for p in SV if "b4_8x10" in p][0])
+    go, ge, T = _params(d)
+    x = torch.from_numpy(d["scores"]).clone().requires_grad_()
+    tgo = torch.tensor(go, requires_grad=True)
+    tge = torch.tensor(ge, requires_grad=True)
+    tT = torch.tensor(T, requires_grad=True)
+    v = ohm.sv_affine_value(x, gap_open_score=tgo, gap_extend_score=tge, temperature=tT)
+    (v * torch.arange(1.0, 5.0)).sum().backward()
+    w = np.arange(1.0, 5.0)
+    assert abs_err(x.grad.numpy(), d["map"] * w[:, None, None]) < 1e-4
+    assert abs(tgo.grad.item() - (w * d["grad_open"]).sum()) < 1e-3 * max(1, abs(tgo.grad.item()))
+    assert abs(tge.grad.item() - (w * d["grad_ext"]).sum()) < 1e-3 * max(1, abs(tge.grad.item()))
+    assert abs(tT.grad.item() - (w * d["grad_temp"]).sum()) < 1e-3 * max(1, abs(tT.grad.item()))
+
+
+def test_prefix_masks_equal_lengths():
+    d = load([p for p in SV if "ragged" in p][0])
+    go, ge, T = _params(d)
+    x = torch.from_numpy(d["scores"])
+    ln = torch.from_numpy(d["lengths"])
+    B, L1, L2 = x.shape
+    mask1 = torch.arange(L1)[None, :] < ln[:, 0:1]
+    mask2 = torch.arange(L2)[None, :] < ln[:, 1:2]
+    v = ohm.sv_affine_value(x, gap_open_score=go, gap_extend_score=ge, temperature=T, lengths=ln)
+    vm = ohm.sv._sv_affine_operator.value(x, go, ge, T, mask1=mask1, mask2=mask2)
+    assert torch.equal(v, vm)
+    vg = ohm.ops._kernels["sv_affine"].value_grad_params(x, go, ge, T, ln)
+    assert len(vg) == 3
+    assert _grad_err(vg[0].numpy(), d["grad_open"]) < 1e-4
+
+
+def test_invalid_lengths_raise():
+    x = torch.randn(1, 4, 5)
+    with pytest.raises((RuntimeError, ValueError)):
+        ohm.sv_affine(x, gap_open_score=-2.0, gap_extend_score=-0.5,
+                      lengths=torch.tensor([[4, 6]], dtype=torch.int32))
+    with pytest.raises((RuntimeError, ValueError)):
+        ohm.sv_affine(x, gap_open_score=-2.0, gap_extend_score=-0.5,
+                      lengths=torch.tensor([[-1, 5]], dtype=torch.int32))
+
+
+def test_second_order_derivatives_say_they_are_missing():
+    x = torch.randn(2, 4, 5, requires_grad=True)
+    mp = ohm.sv_affine(x, gap_open_score=-2.0, gap_extend_score=-0.5)
+    with pytest.raises(NotImplementedError):
+        mp.sum().backward()
+
+
+# --------------------------------------------------------------------------
+# GPU kernels against the oracle
+# --------------------------------------------------------------------------
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", SV, ids=IDS)
+def test_gpu_matches_golden(path):
+    d = load(path)
+    go, ge, T = _params(d)
+    print(f"sv gpu golden {path.split('/')[-1]}", flush=True)
+    v, mp, gop, gex, gT = _library(d["scores"], d["lengths"], go, ge, T, device="cuda")
+    assert _grad_err(v, d["value"]) < 1e-5
+    assert abs_err(mp, d["map"]) < 1e-4
+    for got, key in ((gop, "grad_open"), (gex, "grad_ext")):
+        assert _grad_err(got, d[key]) < 1e-4, key
+    truth = lambda: O.sv_affine_forward(d["scores"], go, ge, T, d["lengths"], prec="f64")[4]  # noqa: E731
+    assert _temp_close(gT, d["grad_temp"], truth, d["value"], T)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("B,L1,L2,go,ge,T", [(3, 300, 257, -2.0, -0.5, 0.5),
+                                             (2, 130, 400, -4.0, -1.0, 1.0),
+                                             (16, 64, 64, -1.0, -0.3, 0.1),
+                                             (4, 65, 3, -2.0, -0.5, 2.0)])
+def test_gpu_matches_fp64_oracle_multi_strip(B, L1, L2, go, ge, T):
+    rng = np.random.default_rng(B * 1000 + L1)
+    s = rng.standard_normal((B, L1, L2)).astype(np.float32)
+    ln = np.tile(np.array([L1, L2], np.int32), (B, 1))
+    ln[0] = [max(L1 - 37, 1), max(L2 - 11, 1)]
+    print(f"sv gpu vs oracle B={B} {L1}x{L2}", flush=True)
+    v, mp, gop, gex, gT = _library(s, ln, go, ge, T, device="cuda")
+    r = O.sv_affine_forward(s, go, ge, T, ln, prec="f64")
+    assert _grad_err(v, r[0]) < 1e-5
+    assert abs_err(mp, r[1]) < 1e-4
+    for got, ref in zip((gop, gex), r[2:4]):
+        assert _grad_err(got, ref) < 1e-4
+    r32 = O.sv_affine_forward(s, go, ge, T, ln)
+    assert _temp_vs_fp64(gT, r32[4], r[4], r[0], T)
+
+
+@pytest.mark.gpu
+def test_gpu_packed_layout_equals_padded():
+    rng = np.random.default_rng(3)
+    ln = np.array([[130, 70], [5, 200], [64, 64], [0, 9], [77, 1]], np.int32)
+    L1m, L2m = int(ln[:, 0].max()), int(ln[:, 1].max())
+    s = rng.standard_normal((len(ln), L1m, L2m)).astype(np.float32)
+    packed = np.concatenate([s[b, :ln[b, 0], :ln[b, 1]].ravel() for b in range(len(ln))])
+    sizes = ln[:, 0].astype(np.int64) * ln[:, 1]
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    print("sv gpu packed vs padded", flush=True)
+    a = _library(s, ln, -2.0, -0.5, 0.7, device="cuda")
+    b = _library(packed, ln, -2.0, -0.5, 0.7, device="cuda", offsets=offsets, L1max=L1m,
+                 L2max=L2m)
+    for x, y in zip((a[0], *a[2:]), (b[0], *b[2:])):
+        assert np.array_equal(x, y)
+    for k in range(len(ln)):
+        m = b[1][offsets[k]:offsets[k] + sizes[k]].reshape(ln[k, 0], ln[k, 1])
+        assert np.array_equal(m, a[1][k, :ln[k, 0], :ln[k, 1]])
+
+
+@pytest.mark.gpu
+def test_gpu_long_pairs_match_the_host_path():
+    rng = np.random.default_rng(9)
+    s = rng.standard_normal((2, 1100, 1000)).astype(np.float32)
+    ln = np.array([[1100, 1000], [700, 990]], np.int32)
+    print("sv gpu vs host 2x1100x1000", flush=True)
+    g = _library(s, ln, -2.0, -0.5, 0.5, device="cuda")
+    h = _library(s, ln, -2.0, -0.5, 0.5)
+    assert _grad_err(g[0], h[0]) < 1e-5
+    assert abs_err(g[1], h[1]) < 1e-4
+    for x, y in zip(g[2:4], h[2:4]):
+        assert _grad_err(x, y) < 1e-4
+    assert _temp_ok(g[4], h[4], h[0], 0.5)
+
+
+@pytest.mark.gpu
+def test_gpu_public_functions_run_on_the_hip_kernels():
+    d = load([p for p in SV if "ragged" in p][0])
+    go, ge, T = _params(d)
+    x = torch.from_numpy(d["scores"]).cuda().requires_grad_()
+    ln = torch.from_numpy(d["lengths"]).cuda()
+    v = ohm.sv_affine_value(x, gap_open_score=go, gap_extend_score=ge, temperature=T, lengths=ln)
+    v.sum().backward()
+    assert abs_err(x.grad.cpu().numpy(), d["map"]) < 1e-4
+    assert _grad_err(v.detach().cpu().numpy(), d["value"]) < 1e-5
