@@ -216,6 +216,35 @@ ohm_status ohm_sv_affine_forward(const float* scores, const int32_t* lengths,
                                  void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Linear-gap Needleman-Wunsch and Saigo-Vert: the affine models with one   */
+/* gap score in both roles (each move sequence has exactly one state        */
+/* sequence, so they are the same function of the inputs).                 */
+/* ------------------------------------------------------------------------ */
+
+size_t ohm_nw_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
+                        int64_t total_cells);
+size_t ohm_sv_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
+                        int64_t total_cells);
+
+/* Value, map, dS/dgap and dS/dtemperature, arguments as ohm_sw_forward.
+ * Replaces: nw_forward, nw_value_grad_params, soft_nw_with_grads
+ * (src/nw/registry.cpp:35-49) and nw_forward_cpu / nw_backward_cpu
+ * (src/nw/kernels_cpu.cpp:78-272). */
+ohm_status ohm_nw_forward(const float* scores, const int32_t* lengths, const int64_t* offsets,
+                          int32_t B, int32_t L1max, int32_t L2max, float gap, float temp,
+                          float* value, float* map, float* grad_gap, float* grad_temp,
+                          void* workspace, size_t workspace_bytes, void* stream);
+
+/* As ohm_nw_forward for Saigo-Vert.  Replaces: sv_linear_forward,
+ * sv_linear_value_grad_params, soft_sv_linear_with_grads
+ * (src/sv_linear/registry.cpp) and sv_linear_forward_cpu /
+ * sv_linear_backward_cpu (src/sv_linear/kernels_cpu.cpp:94-365). */
+ohm_status ohm_sv_forward(const float* scores, const int32_t* lengths, const int64_t* offsets,
+                          int32_t B, int32_t L1max, int32_t L2max, float gap, float temp,
+                          float* value, float* map, float* grad_gap, float* grad_temp,
+                          void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Host (CPU) twins: same arguments on host memory, no workspace or stream. */
 /* `threads` <= 0 uses every hardware thread.  Replace the reference CPU     */
 /* dispatch (src/sw_affine/torch_cpu.cpp:593-707, src/sw/torch_cpu.cpp:522-591) */
@@ -251,6 +280,16 @@ ohm_status ohm_sv_affine_forward_cpu(const float* scores, const int32_t* lengths
                                      int32_t L2max, float gap_open, float gap_ext, float temp,
                                      float* value, float* map, float* grad_open,
                                      float* grad_ext, float* grad_temp, int32_t threads);
+
+ohm_status ohm_nw_forward_cpu(const float* scores, const int32_t* lengths,
+                              const int64_t* offsets, int32_t B, int32_t L1max, int32_t L2max,
+                              float gap, float temp, float* value, float* map, float* grad_gap,
+                              float* grad_temp, int32_t threads);
+
+ohm_status ohm_sv_forward_cpu(const float* scores, const int32_t* lengths,
+                              const int64_t* offsets, int32_t B, int32_t L1max, int32_t L2max,
+                              float gap, float temp, float* value, float* map, float* grad_gap,
+                              float* grad_temp, int32_t threads);
 
 ohm_status ohm_sw_forward_cpu(const float* scores, const int32_t* lengths,
                               const int64_t* offsets, int32_t B, int32_t L1max, int32_t L2max,

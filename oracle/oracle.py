@@ -20,6 +20,7 @@ runs and how it contracts them:
                             (global alignment: forward + backward)
 * ``sv_affine_forward``  -> src/sv_affine/torch_cpu.cpp ``soft_sv_affine_cpu_with_grads``
                             (Saigo-Vert: forward + backward)
+* ``nw_forward``, ``sv_forward`` -> the linear-gap twins (src/nw, src/sv_linear)
 
 All arrays are numpy; ``lengths`` defaults to the full shape.
 """
@@ -71,6 +72,11 @@ def nw_ref_available() -> bool:
 
 def sv_ref_available() -> bool:
     return os.path.exists(os.path.join(_HERE, "_ref", "libsv_affine_ref.so"))
+
+
+def linear_ref_available() -> bool:
+    return all(os.path.exists(os.path.join(_HERE, "_ref", f"lib{n}_ref.so"))
+               for n in ("nw", "sv_linear"))
 
 
 def _ref_lib(name: str):
@@ -347,3 +353,45 @@ def sv_affine_forward(scores, gap_open, gap_ext, temp, lengths=None, impl="oracl
     """(value[B], map[B,L1,L2], grad_open[B], grad_ext[B], grad_temp[B]) of
     the Saigo-Vert affine alignment."""
     return _three_state_forward("sv", scores, gap_open, gap_ext, temp, lengths, impl, prec)
+
+
+def _linear_forward(family, scores, gap, temp, lengths, impl, prec):
+    dtype = np.float32 if prec == "f32" else np.float64
+    C = ctypes.c_float if prec == "f32" else ctypes.c_double
+    states = 1 if family == "nw" else 3
+    if impl == "ref":
+        if prec != "f32":
+            raise ValueError("the reference kernels are FP32 only")
+        name = "nw" if family == "nw" else "sv_linear"
+        lib = _ref_lib(name)
+        fwd = getattr(lib, f"{name}_forward_cpu")
+        bwd = getattr(lib, f"{name}_backward_cpu")
+    else:
+        lib = _lib()
+        fwd = getattr(lib, f"oracle_{prec}_{family}l_forward")
+        bwd = getattr(lib, f"oracle_{prec}_{family}l_backward")
+    s = _arr(scores, dtype)
+    ln = _lens(s, lengths)
+    B, L1, L2 = s.shape
+    asz = states * (L1 + 1) * (L2 + 1)
+    alpha = np.zeros((B, asz), dtype)
+    part = np.zeros(B, dtype)
+    fwd(_ptr(s), _ptr(alpha), _ptr(part), _ptr(ln), B, L1, L2, C(gap), C(temp))
+    beta = np.zeros((B, asz), dtype)
+    post = np.zeros((B, L1, L2), dtype)
+    g = [np.zeros(B, dtype) for _ in range(2)]
+    bwd(_ptr(alpha), _ptr(s), _ptr(part), _ptr(beta), _ptr(post), _ptr(g[0]), _ptr(g[1]),
+        _ptr(ln), B, L1, L2, C(gap), C(temp))
+    return part, post, g[0], g[1]
+
+
+def nw_forward(scores, gap, temp, lengths=None, impl="oracle", prec="f32"):
+    """(value[B], map[B,L1,L2], grad_gap[B], grad_temp[B]) of the linear-gap
+    global (Needleman-Wunsch) alignment."""
+    return _linear_forward("nw", scores, gap, temp, lengths, impl, prec)
+
+
+def sv_forward(scores, gap, temp, lengths=None, impl="oracle", prec="f32"):
+    """(value[B], map[B,L1,L2], grad_gap[B], grad_temp[B]) of the linear-gap
+    Saigo-Vert alignment."""
+    return _linear_forward("sv", scores, gap, temp, lengths, impl, prec)

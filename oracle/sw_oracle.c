@@ -1003,10 +1003,14 @@ void OFN(sva_forward)(const real* scores, real* alpha, real* part, const int* le
 
 /* sv_affine_backward_cpu, src/sv_affine/kernels_cpu.cpp:203-356: terminal
  * weights on the M states only, push-form adjoint, I -> D counted as an
- * open. */
-void OFN(sva_backward)(const real* alpha, const real* scores, const real* part, real* beta,
-                       real* post, real* gopen, real* gext, real* gT, const int* len,
-                       int B, int L1m, int L2m, real go, real ge, real T) {
+ * open.  With `linear` (sv_linear_backward_cpu,
+ * src/sv_linear/kernels_cpu.cpp:203-365: go = ge = gap) every gap edge goes
+ * into one Kahan-summed count, written to gopen, and the expected total is a
+ * Kahan sum of the match and gap terms. */
+static void sva_backward_core(const real* alpha, const real* scores, const real* part,
+                              real* beta, real* post, real* gopen, real* gext, real* gT,
+                              const int* len, int B, int L1m, int L2m, real go, real ge, real T,
+                              int linear) {
     geom g = mk_geom(L1m, L2m);
     for (int b = 0; b < B; ++b) {
         const real* a = alpha + (size_t)b * 3 * g.cs;
@@ -1020,6 +1024,9 @@ void OFN(sva_backward)(const real* alpha, const real* scores, const real* part, 
         for (int i = 1; i <= L1; ++i)
             for (int j = 1; j <= L2; ++j) be[CELL(g, i, j)] = post_w(a[CELL(g, i, j)], S, T);
         real so = 0, se = 0;
+        kahan gl = {0, 0};  /* the linear gap count */
+#define SVA_OPEN(v) do { if (linear) kadd(&gl, (v)); else so += (v); } while (0)
+#define SVA_EXT(v) do { if (linear) kadd(&gl, (v)); else se += (v); } while (0)
         for (int k = L1 + L2; k >= 2; --k) {
             int lo = k - L2 > 1 ? k - L2 : 1, hi = k - 1 < L1 ? k - 1 : L1;
             for (int i = lo; i <= hi; ++i) {
@@ -1042,26 +1049,132 @@ void OFN(sva_backward)(const real* alpha, const real* scores, const real* part, 
                 if (!is_bad(bi)) {
                     wsoft(ci, 2, T, w);
                     if (i > 1) {
-                        if (w[0] > 0) { be[0 * g.cs + up] += bi * w[0]; so += bi * w[0]; }
-                        if (w[1] > 0) { be[1 * g.cs + up] += bi * w[1]; se += bi * w[1]; }
+                        if (w[0] > 0) { be[0 * g.cs + up] += bi * w[0]; SVA_OPEN(bi * w[0]); }
+                        if (w[1] > 0) { be[1 * g.cs + up] += bi * w[1]; SVA_EXT(bi * w[1]); }
                     }
                 }
                 real bd = be[2 * g.cs + c];
                 if (!is_bad(bd)) {
                     wsoft(cd, 3, T, w);
                     if (j > 1) {
-                        if (w[0] > 0) { be[0 * g.cs + lf] += bd * w[0]; so += bd * w[0]; }
-                        if (w[1] > 0) { be[1 * g.cs + lf] += bd * w[1]; so += bd * w[1]; }
-                        if (w[2] > 0) { be[2 * g.cs + lf] += bd * w[2]; se += bd * w[2]; }
+                        if (w[0] > 0) { be[0 * g.cs + lf] += bd * w[0]; SVA_OPEN(bd * w[0]); }
+                        if (w[1] > 0) { be[1 * g.cs + lf] += bd * w[1]; SVA_OPEN(bd * w[1]); }
+                        if (w[2] > 0) { be[2 * g.cs + lf] += bd * w[2]; SVA_EXT(bd * w[2]); }
                     }
                 }
             }
         }
+#undef SVA_OPEN
+#undef SVA_EXT
         kahan ms = {0, 0};
         for (size_t x = 0; x < g.ss; ++x) kadd(&ms, P[x] * s[x]);
-        real gap_cost = so * go + se * ge;  /* kernels_cpu.cpp:348-349 */
-        gopen[b] = so;
-        gext[b] = se;
-        gT[b] = (S - (ms.s + gap_cost)) / T;
+        if (linear) {  /* sv_linear kernels_cpu.cpp:353-360 */
+            real cost = gl.s * go;
+            kahan tot = {0, 0};
+            kadd(&tot, ms.s);
+            kadd(&tot, cost);
+            gopen[b] = gl.s;
+            gT[b] = (S - tot.s) / T;
+        } else {
+            real gap_cost = so * go + se * ge;  /* sv_affine kernels_cpu.cpp:348-349 */
+            gopen[b] = so;
+            gext[b] = se;
+            gT[b] = (S - (ms.s + gap_cost)) / T;
+        }
+    }
+}
+
+void OFN(sva_backward)(const real* alpha, const real* scores, const real* part, real* beta,
+                       real* post, real* gopen, real* gext, real* gT, const int* len,
+                       int B, int L1m, int L2m, real go, real ge, real T) {
+    sva_backward_core(alpha, scores, part, beta, post, gopen, gext, gT, len, B, L1m, L2m, go, ge,
+                      T, 0);
+}
+
+/* sv_linear_forward_cpu, src/sv_linear/kernels_cpu.cpp:94-196: the affine
+ * recurrence with one gap score in both roles (the same operations). */
+void OFN(svl_forward)(const real* scores, real* alpha, real* part, const int* len,
+                      int B, int L1m, int L2m, real gap, real T) {
+    OFN(sva_forward)(scores, alpha, part, len, B, L1m, L2m, gap, gap, T);
+}
+
+void OFN(svl_backward)(const real* alpha, const real* scores, const real* part, real* beta,
+                       real* post, real* ggap, real* gT, const int* len,
+                       int B, int L1m, int L2m, real gap, real T) {
+    sva_backward_core(alpha, scores, part, beta, post, ggap, NULL, gT, len, B, L1m, L2m, gap,
+                      gap, T, 1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Needleman-Wunsch, linear gap: one state, A(i,0) = i gap, A(0,j) = j gap, */
+/* value A(L1, L2).                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* nw_forward_cpu, src/nw/kernels_cpu.cpp:78-156 */
+void OFN(nwl_forward)(const real* scores, real* alpha, real* part, const int* len,
+                      int B, int L1m, int L2m, real gap, real T) {
+    geom g = mk_geom(L1m, L2m);
+    for (int b = 0; b < B; ++b) {
+        const real* s = scores + (size_t)b * g.ss;
+        real* a = alpha + (size_t)b * g.cs;
+        int L1 = len[2 * b], L2 = len[2 * b + 1];
+        for (size_t t = 0; t < g.cs; ++t) a[t] = NINF_;
+        a[0] = 0;
+        for (int i = 1; i <= L1; ++i) a[CELL(g, i, 0)] = (real)i * gap;
+        for (int j = 1; j <= L2; ++j) a[CELL(g, 0, j)] = (real)j * gap;
+        for (int k = 2; k <= L1 + L2; ++k) {
+            int lo = k - L2 > 1 ? k - L2 : 1, hi = k - 1 < L1 ? k - 1 : L1;
+            for (int i = lo; i <= hi; ++i) {
+                int j = k - i;
+                real v[3] = {a[CELL(g, i - 1, j - 1)] + s[SIDX(g, i, j)],
+                             a[CELL(g, i - 1, j)] + gap, a[CELL(g, i, j - 1)] + gap};
+                a[CELL(g, i, j)] = lse_n(v, 3, T);
+            }
+        }
+        part[b] = a[CELL(g, L1, L2)];
+    }
+}
+
+/* nw_backward_cpu, src/nw/kernels_cpu.cpp:162-272: unit terminal adjoint,
+ * map = adjoint x diagonal weight, Kahan-summed gap count including the
+ * boundary cells (i gap, j gap). */
+void OFN(nwl_backward)(const real* alpha, const real* scores, const real* part, real* beta,
+                       real* post, real* ggap, real* gT, const int* len,
+                       int B, int L1m, int L2m, real gap, real T) {
+    geom g = mk_geom(L1m, L2m);
+    for (int b = 0; b < B; ++b) {
+        const real* a = alpha + (size_t)b * g.cs;
+        const real* s = scores + (size_t)b * g.ss;
+        real* be = beta + (size_t)b * g.cs;
+        real* P = post + (size_t)b * g.ss;
+        int L1 = len[2 * b], L2 = len[2 * b + 1];
+        memset(P, 0, g.ss * sizeof(real));
+        memset(be, 0, g.cs * sizeof(real));
+        be[CELL(g, L1, L2)] = 1;
+        kahan gg = {0, 0}, ms = {0, 0};
+        for (int k = L1 + L2; k >= 2; --k) {
+            int lo = k - L2 > 1 ? k - L2 : 1, hi = k - 1 < L1 ? k - 1 : L1;
+            for (int i = lo; i <= hi; ++i) {
+                int j = k - i;
+                size_t c = CELL(g, i, j), dg = CELL(g, i - 1, j - 1);
+                size_t up = CELL(g, i - 1, j), lf = CELL(g, i, j - 1);
+                real bij = be[c];
+                if (bij == 0) continue;
+                real sc = s[SIDX(g, i, j)];
+                real v[3] = {a[dg] + sc, a[up] + gap, a[lf] + gap}, w[3];
+                wsoft(v, 3, T, w);
+                real pij = bij * w[0];
+                P[SIDX(g, i, j)] = pij;
+                kadd(&ms, pij * sc);
+                kadd(&gg, bij * (w[1] + w[2]));
+                if (w[0] > 0) be[dg] += bij * w[0];
+                if (w[1] > 0) be[up] += bij * w[1];
+                if (w[2] > 0) be[lf] += bij * w[2];
+            }
+        }
+        for (int i = 1; i <= L1; ++i) kadd(&gg, be[CELL(g, i, 0)] * (real)i);
+        for (int j = 1; j <= L2; ++j) kadd(&gg, be[CELL(g, 0, j)] * (real)j);
+        ggap[b] = gg.s;
+        gT[b] = (part[b] - (ms.s + gg.s * gap)) / T;
     }
 }

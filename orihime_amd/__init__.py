@@ -2,9 +2,9 @@
 
 Drop-in for the soft Smith-Waterman path of orihime: the same public
 functions (``sw``, ``sw_value``, ``sw_entropy``, ``sw_affine``,
-``sw_affine_value``, ``sw_affine_entropy``; the global-alignment
-``nw_affine``, ``nw_affine_value``, ``nw_affine_entropy``; the Saigo-Vert
-``sv_affine``, ``sv_affine_value``, ``sv_affine_entropy``), ``nn`` modules, ``ops``
+``sw_affine_value``, ``sw_affine_entropy``; the global-alignment ``nw`` and
+``nw_affine`` families; the Saigo-Vert ``sv`` and ``sv_affine`` families,
+each with ``_value`` and ``_entropy``), ``nn`` modules, ``ops``
 named operations, ``raw`` low-level tier and ``torch.ops.orihime.*``
 primitives, computed by hand-written gfx950 HIP kernels (GPU tensors) or
 the library's host implementation (CPU tensors), both behind the C ABI in
@@ -15,8 +15,8 @@ __version__ = "0.1.0"
 
 from . import _ops as _ops  # registers torch.ops.orihime.* (loads libohm_sw.so)
 from .sw import sw, sw_affine, sw_affine_entropy, sw_affine_value, sw_entropy, sw_value
-from .nw import nw_affine, nw_affine_entropy, nw_affine_value
-from .sv import sv_affine, sv_affine_entropy, sv_affine_value
+from .nw import nw, nw_affine, nw_affine_entropy, nw_affine_value, nw_entropy, nw_value
+from .sv import sv, sv_affine, sv_affine_entropy, sv_affine_value, sv_entropy, sv_value
 from . import nn as nn
 from . import raw as raw
 from . import ops as ops
@@ -33,9 +33,15 @@ __all__ = [
     "sw_affine",
     "sw_affine_value",
     "sw_affine_entropy",
+    "nw",
+    "nw_value",
+    "nw_entropy",
     "nw_affine",
     "nw_affine_value",
     "nw_affine_entropy",
+    "sv",
+    "sv_value",
+    "sv_entropy",
     "sv_affine",
     "sv_affine_value",
     "sv_affine_entropy",

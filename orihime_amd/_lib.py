@@ -53,6 +53,12 @@ EXPORTS = (
     "ohm_sv_affine_workspace",
     "ohm_sv_affine_forward",
     "ohm_sv_affine_forward_cpu",
+    "ohm_nw_workspace",
+    "ohm_sv_workspace",
+    "ohm_nw_forward",
+    "ohm_sv_forward",
+    "ohm_nw_forward_cpu",
+    "ohm_sv_forward_cpu",
 )
 
 
@@ -71,6 +77,8 @@ _ws_affine = _sig("ohm_sw_affine_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 _ws_linear = _sig("ohm_sw_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 _ws_nw_affine = _sig("ohm_nw_affine_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 _ws_sv_affine = _sig("ohm_sv_affine_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
+_ws_nw = _sig("ohm_nw_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
+_ws_sv = _sig("ohm_sv_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 
 _PROBLEM = (_vp, _vp, _vp, _i32, _i32, _i32)  # scores, lengths, offsets, B, L1max, L2max
 _aff_fwd = _sig("ohm_sw_affine_forward", _i32, *_PROBLEM, _f, _f, _f,
@@ -97,6 +105,10 @@ _sva_fwd = _sig("ohm_sv_affine_forward", _i32, *_PROBLEM, _f, _f, _f,
                 _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp)
 _sva_fwd_cpu = _sig("ohm_sv_affine_forward_cpu", _i32, *_PROBLEM, _f, _f, _f,
                     _vp, _vp, _vp, _vp, _vp, _i32)
+_nwl_fwd = _sig("ohm_nw_forward", _i32, *_PROBLEM, _f, _f, _vp, _vp, _vp, _vp, _vp, _sz, _vp)
+_svl_fwd = _sig("ohm_sv_forward", _i32, *_PROBLEM, _f, _f, _vp, _vp, _vp, _vp, _vp, _sz, _vp)
+_nwl_fwd_cpu = _sig("ohm_nw_forward_cpu", _i32, *_PROBLEM, _f, _f, _vp, _vp, _vp, _vp, _i32)
+_svl_fwd_cpu = _sig("ohm_sv_forward_cpu", _i32, *_PROBLEM, _f, _f, _vp, _vp, _vp, _vp, _i32)
 
 OP_VALUE, OP_FORWARD, OP_TANGENT = 0, 1, 2
 
@@ -283,3 +295,31 @@ def sv_affine_forward(scores, lengths, offsets, gap_open, gap_ext, temp, *, valu
         _check(_sva_fwd(*args, ws.data_ptr(), ws.numel(), stream), "ohm_sv_affine_forward")
     else:
         _check(_sva_fwd_cpu(*args, _cpu_threads()), "ohm_sv_affine_forward_cpu")
+
+
+def _linear_global_local(family):
+    gpu, cpu, ws_fn = {"nw": (_nwl_fwd, _nwl_fwd_cpu, _ws_nw),
+                       "sv": (_svl_fwd, _svl_fwd_cpu, _ws_sv)}[family]
+
+    def run(scores, lengths, offsets, gap, temp, *, value, map=None, grad_gap=None,
+            grad_temp=None, L1max=None, L2max=None, workspace=None):
+        B, L1, L2 = _dims(scores, lengths, offsets, L1max, L2max)
+        args = (_ptr(scores), _ptr(lengths), _ptr(offsets), B, L1, L2, float(gap), float(temp),
+                _ptr(value), _ptr(map), _ptr(grad_gap), _ptr(grad_temp))
+        if scores.is_cuda:
+            op = OP_FORWARD if map is not None else OP_VALUE
+            total = -1 if offsets is None else scores.numel()
+            need = int(ws_fn(op, B, L1, L2, total))
+            stream = torch.cuda.current_stream(scores.device).cuda_stream
+            ws = workspace if workspace is not None else _workspace(scores.device, need, stream)
+            _check(gpu(*args, ws.data_ptr(), ws.numel(), stream), f"ohm_{family}_forward")
+        else:
+            _check(cpu(*args, _cpu_threads()), f"ohm_{family}_forward_cpu")
+
+    run.__doc__ = (f"Linear-gap {'Needleman-Wunsch' if family == 'nw' else 'Saigo-Vert'}: "
+                   "value, and map + dS/dgap, dS/dT when ``map`` is given.")
+    return run
+
+
+nw_forward = _linear_global_local("nw")
+sv_forward = _linear_global_local("sv")

@@ -382,6 +382,71 @@ def _three_state_schemas(family: str, call, value_first: bool) -> dict:
     }
 
 
+def _linear_family_schemas(ns: str, soft: str, call, model: str) -> dict:
+    """First-order schemas of a linear-gap family (src/nw/registry.cpp:35-49,
+    src/sv_linear/registry.cpp): the affine sweeps with one gap score in both
+    roles, dS/dgap = dS/dgap_open + dS/dgap_ext (summed in the kernel)."""
+
+    def run(scores, gap, temp, lengths, want_map=True, want_grads=False):
+        _check_scores(scores)
+        ln = _check_lengths(lengths, scores)
+        B = scores.shape[0]
+        value = _vec(scores, B)
+        mp = torch.empty_like(scores) if (want_map or want_grads) else None
+        g = [_vec(scores, B) for _ in range(2)] if want_grads else [None] * 2
+        call(scores, ln, None, gap, temp, value=value, map=mp, grad_gap=g[0], grad_temp=g[1])
+        return value, mp, g
+
+    def forward(scores, gap: float, temp: float, lengths=None):
+        value, mp, _ = run(scores, gap, temp, lengths)
+        return [value, mp]
+
+    def forward_t(scores, gap, temp, lengths):
+        value, mp, g = run(scores, _scalar(gap, "gap"), _scalar(temp, "temperature"), lengths,
+                           want_grads=True)
+        setattr(value, _PARAM_GRADS, tuple(g))
+        return [value, mp]
+
+    def value_grad_params(scores, gap: float, temp: float, lengths=None):
+        _, _, g = run(scores, gap, temp, lengths, want_map=False, want_grads=True)
+        return g[0], g[1]
+
+    def with_grads(scores, gap: float, temperature: float, lengths=None):
+        value, mp, g = run(scores, gap, temperature, lengths, want_grads=True)
+        return value, mp, g[0], g[1]
+
+    def value_only(scores, gap: float, temp: float, lengths=None):
+        value, _, _ = run(scores, gap, temp, lengths, want_map=False)
+        return value
+
+    def second_order(what: str):
+        def impl(scores, *args):
+            raise NotImplementedError(
+                f"{model} {what} (second-order derivatives of the map) has no kernel in "
+                "orihime_amd yet; the value, map and parameter gradients are available")
+
+        return impl
+
+    S = "(Tensor scores, float gap, float temp, Tensor? lengths)"
+    L = "(Tensor scores, float gap, float temperature, Tensor? lengths)"
+    return {
+        f"{ns}_forward": (f"{S} -> Tensor[]", forward),
+        f"{ns}_forward_t": ("(Tensor scores, Tensor gap, Tensor temp, Tensor lengths) -> Tensor[]", forward_t),
+        f"{ns}_value_grad_params": (f"{S} -> (Tensor, Tensor)", value_grad_params),
+        f"{ns}_marginals_backward": ("(Tensor scores, Tensor grad_marginals, float gap, float temp, Tensor? lengths) -> (Tensor, Tensor, Tensor)", second_order("marginals_backward")),
+        f"{ns}_marginals_hvp": ("(Tensor scores, Tensor v, float gap, float temp, Tensor? lengths) -> Tensor", second_order("marginals_hvp")),
+        f"{ns}_marginals_grad_gap": (f"{S} -> Tensor", second_order("marginals_grad_gap")),
+        f"{ns}_marginals_grad_temp": (f"{S} -> Tensor", second_order("marginals_grad_temp")),
+        f"{ns}_value_only": (f"{S} -> Tensor", value_only),
+        soft: ("(Tensor scores, Tensor gap, Tensor temperature, Tensor lengths) -> Tensor[]", forward_t),
+        f"{soft}_float": (f"{L} -> Tensor[]", forward),
+        f"{soft}_with_grads": (f"{L} -> (Tensor, Tensor, Tensor, Tensor)", with_grads),
+        f"{soft}_hvp": ("(Tensor scores, Tensor tangent, float gap, float temperature, Tensor? lengths) -> Tensor", second_order("hvp")),
+        f"{soft}_param_jacobian": ("(Tensor scores, int param_type, float gap, float temperature, Tensor? lengths) -> Tensor", second_order("param_jacobian")),
+        f"{soft}_backward_full": ("(Tensor scores, Tensor grad_alignment, float gap, float temperature, Tensor? lengths) -> (Tensor, Tensor, Tensor)", second_order("backward_full")),
+    }
+
+
 # --------------------------------------------------------------------------
 # Registration
 # --------------------------------------------------------------------------
@@ -427,6 +492,8 @@ _SCHEMAS = {
 
 _SCHEMAS.update(_three_state_schemas("nw", _lib.nw_affine_forward, True))
 _SCHEMAS.update(_three_state_schemas("sv", _lib.sv_affine_forward, False))
+_SCHEMAS.update(_linear_family_schemas("nw", "soft_nw", _lib.nw_forward, "nw"))
+_SCHEMAS.update(_linear_family_schemas("sv_linear", "soft_sv_linear", _lib.sv_forward, "sv"))
 
 
 def _fake_for(name: str, schema: str):
@@ -446,7 +513,7 @@ def _fake_for(name: str, schema: str):
         if ret == "Tensor[]":
             return [vec(scores), like(scores)]
         if name.endswith("value_grad_params"):
-            n = 4 if name.startswith("nw_") else (3 if "affine" in name else 2)
+            n = 4 if name.startswith("nw_affine") else (3 if "affine" in name else 2)
             return tuple(vec(scores) for _ in range(n))
         if name.endswith("_with_grads"):
             n = 3 if "affine" in name else 2
@@ -535,6 +602,13 @@ for _name, _aff, _tp in (
         f"orihime::{_name}", _make_forward_backward(_aff, _tp), setup_context=_forward_setup,
         lib=_LIB,
     )
+for _ns, _soft in (("nw", "soft_nw"), ("sv_linear", "soft_sv_linear")):
+    for _name, _tp in ((f"{_ns}_forward", False), (f"{_ns}_forward_t", True),
+                       (f"{_soft}_float", False), (_soft, True)):
+        torch.library.register_autograd(
+            f"orihime::{_name}", _make_forward_backward(False, _tp, _ns),
+            setup_context=_forward_setup, lib=_LIB,
+        )
 for _fam in ("nw", "sv"):
     for _name, _tp in ((f"{_fam}_affine_forward", False), (f"{_fam}_affine_forward_t", True),
                        (f"soft_{_fam}_affine_float", False), (f"soft_{_fam}_affine", True)):
@@ -635,9 +709,9 @@ sw_affine_marginals_grad_gap_ext = orihime.sw_affine_marginals_grad_gap_ext
 sw_affine_marginals_grad_temp = orihime.sw_affine_marginals_grad_temp
 sw_affine_value_only = orihime.sw_affine_value_only
 
-for _fam in ("nw", "sv"):
-    for _op in (f"{_fam}_affine_forward", f"{_fam}_affine_forward_t",
-                f"{_fam}_affine_value_grad_params", f"{_fam}_affine_value_only"):
+for _fam in ("nw_affine", "sv_affine", "nw", "sv_linear"):
+    for _op in (f"{_fam}_forward", f"{_fam}_forward_t", f"{_fam}_value_grad_params",
+                f"{_fam}_value_only"):
         torch.library.register_vmap(f"orihime::{_op}", _make_batch_folding_vmap_rule(_op), lib=_LIB)
 
 nw_affine_forward = orihime.nw_affine_forward
@@ -659,6 +733,24 @@ sv_affine_marginals_grad_gap_open = orihime.sv_affine_marginals_grad_gap_open
 sv_affine_marginals_grad_gap_ext = orihime.sv_affine_marginals_grad_gap_ext
 sv_affine_marginals_grad_temp = orihime.sv_affine_marginals_grad_temp
 sv_affine_value_only = orihime.sv_affine_value_only
+
+nw_forward = orihime.nw_forward
+nw_forward_t = orihime.nw_forward_t
+nw_value_grad_params = orihime.nw_value_grad_params
+nw_marginals_backward = orihime.nw_marginals_backward
+nw_marginals_hvp = orihime.nw_marginals_hvp
+nw_marginals_grad_gap = orihime.nw_marginals_grad_gap
+nw_marginals_grad_temp = orihime.nw_marginals_grad_temp
+nw_value_only = orihime.nw_value_only
+
+sv_linear_forward = orihime.sv_linear_forward
+sv_linear_forward_t = orihime.sv_linear_forward_t
+sv_linear_value_grad_params = orihime.sv_linear_value_grad_params
+sv_linear_marginals_backward = orihime.sv_linear_marginals_backward
+sv_linear_marginals_hvp = orihime.sv_linear_marginals_hvp
+sv_linear_marginals_grad_gap = orihime.sv_linear_marginals_grad_gap
+sv_linear_marginals_grad_temp = orihime.sv_linear_marginals_grad_temp
+sv_linear_value_only = orihime.sv_linear_value_only
 
 
 def _has_cuda_dispatch() -> bool:

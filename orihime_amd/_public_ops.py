@@ -153,6 +153,8 @@ def install_public_operations(namespace: ModuleType) -> None:
     k_nwa = import_module(".ops.nw_affine", __package__)
     svmod = import_module(".sv", __package__)
     k_sva = import_module(".ops.sv_affine", __package__)
+    k_nwl = import_module(".ops.nw", __package__)
+    k_svl = import_module(".ops.sv_linear", __package__)
 
     ops = {
         "sw": PublicOperation(name="sw", operator=swmod._sw_operator, map_function=swmod.sw,
@@ -165,12 +167,17 @@ def install_public_operations(namespace: ModuleType) -> None:
                                      map_function=nwmod.nw_affine,
                                      value_function=nwmod.nw_affine_value,
                                      entropy_function=nwmod.nw_affine_entropy),
+        "nw": PublicOperation(name="nw", operator=nwmod._nw_operator, map_function=nwmod.nw,
+                              value_function=nwmod.nw_value, entropy_function=nwmod.nw_entropy),
+        "sv": PublicOperation(name="sv", operator=svmod._sv_operator, map_function=svmod.sv,
+                              value_function=svmod.sv_value, entropy_function=svmod.sv_entropy),
         "sv_affine": PublicOperation(name="sv_affine", operator=svmod._sv_affine_operator,
                                      map_function=svmod.sv_affine,
                                      value_function=svmod.sv_affine_value,
                                      entropy_function=svmod.sv_affine_entropy),
     }
-    namespace._kernels = MappingProxyType({"sw": k_sw, "sw_affine": k_aff, "nw_affine": k_nwa,
+    namespace._kernels = MappingProxyType({"sw": k_sw, "sw_affine": k_aff, "nw": k_nwl,
+                                           "nw_affine": k_nwa, "sv_linear": k_svl,
                                            "sv_affine": k_sva})
     for name, op in ops.items():
         setattr(namespace, name, op)

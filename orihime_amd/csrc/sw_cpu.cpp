@@ -86,6 +86,7 @@ struct Problem {
     float* g1;
     float* g2;
     float* contract;
+    bool gap_sum;             // NW / SV: g0 = opens + extends (linear-gap models)
 };
 
 struct PairRef {
@@ -391,7 +392,7 @@ void nw_pair(const Problem& P, int b, bool want_bwd) {
         // one leading gap over the non-empty side, or the empty alignment
         const int n = L1 + L2;
         P.value[b] = n == 0 ? 0.0f : P.go + (float)(n - 1) * P.ge;
-        if (P.g0) P.g0[b] = n > 0 ? 1.0f : 0.0f;
+        if (P.g0) P.g0[b] = n > 0 ? (P.gap_sum ? (float)n : 1.0f) : 0.0f;
         if (P.g1) P.g1[b] = n > 0 ? (float)(n - 1) : 0.0f;
         if (P.g2) P.g2[b] = 0.0f;
         if (want_bwd) zero_pad(P, r, P.map);
@@ -453,7 +454,7 @@ void nw_pair(const Problem& P, int b, bool want_bwd) {
                 ss += (double)PM[c] * s[q];
             }
         }
-    if (P.g0) P.g0[b] = (float)so;
+    if (P.g0) P.g0[b] = (float)(P.gap_sum ? so + se : so);
     if (P.g1) P.g1[b] = (float)se;
     if (P.g2) P.g2[b] = (float)((S - (ss + (double)P.go * so + (double)P.ge * se)) / P.T);
     zero_pad(P, r, P.map);
@@ -530,7 +531,7 @@ void sv_pair(const Problem& P, int b, bool want_bwd) {
             P.map[r.off + q] = PM[c];
             ss += (double)PM[c] * s[q];
         }
-    if (P.g0) P.g0[b] = (float)so;
+    if (P.g0) P.g0[b] = (float)(P.gap_sum ? so + se : so);
     if (P.g1) P.g1[b] = (float)se;
     if (P.g2) P.g2[b] = (float)((S - (ss + (double)P.go * so + (double)P.ge * se)) / P.T);
     zero_pad(P, r, P.map);
@@ -635,6 +636,38 @@ extern "C" ohm_status ohm_sv_affine_forward_cpu(const float* scores, const int32
     if (!std::isfinite(gap_open) || !std::isfinite(gap_ext))
         return cpu_fail(OHM_EINVAL, "gap scores must be finite");
     const bool bwd = map || grad_open || grad_ext || grad_temp;
+    if (bwd && !map) return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
+    parallel_pairs(B, threads, [&](int b) { sv_pair(P, b, bwd); });
+    return OHM_OK;
+}
+
+extern "C" ohm_status ohm_nw_forward_cpu(const float* scores, const int32_t* lengths,
+                                         const int64_t* offsets, int32_t B, int32_t L1max,
+                                         int32_t L2max, float gap, float temp, float* value,
+                                         float* map, float* grad_gap, float* grad_temp,
+                                         int32_t threads) {
+    Problem P = make(scores, lengths, offsets, B, L1max, L2max, gap, gap, temp);
+    P.value = value; P.map = map; P.g0 = grad_gap; P.g2 = grad_temp; P.gap_sum = true;
+    ohm_status st = check(P);
+    if (st != OHM_OK) return st;
+    if (!std::isfinite(gap)) return cpu_fail(OHM_EINVAL, "gap scores must be finite");
+    const bool bwd = map || grad_gap || grad_temp;
+    if (bwd && !map) return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
+    parallel_pairs(B, threads, [&](int b) { nw_pair(P, b, bwd); });
+    return OHM_OK;
+}
+
+extern "C" ohm_status ohm_sv_forward_cpu(const float* scores, const int32_t* lengths,
+                                         const int64_t* offsets, int32_t B, int32_t L1max,
+                                         int32_t L2max, float gap, float temp, float* value,
+                                         float* map, float* grad_gap, float* grad_temp,
+                                         int32_t threads) {
+    Problem P = make(scores, lengths, offsets, B, L1max, L2max, gap, gap, temp);
+    P.value = value; P.map = map; P.g0 = grad_gap; P.g2 = grad_temp; P.gap_sum = true;
+    ohm_status st = check(P);
+    if (st != OHM_OK) return st;
+    if (!std::isfinite(gap)) return cpu_fail(OHM_EINVAL, "gap scores must be finite");
+    const bool bwd = map || grad_gap || grad_temp;
     if (bwd && !map) return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
     parallel_pairs(B, threads, [&](int b) { sv_pair(P, b, bwd); });
     return OHM_OK;

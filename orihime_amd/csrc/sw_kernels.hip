@@ -509,6 +509,7 @@ struct SwArgs {
     int* pst;                // per pair: kPst* state words of the scaled sweeps
     int sel;                 // log-domain affine sweeps: 0 every pair, 1 only flagged pairs
     int* grid;               // grid-scheduling region (null for more than kGridMaxPairs pairs)
+    int gap_sum;             // NW / SV reverse sweeps: g0 = opens + extends (linear-gap models)
 };
 
 // Claim the next pair for this workgroup (largest first).  Returns -1 when done.
@@ -1457,7 +1458,7 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
 ohm_status nw_forward_impl(const float* scores, const int32_t* lengths, const int64_t* offsets,
                            int B, int L1max, int L2max, float go, float ge, float temp,
                            float* value, float* map, float* g0, float* g1, float* g2, void* ws,
-                           size_t ws_bytes, void* stream) {
+                           size_t ws_bytes, void* stream, bool gap_sum = false) {
     if (B > 0 && !value) return fail(OHM_EINVAL, "value is null");
     const bool need_bwd = map || g0 || g1 || g2;
     if (need_bwd && !map) return fail(OHM_EINVAL, "the gradients need the map buffer");
@@ -1470,6 +1471,7 @@ ohm_status nw_forward_impl(const float* scores, const int32_t* lengths, const in
     L.a.g0 = g0;
     L.a.g1 = g1;
     L.a.g2 = g2;
+    L.a.gap_sum = gap_sum ? 1 : 0;
     if (need_bwd) OHM_DISPATCH_W(L, (nw_affine_forward_kernel<true, W>));
     else OHM_DISPATCH_W(L, (nw_affine_forward_kernel<false, W>));
     stage_mark(1, L.stream);
@@ -1486,7 +1488,7 @@ ohm_status nw_forward_impl(const float* scores, const int32_t* lengths, const in
 ohm_status sv_forward_impl(const float* scores, const int32_t* lengths, const int64_t* offsets,
                            int B, int L1max, int L2max, float go, float ge, float temp,
                            float* value, float* map, float* g0, float* g1, float* g2, void* ws,
-                           size_t ws_bytes, void* stream) {
+                           size_t ws_bytes, void* stream, bool gap_sum = false) {
     if (B > 0 && !value) return fail(OHM_EINVAL, "value is null");
     const bool need_bwd = map || g0 || g1 || g2;
     if (need_bwd && !map) return fail(OHM_EINVAL, "the gradients need the map buffer");
@@ -1499,6 +1501,7 @@ ohm_status sv_forward_impl(const float* scores, const int32_t* lengths, const in
     L.a.g0 = g0;
     L.a.g1 = g1;
     L.a.g2 = g2;
+    L.a.gap_sum = gap_sum ? 1 : 0;
     if (need_bwd) OHM_DISPATCH_W(L, (sv_affine_forward_kernel<true, W>));
     else OHM_DISPATCH_W(L, (sv_affine_forward_kernel<false, W>));
     stage_mark(1, L.stream);
@@ -1669,6 +1672,37 @@ extern "C" ohm_status ohm_sv_affine_forward(const float* scores, const int32_t* 
     return sv_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
                            value, map, grad_open, grad_ext, grad_temp, workspace,
                            workspace_bytes, stream);
+}
+
+// Linear-gap Needleman-Wunsch and Saigo-Vert: the affine sweeps with one gap
+// score in both roles (every move sequence has one state sequence, so the two
+// models are the same function); dS/dgap = dS/dgap_open + dS/dgap_ext.
+extern "C" size_t ohm_nw_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
+                                   int64_t total_cells) {
+    return ohm_nw_affine_workspace(op, B, L1max, L2max, total_cells);
+}
+
+extern "C" size_t ohm_sv_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
+                                   int64_t total_cells) {
+    return ohm_sv_affine_workspace(op, B, L1max, L2max, total_cells);
+}
+
+extern "C" ohm_status ohm_nw_forward(const float* scores, const int32_t* lengths,
+                                     const int64_t* offsets, int32_t B, int32_t L1max,
+                                     int32_t L2max, float gap, float temp, float* value,
+                                     float* map, float* grad_gap, float* grad_temp,
+                                     void* workspace, size_t workspace_bytes, void* stream) {
+    return nw_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap, gap, temp, value, map,
+                           grad_gap, nullptr, grad_temp, workspace, workspace_bytes, stream, true);
+}
+
+extern "C" ohm_status ohm_sv_forward(const float* scores, const int32_t* lengths,
+                                     const int64_t* offsets, int32_t B, int32_t L1max,
+                                     int32_t L2max, float gap, float temp, float* value,
+                                     float* map, float* grad_gap, float* grad_temp,
+                                     void* workspace, size_t workspace_bytes, void* stream) {
+    return sv_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap, gap, temp, value, map,
+                           grad_gap, nullptr, grad_temp, workspace, workspace_bytes, stream, true);
 }
 
 extern "C" size_t ohm_sw_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,

@@ -2,7 +2,8 @@
 
 Every expected output comes from the reference's own CPU kernels
 (src/sw/kernels_cpu.cpp, src/sw_affine/kernels_cpu.cpp,
-src/nw_affine/kernels_cpu.cpp, src/sv_affine/kernels_cpu.cpp) compiled from their
+src/nw_affine/kernels_cpu.cpp, src/sv_affine/kernels_cpu.cpp, src/nw/kernels_cpu.cpp,
+src/sv_linear/kernels_cpu.cpp) compiled from their
 sources into oracle/_ref/ by ``make -C oracle ref``; the composites follow
 src/sw*/torch_cpu.cpp (which passes each primitive runs).  The fixtures hold
 inputs and expected outputs only.  Regenerate with
@@ -103,6 +104,19 @@ def nw_affine_case(name, s, go, ge, T, lengths=None, family="nw"):
     return d
 
 
+def linear_gl_case(name, s, gap, T, lengths=None, family="nw"):
+    B, L1, L2 = s.shape
+    if lengths is None:
+        lengths = np.tile(np.array([L1, L2], np.int32), (B, 1))
+    d = dict(kind=f"{family}_linear", scores=s, lengths=lengths, gap=np.float32(gap),
+             temp=np.float32(T))
+    fwd = O.nw_forward if family == "nw" else O.sv_forward
+    v, m, gg, gt = fwd(s, gap, T, lengths, impl="ref")
+    d.update(value=v, map=m, grad_gap=gg, grad_temp=gt)
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **d)
+    return d
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("build the reference kernels first: make -C oracle ref")
@@ -188,6 +202,18 @@ def main():
                                                  ("gaps_pos", (2, 8, 10), 0.5, 0.2, 1.0)]):
         s = _draw(120 + i, shape, extra=0)[0]
         nw_affine_case(f"sv_affine_edge_{tag}", s, go, ge, T, family="sv")
+    # Linear-gap Needleman-Wunsch and Saigo-Vert (src/nw, src/sv_linear).
+    if not O.linear_ref_available():
+        raise SystemExit("build the reference kernels first: make -C oracle ref")
+    for fam in ("nw", "sv"):
+        for i, (shape, gap, T) in enumerate([((4, 8, 10), -1.0, 1.0), ((4, 16, 16), -0.7, 0.2),
+                                             ((2, 120, 77), -1.5, 0.5)]):
+            s = _draw(130 + i, shape, extra=0)[0]
+            linear_gl_case(f"{fam}_linear_b{shape[0]}_{shape[1]}x{shape[2]}", s, gap, T,
+                           family=fam)
+        s = _draw(140, (5, 40, 36), extra=0)[0]
+        ln = np.array([[40, 36], [17, 29], [0, 6], [33, 0], [1, 1]], np.int32)
+        linear_gl_case(f"{fam}_linear_ragged_b5_40x36", s, -1.0, 0.7, ln, family=fam)
     print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
 
 

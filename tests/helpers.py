@@ -127,6 +127,27 @@ def brute_sv_affine(s, go, ge, T):
     return _lse(totals, T)
 
 
+def brute_nw_linear(s, g, T):
+    """Linear-gap global alignment: every path of diagonal (+ the new cell's
+    score), down (+g) and right (+g) moves from (0, 0) to (L1, L2)."""
+    L1, L2 = s.shape
+    totals = []
+
+    def walk(i, j, acc):
+        if i == L1 and j == L2:
+            totals.append(acc)
+            return
+        if i < L1 and j < L2:
+            walk(i + 1, j + 1, acc + s[i, j])
+        if i < L1:
+            walk(i + 1, j, acc + g)
+        if j < L2:
+            walk(i, j + 1, acc + g)
+
+    walk(0, 0, 0.0)
+    return _lse(totals, T)
+
+
 def brute_linear(s, g, T):
     """Linear gap: start at any cell with its score; moves diag (+s of the new
     cell), down (+g), right (+g); every prefix counts; empty alignment 0."""
