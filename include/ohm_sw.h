@@ -173,7 +173,8 @@ ohm_status ohm_sw_tangent(const float* scores, const int32_t* lengths, const int
 /* ge, no restart, value LSE(M, I, D) at (L1, L2).                          */
 /* ------------------------------------------------------------------------ */
 
-/* Workspace bytes; op is OHM_OP_VALUE or OHM_OP_FORWARD (0 otherwise). */
+/* Workspace bytes; op is OHM_OP_VALUE, OHM_OP_FORWARD or OHM_OP_TANGENT (0
+ * otherwise). */
 size_t ohm_nw_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
                                int64_t total_cells);
 
@@ -190,6 +191,23 @@ ohm_status ohm_nw_affine_forward(const float* scores, const int32_t* lengths,
                                  int32_t L2max, float gap_open, float gap_ext, float temp,
                                  float* value, float* map, float* grad_open, float* grad_ext,
                                  float* grad_temp, void* workspace, size_t workspace_bytes,
+                                 void* stream);
+
+/* Forward-over-reverse pass of the Needleman-Wunsch affine model, arguments
+ * and outputs as ohm_sw_affine_tangent (op OHM_OP_TANGENT for the workspace
+ * query): the HVP (direction), the map's parameter sensitivities (unit
+ * d_gap_open / d_gap_ext / d_temp) and, with contract [B*3], the map VJP's
+ * parameter parts.  Replaces: nw_affine_marginals_hvp,
+ * nw_affine_marginals_grad_*, nw_affine_marginals_backward,
+ * soft_nw_affine_{hvp,param_jacobian,backward_full}
+ * (src/nw_affine/registry.cpp:33-46) and nw_affine_hvp_cpu /
+ * nw_affine_param_grad_cpu (src/nw_affine/kernels_cpu.cpp:343-1002). */
+ohm_status ohm_nw_affine_tangent(const float* scores, const int32_t* lengths,
+                                 const int64_t* offsets, int32_t B, int32_t L1max,
+                                 int32_t L2max, float gap_open, float gap_ext, float temp,
+                                 const float* direction, float d_gap_open, float d_gap_ext,
+                                 float d_temp, float* map_tangent, float* contract, float* value,
+                                 float* map, void* workspace, size_t workspace_bytes,
                                  void* stream);
 
 /* ------------------------------------------------------------------------ */
@@ -290,6 +308,13 @@ ohm_status ohm_sv_forward_cpu(const float* scores, const int32_t* lengths,
                               const int64_t* offsets, int32_t B, int32_t L1max, int32_t L2max,
                               float gap, float temp, float* value, float* map, float* grad_gap,
                               float* grad_temp, int32_t threads);
+
+ohm_status ohm_nw_affine_tangent_cpu(const float* scores, const int32_t* lengths,
+                                     const int64_t* offsets, int32_t B, int32_t L1max,
+                                     int32_t L2max, float gap_open, float gap_ext, float temp,
+                                     const float* direction, float d_gap_open, float d_gap_ext,
+                                     float d_temp, float* map_tangent, float* contract,
+                                     float* value, float* map, int32_t threads);
 
 ohm_status ohm_sw_forward_cpu(const float* scores, const int32_t* lengths,
                               const int64_t* offsets, int32_t B, int32_t L1max, int32_t L2max,

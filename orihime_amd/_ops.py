@@ -303,6 +303,69 @@ def soft_sw_param_jacobian_impl(scores, param_type: int, gap: float, temperature
 # --------------------------------------------------------------------------
 
 
+def _second_order(tangent, linear: bool):
+    """Second-order schema implementations on a forward-over-reverse entry
+    point: HVP (score direction), parameter sensitivities (unit parameter
+    direction) and the map VJP (score direction with contractions).  For a
+    linear-gap family the gap direction moves both affine gap scores and its
+    contraction is the sum of theirs."""
+
+    def run(scores, gap_args, temp, lengths, direction=None, dgo=0.0, dge=0.0, dT=0.0,
+            contract=False):
+        _check_scores(scores, short=True)
+        ln = _check_lengths(lengths, scores)
+        go, ge = gap_args
+        mt = torch.empty_like(scores)
+        c = torch.empty(scores.shape[0], 3, dtype=torch.float32,
+                        device=scores.device) if contract else None
+        tangent(scores, ln, None, go, ge, temp, direction=direction, d_gap_open=dgo,
+                d_gap_ext=dge, d_temp=dT, map_tangent=mt, contract=c)
+        return mt, c
+
+    if linear:
+        def hvp(scores, v, gap: float, temp: float, lengths=None):
+            return run(scores, (gap, gap), temp, lengths, _check_vector(scores, v, "v"))[0]
+
+        def sens(param: int):
+            def impl(scores, gap: float, temp: float, lengths=None):
+                d = (1.0, 1.0, 0.0) if param == 0 else (0.0, 0.0, 1.0)
+                return run(scores, (gap, gap), temp, lengths, None, *d)[0]
+            return impl
+
+        def backward(scores, grad_marginals, gap: float, temp: float, lengths=None):
+            mt, c = run(scores, (gap, gap), temp, lengths,
+                        _check_vector(scores, grad_marginals, "grad_marginals"), contract=True)
+            return mt, (c[:, 0] + c[:, 1]).sum().reshape(1), c[:, 2].sum().reshape(1)
+
+        def jacobian(scores, param_type: int, gap: float, temperature: float, lengths=None):
+            if param_type not in (0, 1):
+                raise RuntimeError("param_type must be 0 (gap) or 1 (temperature)")
+            return sens(param_type)(scores, gap, temperature, lengths)
+    else:
+        def hvp(scores, v, go: float, ge: float, temp: float, lengths=None):
+            return run(scores, (go, ge), temp, lengths, _check_vector(scores, v, "v"))[0]
+
+        def sens(param: int):
+            def impl(scores, go: float, ge: float, temp: float, lengths=None):
+                d = [0.0, 0.0, 0.0]
+                d[param] = 1.0
+                return run(scores, (go, ge), temp, lengths, None, *d)[0]
+            return impl
+
+        def backward(scores, grad_marginals, go: float, ge: float, temp: float, lengths=None):
+            mt, c = run(scores, (go, ge), temp, lengths,
+                        _check_vector(scores, grad_marginals, "grad_marginals"), contract=True)
+            return mt, c[:, 0].sum().reshape(1), c[:, 1].sum().reshape(1), c[:, 2].sum().reshape(1)
+
+        def jacobian(scores, param_type: int, go: float, ge: float, temperature: float,
+                     lengths=None):
+            if param_type not in (0, 1, 2):
+                raise RuntimeError("param_type must be 0 (gap_open), 1 (gap_ext) or 2 (temperature)")
+            return sens(param_type)(scores, go, ge, temperature, lengths)
+
+    return hvp, sens, backward, jacobian
+
+
 def _three_state_family(family: str, call, value_first: bool):
     """Implementations of one family's first-order schemas.  ``call`` is the
     ``_lib`` entry point; ``value_first``: ``*_value_grad_params`` returns the
@@ -357,8 +420,14 @@ def _three_state_family(family: str, call, value_first: bool):
     return forward, forward_t, value_grad_params, with_grads, value_only, second_order
 
 
-def _three_state_schemas(family: str, call, value_first: bool) -> dict:
+def _three_state_schemas(family: str, call, value_first: bool, tangent=None) -> dict:
     fwd, fwd_t, vgp, wg, vo, so = _three_state_family(family, call, value_first)
+    if tangent is not None:
+        hvp, sens, mb, jac = _second_order(tangent, linear=False)
+        so2 = {"marginals_backward": mb, "marginals_hvp": hvp, "marginals_grad_gap_open": sens(0),
+               "marginals_grad_gap_ext": sens(1), "marginals_grad_temp": sens(2), "hvp": hvp,
+               "param_jacobian": jac, "backward_full": mb}
+        so = so2.__getitem__
     f = family
     S = "(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths)"
     L = "(Tensor scores, float gap_open, float gap_ext, float temperature, Tensor? lengths)"
@@ -382,7 +451,7 @@ def _three_state_schemas(family: str, call, value_first: bool) -> dict:
     }
 
 
-def _linear_family_schemas(ns: str, soft: str, call, model: str) -> dict:
+def _linear_family_schemas(ns: str, soft: str, call, model: str, tangent=None) -> dict:
     """First-order schemas of a linear-gap family (src/nw/registry.cpp:35-49,
     src/sv_linear/registry.cpp): the affine sweeps with one gap score in both
     roles, dS/dgap = dS/dgap_open + dS/dgap_ext (summed in the kernel)."""
@@ -426,6 +495,12 @@ def _linear_family_schemas(ns: str, soft: str, call, model: str) -> dict:
                 "orihime_amd yet; the value, map and parameter gradients are available")
 
         return impl
+
+    if tangent is not None:
+        hvp, sens, mb, jac = _second_order(tangent, linear=True)
+        second_order = {"marginals_backward": mb, "marginals_hvp": hvp,
+                        "marginals_grad_gap": sens(0), "marginals_grad_temp": sens(1),
+                        "hvp": hvp, "param_jacobian": jac, "backward_full": mb}.__getitem__
 
     S = "(Tensor scores, float gap, float temp, Tensor? lengths)"
     L = "(Tensor scores, float gap, float temperature, Tensor? lengths)"
@@ -490,9 +565,10 @@ _SCHEMAS = {
 }
 
 
-_SCHEMAS.update(_three_state_schemas("nw", _lib.nw_affine_forward, True))
+_SCHEMAS.update(_three_state_schemas("nw", _lib.nw_affine_forward, True, _lib.nw_affine_tangent))
 _SCHEMAS.update(_three_state_schemas("sv", _lib.sv_affine_forward, False))
-_SCHEMAS.update(_linear_family_schemas("nw", "soft_nw", _lib.nw_forward, "nw"))
+_SCHEMAS.update(_linear_family_schemas("nw", "soft_nw", _lib.nw_forward, "nw",
+                                       _lib.nw_affine_tangent))
 _SCHEMAS.update(_linear_family_schemas("sv_linear", "soft_sv_linear", _lib.sv_forward, "sv"))
 
 

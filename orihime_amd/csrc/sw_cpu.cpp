@@ -537,6 +537,135 @@ void sv_pair(const Problem& P, int b, bool want_bwd) {
     zero_pad(P, r, P.map);
 }
 
+// --------------------------------------------------------------------------
+// Needleman-Wunsch affine, forward-over-reverse along a direction (v on the
+// scores, dgo, dge, dT on the parameters), one pair.  All values in log2
+// units x~ = x log2(e) / T, so the temperature enters only through the
+// inputs s~ = s k, g~ = g k (k' = -k dT / T) and every LSE is a plain base-2
+// LSE: the forward tangent of a LSE is the weighted sum of its arguments'
+// tangents; the reverse carries P and P' with w' = ln2 w (z' - sum w z') for
+// each base-2 softmax.  Outputs: map tangent P_M', and with `contract` the
+// tangents of (opens, extends, dS/dT) along a score direction.
+// --------------------------------------------------------------------------
+
+void nw_tangent_pair(const Problem& P, int b) {
+    const PairRef r = pair_ref(P, b);
+    const int L1 = r.L1, L2 = r.L2;
+    if (L1 == 0 || L2 == 0) {  // closed form: no score enters, the map is zero
+        const int n = L1 + L2;
+        if (P.value) P.value[b] = n == 0 ? 0.0f : P.go + (float)(n - 1) * P.ge;
+        if (P.map) zero_pad(P, r, P.map);
+        if (P.map_t) zero_pad(P, r, P.map_t);
+        if (P.contract) for (int c = 0; c < 3; ++c) P.contract[3 * b + c] = 0.0f;
+        return;
+    }
+    const float* s = P.scores + r.off;
+    const float* v = P.dir ? P.dir + r.off : nullptr;
+    const float k = P.kap, kd = -P.kap * P.dT / P.T;  // k'
+    const float go2 = P.go2, ge2 = P.ge2;
+    const float dgo2 = P.dgo * k + P.go * kd, dge2 = P.dge * k + P.ge * kd;
+    const int C = L2 + 1;
+    const size_t n = (size_t)(L1 + 1) * C;
+    auto at = [C](int i, int j) { return (size_t)i * C + j; };
+    // base-2 softmax of three values
+    auto sm3 = [](float a, float c, float d, float* w) {
+        float hi = std::max(std::max(a, c), d);
+        float ea = ex2(a - hi), ec = ex2(c - hi), ed = ex2(d - hi), q = 1.0f / (ea + ec + ed);
+        w[0] = ea * q; w[1] = ec * q; w[2] = ed * q;
+    };
+    auto l3 = [&](float a, float c, float d, float da, float dc, float dd, float& dy) {
+        float w[3];
+        sm3(a, c, d, w);
+        dy = w[0] * da + w[1] * dc + w[2] * dd;
+        float hi = std::max(std::max(a, c), d);
+        return hi + lg2(ex2(a - hi) + ex2(c - hi) + ex2(d - hi));
+    };
+    std::vector<float> M(n, kNinf), I(n, kNinf), D(n, kNinf), dM(n, 0.0f), dI(n, 0.0f), dD(n, 0.0f);
+    M[0] = 0.0f;
+    for (int i = 1; i <= L1; ++i) {
+        I[at(i, 0)] = go2 + (float)(i - 1) * ge2;
+        dI[at(i, 0)] = dgo2 + (float)(i - 1) * dge2;
+    }
+    for (int j = 1; j <= L2; ++j) {
+        D[at(0, j)] = go2 + (float)(j - 1) * ge2;
+        dD[at(0, j)] = dgo2 + (float)(j - 1) * dge2;
+    }
+    for (int i = 1; i <= L1; ++i)
+        for (int j = 1; j <= L2; ++j) {
+            const size_t c = at(i, j), dg = at(i - 1, j - 1), up = at(i - 1, j), lf = at(i, j - 1);
+            const int64_t q = (int64_t)(i - 1) * r.ld + j - 1;
+            float dA;
+            const float A = l3(M[dg], I[dg], D[dg], dM[dg], dI[dg], dD[dg], dA);
+            M[c] = s[q] * k + A;
+            dM[c] = (v ? v[q] * k : 0.0f) + s[q] * kd + dA;
+            float di, dd;
+            I[c] = l3(M[up] + go2, I[up] + ge2, D[up] + go2, dM[up] + dgo2, dI[up] + dge2,
+                      dD[up] + dgo2, di);
+            D[c] = l3(M[lf] + go2, I[lf] + go2, D[lf] + ge2, dM[lf] + dgo2, dI[lf] + dgo2,
+                      dD[lf] + dge2, dd);
+            dI[c] = di;
+            dD[c] = dd;
+        }
+    const size_t f = at(L1, L2);
+    float dS2;
+    const float S2 = l3(M[f], I[f], D[f], dM[f], dI[f], dD[f], dS2);
+    if (P.value) P.value[b] = S2 / k;
+    std::vector<float> PM(n, 0.0f), PI(n, 0.0f), PD(n, 0.0f), tM(n, 0.0f), tI(n, 0.0f), tD(n, 0.0f);
+    double so = 0.0, se = 0.0, ss = 0.0;
+    const float ln2 = kLn2;
+    // weights of a base-2 softmax over z and their tangents along dz
+    auto wt = [&](const float* z, const float* dz, float* w, float* dw) {
+        sm3(z[0], z[1], z[2], w);
+        const float m = w[0] * dz[0] + w[1] * dz[1] + w[2] * dz[2];
+        for (int t = 0; t < 3; ++t) dw[t] = ln2 * w[t] * (dz[t] - m);
+    };
+    for (int i = L1; i >= 0; --i)
+        for (int j = L2; j >= 0; --j) {
+            if (i == 0 && j == 0) continue;
+            const size_t c = at(i, j);
+            // zero tangents for the minus-infinity states
+            const float dm = M[c] > kNinf ? dM[c] : 0.0f, dii = I[c] > kNinf ? dI[c] : 0.0f,
+                        ddd = D[c] > kNinf ? dD[c] : 0.0f;
+            float zd[3] = {M[c], I[c], D[c]}, dzd[3] = {dm, dii, ddd};
+            float zi[3] = {M[c] + go2, I[c] + ge2, D[c] + go2}, dzi[3] = {dm + dgo2, dii + dge2, ddd + dgo2};
+            float zx[3] = {M[c] + go2, I[c] + go2, D[c] + ge2}, dzx[3] = {dm + dgo2, dii + dgo2, ddd + dge2};
+            float wd[3], wi[3], wx[3], dwd[3], dwi[3], dwx[3];
+            wt(zd, dzd, wd, dwd);
+            wt(zi, dzi, wi, dwi);
+            wt(zx, dzx, wx, dwx);
+            const bool dgn = i < L1 && j < L2, iin = i < L1 && j >= 1, xin = i >= 1 && j < L2;
+            const float Pd = (dgn ? PM[at(i + 1, j + 1)] : 0.0f) + (i == L1 && j == L2 ? 1.0f : 0.0f);
+            const float Td = dgn ? tM[at(i + 1, j + 1)] : 0.0f;
+            const float Pi = iin ? PI[at(i + 1, j)] : 0.0f, Ti = iin ? tI[at(i + 1, j)] : 0.0f;
+            const float Px = xin ? PD[at(i, j + 1)] : 0.0f, Tx = xin ? tD[at(i, j + 1)] : 0.0f;
+            float* Pc[3] = {&PM[c], &PI[c], &PD[c]};
+            float* Tc[3] = {&tM[c], &tI[c], &tD[c]};
+            for (int t = 0; t < 3; ++t) {
+                *Pc[t] = wd[t] * Pd + wi[t] * Pi + wx[t] * Px;
+                *Tc[t] = dwd[t] * Pd + wd[t] * Td + dwi[t] * Pi + wi[t] * Ti + dwx[t] * Px + wx[t] * Tx;
+            }
+            // tangents of the open / extend flows
+            so += (double)Ti * (wi[0] + wi[2]) + (double)Pi * (dwi[0] + dwi[2]) +
+                  (double)Tx * (wx[0] + wx[1]) + (double)Px * (dwx[0] + dwx[1]);
+            se += (double)Ti * wi[1] + (double)Pi * dwi[1] + (double)Tx * wx[2] + (double)Px * dwx[2];
+            if (j == 0) { so += tI[c]; se += (double)(i - 1) * tI[c]; }  // I(i, 0)
+            if (i == 0) { so += tD[c]; se += (double)(j - 1) * tD[c]; }  // D(0, j)
+            if (i >= 1 && j >= 1) {
+                const int64_t q = (int64_t)(i - 1) * r.ld + j - 1;
+                if (P.map) P.map[r.off + q] = PM[c];
+                if (P.map_t) P.map_t[r.off + q] = tM[c];
+                ss += (double)tM[c] * s[q];
+            }
+        }
+    if (P.contract) {
+        P.contract[3 * b + 0] = (float)so;
+        P.contract[3 * b + 1] = (float)se;
+        P.contract[3 * b + 2] = (float)(-(ss + (double)P.go * so + (double)P.ge * se) / P.T);
+    }
+    if (P.map) zero_pad(P, r, P.map);
+    if (P.map_t) zero_pad(P, r, P.map_t);
+}
+
 template <typename F>
 void parallel_pairs(int B, int threads, F&& fn) {
     int nt = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
@@ -670,6 +799,24 @@ extern "C" ohm_status ohm_sv_forward_cpu(const float* scores, const int32_t* len
     const bool bwd = map || grad_gap || grad_temp;
     if (bwd && !map) return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
     parallel_pairs(B, threads, [&](int b) { sv_pair(P, b, bwd); });
+    return OHM_OK;
+}
+
+extern "C" ohm_status ohm_nw_affine_tangent_cpu(const float* scores, const int32_t* lengths,
+                                                const int64_t* offsets, int32_t B, int32_t L1max,
+                                                int32_t L2max, float gap_open, float gap_ext,
+                                                float temp, const float* direction,
+                                                float d_gap_open, float d_gap_ext, float d_temp,
+                                                float* map_tangent, float* contract, float* value,
+                                                float* map, int32_t threads) {
+    Problem P = make(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp);
+    P.dir = direction; P.dgo = d_gap_open; P.dge = d_gap_ext; P.dT = d_temp;
+    P.map_t = map_tangent; P.contract = contract; P.value = value; P.map = map;
+    ohm_status st = check(P);
+    if (st != OHM_OK) return st;
+    if (contract && (d_gap_open != 0.0f || d_gap_ext != 0.0f || d_temp != 0.0f))
+        return cpu_fail(OHM_EINVAL, "contract needs a pure score direction");
+    parallel_pairs(B, threads, [&](int b) { nw_tangent_pair(P, b); });
     return OHM_OK;
 }
 

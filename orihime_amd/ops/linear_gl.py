@@ -4,8 +4,9 @@
 One native call (``soft_nw_with_grads`` / ``soft_sv_linear_with_grads``)
 gives value, map and the per-pair gap and temperature gradients, so the
 backward role reuses them; value-only calls run the forward sweep alone.
-Derivatives of the map (HVP, parameter sensitivities, map VJP) have no
-kernel yet and raise NotImplementedError.
+Derivatives of the map (HVP, parameter sensitivities, map VJP) run the
+Needleman-Wunsch forward-over-reverse pass (one gap direction moving both
+affine gap scores) or raise NotImplementedError (Saigo-Vert: no kernel yet).
 """
 
 from __future__ import annotations
@@ -48,11 +49,24 @@ def _kernels(ns: str, soft: str) -> OperatorKernels:
 
     def param_backward(call: KernelCall, state, name: str, grad_map: Tensor | None) -> Tensor:
         params, lengths = prepare(call)
-        return getattr(_ops, f"{ns}_marginals_grad_{name}")(call.primary, *params, lengths)
+        field = getattr(_ops, f"{ns}_marginals_grad_{name}")(call.primary, *params, lengths)
+        if grad_map is None:
+            return field
+        result = (grad_map * field).sum()
+        param = call.params[0 if name == "gap" else 1]
+        if isinstance(param, Tensor):
+            return result.reshape(param.shape).to(device=param.device, dtype=param.dtype)
+        return result.reshape(1)
+
+    def map_vjp(call: KernelCall, grad_map: Tensor):
+        params, lengths = prepare(call)
+        gs, a, b = getattr(_ops, f"{ns}_marginals_backward")(call.primary, grad_map.contiguous(),
+                                                             *params, lengths)
+        return gs, (a, b)
 
     return OperatorKernels(forward=forward, backward=backward, map_backward=map_backward,
-                           param_backward=param_backward, value_only=value_only,
-                           config={"mask1": None, "mask2": None})
+                           param_backward=param_backward, map_vjp=map_vjp,
+                           value_only=value_only, config={"mask1": None, "mask2": None})
 
 
 NW_KERNELS = _kernels("nw", "soft_nw")

@@ -6,7 +6,8 @@ gives value, map and the per-pair parameter gradients, so the backward role
 reuses them instead of running ``nw_affine_value_grad_params`` again
 (orihime/ops/nw_affine.py:117-130 does); value-only calls run the forward
 sweep alone.  Derivatives of the map (HVP, parameter sensitivities, map
-VJP) have no kernel yet and raise NotImplementedError.
+VJP) run the forward-over-reverse pass (Needleman-Wunsch) or raise
+NotImplementedError (Saigo-Vert: no kernel yet).
 """
 
 from __future__ import annotations
@@ -62,7 +63,21 @@ _SENS = {
 
 def _param_backward(call: KernelCall, state, param_name: str, grad_map: Tensor | None) -> Tensor:
     params, lengths = _prepare(call)
-    return getattr(_ops, _SENS[param_name])(call.primary, *params, lengths)
+    field = getattr(_ops, _SENS[param_name])(call.primary, *params, lengths)
+    if grad_map is None:
+        return field
+    result = (grad_map * field).sum()
+    param = call.params[_PARAM_NAMES.index(param_name)]
+    if isinstance(param, Tensor):
+        return result.reshape(param.shape).to(device=param.device, dtype=param.dtype)
+    return result.reshape(1)
+
+
+def _map_vjp(call: KernelCall, grad_map: Tensor):
+    params, lengths = _prepare(call)
+    gs, a, b, c = _ops.nw_affine_marginals_backward(call.primary, grad_map.contiguous(), *params,
+                                                 lengths)
+    return gs, (a, b, c)
 
 
 NW_AFFINE_KERNELS = OperatorKernels(
@@ -70,6 +85,7 @@ NW_AFFINE_KERNELS = OperatorKernels(
     backward=_backward_pass,
     map_backward=_map_backward,
     param_backward=_param_backward,
+    map_vjp=_map_vjp,
     value_only=_value_only,
     config={"mask1": None, "mask2": None},
 )

@@ -125,8 +125,9 @@ def test_public_functions_and_gradients(fam):
     assert abs(g.grad.item() - float(d["grad_gap"].sum())) < 1e-3 * max(1.0, abs(g.grad.item()))
     out = getattr(ohm.ops, fam).forward(x.detach(), gap_score=gap, temperature=T, lengths=ln)
     assert set(out) == {"map", "value", "entropy"}
-    with pytest.raises(NotImplementedError):
-        getattr(ohm, fam)(x, gap_score=gap, temperature=T, lengths=ln).sum().backward()
+    if fam == "sv":  # the Saigo-Vert map has no second-order kernel yet
+        with pytest.raises(NotImplementedError):
+            getattr(ohm, fam)(x, gap_score=gap, temperature=T, lengths=ln).sum().backward()
 
 
 @pytest.mark.gpu
