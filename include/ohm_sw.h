@@ -216,7 +216,8 @@ ohm_status ohm_nw_affine_tangent(const float* scores, const int32_t* lengths,
 /* M only, S = LSE(0, {M(i,j)}).                                           */
 /* ------------------------------------------------------------------------ */
 
-/* Workspace bytes; op is OHM_OP_VALUE or OHM_OP_FORWARD (0 otherwise). */
+/* Workspace bytes; op is OHM_OP_VALUE, OHM_OP_FORWARD or OHM_OP_TANGENT (0
+ * otherwise). */
 size_t ohm_sv_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
                                int64_t total_cells);
 
@@ -261,6 +262,19 @@ ohm_status ohm_sv_forward(const float* scores, const int32_t* lengths, const int
                           int32_t B, int32_t L1max, int32_t L2max, float gap, float temp,
                           float* value, float* map, float* grad_gap, float* grad_temp,
                           void* workspace, size_t workspace_bytes, void* stream);
+
+/* Forward-over-reverse pass of the Saigo-Vert affine model, as
+ * ohm_nw_affine_tangent.  Replaces: sv_affine_marginals_hvp,
+ * sv_affine_marginals_grad_*, sv_affine_marginals_backward,
+ * soft_sv_affine_{hvp,param_jacobian,backward_full} and sv_affine_hvp_cpu /
+ * sv_affine_param_grad_cpu (src/sv_affine/kernels_cpu.cpp:362-935). */
+ohm_status ohm_sv_affine_tangent(const float* scores, const int32_t* lengths,
+                                 const int64_t* offsets, int32_t B, int32_t L1max,
+                                 int32_t L2max, float gap_open, float gap_ext, float temp,
+                                 const float* direction, float d_gap_open, float d_gap_ext,
+                                 float d_temp, float* map_tangent, float* contract, float* value,
+                                 float* map, void* workspace, size_t workspace_bytes,
+                                 void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Host (CPU) twins: same arguments on host memory, no workspace or stream. */
@@ -310,6 +324,13 @@ ohm_status ohm_sv_forward_cpu(const float* scores, const int32_t* lengths,
                               float* grad_temp, int32_t threads);
 
 ohm_status ohm_nw_affine_tangent_cpu(const float* scores, const int32_t* lengths,
+                                     const int64_t* offsets, int32_t B, int32_t L1max,
+                                     int32_t L2max, float gap_open, float gap_ext, float temp,
+                                     const float* direction, float d_gap_open, float d_gap_ext,
+                                     float d_temp, float* map_tangent, float* contract,
+                                     float* value, float* map, int32_t threads);
+
+ohm_status ohm_sv_affine_tangent_cpu(const float* scores, const int32_t* lengths,
                                      const int64_t* offsets, int32_t B, int32_t L1max,
                                      int32_t L2max, float gap_open, float gap_ext, float temp,
                                      const float* direction, float d_gap_open, float d_gap_ext,

@@ -61,6 +61,8 @@ EXPORTS = (
     "ohm_sv_forward_cpu",
     "ohm_nw_affine_tangent",
     "ohm_nw_affine_tangent_cpu",
+    "ohm_sv_affine_tangent",
+    "ohm_sv_affine_tangent_cpu",
 )
 
 
@@ -110,6 +112,10 @@ _sva_fwd_cpu = _sig("ohm_sv_affine_forward_cpu", _i32, *_PROBLEM, _f, _f, _f,
 _nwa_tan = _sig("ohm_nw_affine_tangent", _i32, *_PROBLEM, _f, _f, _f,
                 _vp, _f, _f, _f, _vp, _vp, _vp, _vp, _vp, _sz, _vp)
 _nwa_tan_cpu = _sig("ohm_nw_affine_tangent_cpu", _i32, *_PROBLEM, _f, _f, _f,
+                    _vp, _f, _f, _f, _vp, _vp, _vp, _vp, _i32)
+_sva_tan = _sig("ohm_sv_affine_tangent", _i32, *_PROBLEM, _f, _f, _f,
+                _vp, _f, _f, _f, _vp, _vp, _vp, _vp, _vp, _sz, _vp)
+_sva_tan_cpu = _sig("ohm_sv_affine_tangent_cpu", _i32, *_PROBLEM, _f, _f, _f,
                     _vp, _f, _f, _f, _vp, _vp, _vp, _vp, _i32)
 _nwl_fwd = _sig("ohm_nw_forward", _i32, *_PROBLEM, _f, _f, _vp, _vp, _vp, _vp, _vp, _sz, _vp)
 _svl_fwd = _sig("ohm_sv_forward", _i32, *_PROBLEM, _f, _f, _vp, _vp, _vp, _vp, _vp, _sz, _vp)
@@ -331,12 +337,10 @@ nw_forward = _linear_global_local("nw")
 sv_forward = _linear_global_local("sv")
 
 
-def nw_affine_tangent(scores, lengths, offsets, gap_open, gap_ext, temp, *, direction=None,
+def _three_state_tangent(gpu, cpu, ws_fn, name, scores, lengths, offsets, gap_open, gap_ext, temp, *, direction=None,
                       d_gap_open=0.0, d_gap_ext=0.0, d_temp=0.0, map_tangent=None,
                       contract=None, value=None, map=None, L1max=None, L2max=None,
                       workspace=None):
-    """Needleman-Wunsch affine forward-over-reverse pass (arguments as
-    ``affine_tangent``)."""
     B, L1, L2 = _dims(scores, lengths, offsets, L1max, L2max)
     args = (_ptr(scores), _ptr(lengths), _ptr(offsets), B, L1, L2,
             float(gap_open), float(gap_ext), float(temp), _ptr(direction),
@@ -344,9 +348,23 @@ def nw_affine_tangent(scores, lengths, offsets, gap_open, gap_ext, temp, *, dire
             _ptr(map_tangent), _ptr(contract), _ptr(value), _ptr(map))
     if scores.is_cuda:
         total = -1 if offsets is None else scores.numel()
-        need = int(_ws_nw_affine(OP_TANGENT, B, L1, L2, total))
+        need = int(ws_fn(OP_TANGENT, B, L1, L2, total))
         stream = torch.cuda.current_stream(scores.device).cuda_stream
         ws = workspace if workspace is not None else _workspace(scores.device, need, stream)
-        _check(_nwa_tan(*args, ws.data_ptr(), ws.numel(), stream), "ohm_nw_affine_tangent")
+        _check(gpu(*args, ws.data_ptr(), ws.numel(), stream), name)
     else:
-        _check(_nwa_tan_cpu(*args, _cpu_threads()), "ohm_nw_affine_tangent_cpu")
+        _check(cpu(*args, _cpu_threads()), name + "_cpu")
+
+
+def nw_affine_tangent(*args, **kw):
+    """Needleman-Wunsch affine forward-over-reverse pass (arguments as
+    ``affine_tangent``)."""
+    return _three_state_tangent(_nwa_tan, _nwa_tan_cpu, _ws_nw_affine, "ohm_nw_affine_tangent",
+                                *args, **kw)
+
+
+def sv_affine_tangent(*args, **kw):
+    """Saigo-Vert affine forward-over-reverse pass (arguments as
+    ``affine_tangent``)."""
+    return _three_state_tangent(_sva_tan, _sva_tan_cpu, _ws_sv_affine, "ohm_sv_affine_tangent",
+                                *args, **kw)

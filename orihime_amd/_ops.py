@@ -566,10 +566,11 @@ _SCHEMAS = {
 
 
 _SCHEMAS.update(_three_state_schemas("nw", _lib.nw_affine_forward, True, _lib.nw_affine_tangent))
-_SCHEMAS.update(_three_state_schemas("sv", _lib.sv_affine_forward, False))
+_SCHEMAS.update(_three_state_schemas("sv", _lib.sv_affine_forward, False, _lib.sv_affine_tangent))
 _SCHEMAS.update(_linear_family_schemas("nw", "soft_nw", _lib.nw_forward, "nw",
                                        _lib.nw_affine_tangent))
-_SCHEMAS.update(_linear_family_schemas("sv_linear", "soft_sv_linear", _lib.sv_forward, "sv"))
+_SCHEMAS.update(_linear_family_schemas("sv_linear", "soft_sv_linear", _lib.sv_forward, "sv",
+                                       _lib.sv_affine_tangent))
 
 
 def _fake_for(name: str, schema: str):

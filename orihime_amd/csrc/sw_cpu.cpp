@@ -666,6 +666,123 @@ void nw_tangent_pair(const Problem& P, int b) {
     if (P.map_t) zero_pad(P, r, P.map_t);
 }
 
+// --------------------------------------------------------------------------
+// Saigo-Vert affine, forward-over-reverse along a direction, one pair (log2
+// units as nw_tangent_pair): the restart split sig(A) has the tangent
+// ln2 sig (1 - sig) A', the terminal seeds 2^(M - S) the tangent
+// ln2 2^(M - S) (M' - S') with S' = sum 2^(M - S) M'.
+// --------------------------------------------------------------------------
+
+void sv_tangent_pair(const Problem& P, int b) {
+    const PairRef r = pair_ref(P, b);
+    const int L1 = r.L1, L2 = r.L2;
+    if (L1 == 0 || L2 == 0) {
+        if (P.value) P.value[b] = 0.0f;
+        if (P.map) zero_pad(P, r, P.map);
+        if (P.map_t) zero_pad(P, r, P.map_t);
+        if (P.contract) for (int c = 0; c < 3; ++c) P.contract[3 * b + c] = 0.0f;
+        return;
+    }
+    const float* s = P.scores + r.off;
+    const float* v = P.dir ? P.dir + r.off : nullptr;
+    const float k = P.kap, kd = -P.kap * P.dT / P.T;
+    const float go2 = P.go2, ge2 = P.ge2;
+    const float dgo2 = P.dgo * k + P.go * kd, dge2 = P.dge * k + P.ge * kd;
+    const int C = L2 + 1;
+    const size_t n = (size_t)(L1 + 1) * C;
+    auto at = [C](int i, int j) { return (size_t)i * C + j; };
+    auto sm3 = [](float a, float c, float d, float* w) {
+        float hi = std::max(std::max(a, c), d);
+        float ea = ex2(a - hi), ec = ex2(c - hi), ed = ex2(d - hi), q = 1.0f / (ea + ec + ed);
+        w[0] = ea * q; w[1] = ec * q; w[2] = ed * q;
+    };
+    auto l3 = [&](float a, float c, float d, float da, float dc, float dd, float& dy) {
+        float w[3];
+        sm3(a, c, d, w);
+        dy = w[0] * da + w[1] * dc + w[2] * dd;
+        float hi = std::max(std::max(a, c), d);
+        return hi + lg2(ex2(a - hi) + ex2(c - hi) + ex2(d - hi));
+    };
+    std::vector<float> M(n, kNinf), I(n, kNinf), D(n, kNinf), A(n, kNinf);
+    std::vector<float> dM(n, 0.0f), dI(n, 0.0f), dD(n, 0.0f), dA(n, 0.0f);
+    double m = 0.0;
+    for (int i = 1; i <= L1; ++i)
+        for (int j = 1; j <= L2; ++j) {
+            const size_t c = at(i, j), dg = at(i - 1, j - 1), up = at(i - 1, j), lf = at(i, j - 1);
+            const int64_t q = (int64_t)(i - 1) * r.ld + j - 1;
+            float dsp;  // LSE(A_diag, 0) and its tangent
+            const float sp = l3(A[dg], 0.0f, kNinf, dA[dg], 0.0f, 0.0f, dsp);
+            M[c] = s[q] * k + sp;
+            dM[c] = (v ? v[q] * k : 0.0f) + s[q] * kd + dsp;
+            float di, dd, da;
+            I[c] = l3(M[up] + go2, I[up] + ge2, kNinf, dM[up] + dgo2, dI[up] + dge2, 0.0f, di);
+            D[c] = l3(M[lf] + go2, I[lf] + go2, D[lf] + ge2, dM[lf] + dgo2, dI[lf] + dgo2,
+                      dD[lf] + dge2, dd);
+            dI[c] = di;
+            dD[c] = dd;
+            A[c] = l3(M[c], I[c], D[c], dM[c], dI[c], dD[c], da);
+            dA[c] = da;
+            m = std::max(m, (double)M[c]);
+        }
+    double z = std::exp2(-m), zt = 0.0;
+    for (int i = 1; i <= L1; ++i)
+        for (int j = 1; j <= L2; ++j) {
+            const double w = std::exp2((double)M[at(i, j)] - m);
+            z += w;
+            zt += w * dM[at(i, j)];
+        }
+    const double S2 = m + std::log2(z), dS2 = zt / z;
+    if (P.value) P.value[b] = (float)(S2 / k);
+    const float ln2 = kLn2;
+    auto wt = [&](const float* zz, const float* dz, float* w, float* dw) {
+        sm3(zz[0], zz[1], zz[2], w);
+        const float mm = w[0] * dz[0] + w[1] * dz[1] + w[2] * dz[2];
+        for (int t = 0; t < 3; ++t) dw[t] = ln2 * w[t] * (dz[t] - mm);
+    };
+    std::vector<float> PM(n, 0.0f), PI(n, 0.0f), PD(n, 0.0f), tM(n, 0.0f), tI(n, 0.0f), tD(n, 0.0f);
+    double so = 0.0, se = 0.0, ss = 0.0;
+    for (int i = L1; i >= 1; --i)
+        for (int j = L2; j >= 1; --j) {
+            const size_t c = at(i, j);
+            float zd[3] = {M[c], I[c], D[c]}, dzd[3] = {dM[c], dI[c], dD[c]};
+            float zi[3] = {M[c] + go2, I[c] + ge2, kNinf}, dzi[3] = {dM[c] + dgo2, dI[c] + dge2, 0.0f};
+            float zx[3] = {M[c] + go2, I[c] + go2, D[c] + ge2};
+            float dzx[3] = {dM[c] + dgo2, dI[c] + dgo2, dD[c] + dge2};
+            float wd[3], wi[3], wx[3], dwd[3], dwi[3], dwx[3];
+            wt(zd, dzd, wd, dwd);
+            wt(zi, dzi, wi, dwi);
+            wt(zx, dzx, wx, dwx);
+            const float sg = 1.0f / (1.0f + ex2(-A[c])), dsg = ln2 * sg * (1.0f - sg) * dA[c];
+            const bool dgn = i < L1 && j < L2;
+            const float Pd = dgn ? PM[at(i + 1, j + 1)] : 0.0f, Td = dgn ? tM[at(i + 1, j + 1)] : 0.0f;
+            const float Pi = i < L1 ? PI[at(i + 1, j)] : 0.0f, Ti = i < L1 ? tI[at(i + 1, j)] : 0.0f;
+            const float Px = j < L2 ? PD[at(i, j + 1)] : 0.0f, Tx = j < L2 ? tD[at(i, j + 1)] : 0.0f;
+            const double e = std::exp2(std::min((double)M[c] - S2, 0.0));
+            const float seed = (float)e, dseed = (float)(e * kLn2 * ((double)dM[c] - dS2));
+            float* Pc[3] = {&PM[c], &PI[c], &PD[c]};
+            float* Tc[3] = {&tM[c], &tI[c], &tD[c]};
+            for (int t = 0; t < 3; ++t) {
+                *Pc[t] = wd[t] * sg * Pd + wi[t] * Pi + wx[t] * Px + (t == 0 ? seed : 0.0f);
+                *Tc[t] = (dwd[t] * sg + wd[t] * dsg) * Pd + wd[t] * sg * Td + dwi[t] * Pi +
+                         wi[t] * Ti + dwx[t] * Px + wx[t] * Tx + (t == 0 ? dseed : 0.0f);
+            }
+            so += (double)Ti * wi[0] + (double)Pi * dwi[0] + (double)Tx * (wx[0] + wx[1]) +
+                  (double)Px * (dwx[0] + dwx[1]);
+            se += (double)Ti * wi[1] + (double)Pi * dwi[1] + (double)Tx * wx[2] + (double)Px * dwx[2];
+            const int64_t q = (int64_t)(i - 1) * r.ld + j - 1;
+            if (P.map) P.map[r.off + q] = PM[c];
+            if (P.map_t) P.map_t[r.off + q] = tM[c];
+            ss += (double)tM[c] * s[q];
+        }
+    if (P.contract) {
+        P.contract[3 * b + 0] = (float)so;
+        P.contract[3 * b + 1] = (float)se;
+        P.contract[3 * b + 2] = (float)(-(ss + (double)P.go * so + (double)P.ge * se) / P.T);
+    }
+    if (P.map) zero_pad(P, r, P.map);
+    if (P.map_t) zero_pad(P, r, P.map_t);
+}
+
 template <typename F>
 void parallel_pairs(int B, int threads, F&& fn) {
     int nt = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
@@ -817,6 +934,24 @@ extern "C" ohm_status ohm_nw_affine_tangent_cpu(const float* scores, const int32
     if (contract && (d_gap_open != 0.0f || d_gap_ext != 0.0f || d_temp != 0.0f))
         return cpu_fail(OHM_EINVAL, "contract needs a pure score direction");
     parallel_pairs(B, threads, [&](int b) { nw_tangent_pair(P, b); });
+    return OHM_OK;
+}
+
+extern "C" ohm_status ohm_sv_affine_tangent_cpu(const float* scores, const int32_t* lengths,
+                                                const int64_t* offsets, int32_t B, int32_t L1max,
+                                                int32_t L2max, float gap_open, float gap_ext,
+                                                float temp, const float* direction,
+                                                float d_gap_open, float d_gap_ext, float d_temp,
+                                                float* map_tangent, float* contract, float* value,
+                                                float* map, int32_t threads) {
+    Problem P = make(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp);
+    P.dir = direction; P.dgo = d_gap_open; P.dge = d_gap_ext; P.dT = d_temp;
+    P.map_t = map_tangent; P.contract = contract; P.value = value; P.map = map;
+    ohm_status st = check(P);
+    if (st != OHM_OK) return st;
+    if (contract && (d_gap_open != 0.0f || d_gap_ext != 0.0f || d_temp != 0.0f))
+        return cpu_fail(OHM_EINVAL, "contract needs a pure score direction");
+    parallel_pairs(B, threads, [&](int b) { sv_tangent_pair(P, b); });
     return OHM_OK;
 }
 

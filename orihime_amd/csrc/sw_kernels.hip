@@ -931,6 +931,7 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
 #include "nw_affine.hiph"
 #include "nw_tangent.hiph"
 #include "sv_affine.hiph"
+#include "sv_tangent.hiph"
 #include "sw_teams.hiph"
 #include "sw_grid.hiph"
 #include "sw_affine_scaled.hiph"
@@ -1235,6 +1236,7 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     const TableSpec tf = op == 3 ? TableSpec{kPlanesNw, 1}
                          : op == 4 ? TableSpec{kPlanesSv, 1}
                          : op == 5 ? TableSpec{kPlanesNwTan, 1}
+                         : op == 6 ? TableSpec{kPlanesSvTan, 1}
                          : affine ? TableSpec{kPlanesAffineCkpt, kCkpt} : TableSpec{kPlanesLinearFwd, 1};
     const TableSpec tt = {affine ? kPlanesAffineTan : kPlanesLinearTan, 1};
     WsLayout w = ws_layout(op >= 3 ? 1 : op, B, L1max, L2max, -1, tf, tt);
@@ -1518,6 +1520,38 @@ ohm_status nw_tangent_impl(const float* scores, const int32_t* lengths, const in
     return check_launch("nw tangent backward");
 }
 
+// Saigo-Vert affine forward-over-reverse pass along one direction.
+ohm_status sv_tangent_impl(const float* scores, const int32_t* lengths, const int64_t* offsets,
+                           int B, int L1max, int L2max, float go, float ge, float temp,
+                           const float* dir, float dgo, float dge, float dT, float* map_t,
+                           float* contract, float* value, float* map, void* ws, size_t ws_bytes,
+                           void* stream) {
+    if (!std::isfinite(dgo) || !std::isfinite(dge) || !std::isfinite(dT))
+        return fail(OHM_EINVAL, "tangent directions must be finite");
+    if (contract && (dgo != 0.0f || dge != 0.0f || dT != 0.0f))
+        return fail(OHM_EINVAL, "contract needs a pure score direction");
+    Launch L;
+    ohm_status st = setup(6, true, scores, lengths, offsets, B, L1max, L2max, go, ge, temp, ws,
+                          ws_bytes, stream, L);
+    if (st != OHM_OK || B == 0) return st;
+    L.a.dir = dir;
+    L.a.d_go = dgo;
+    L.a.d_ge = dge;
+    L.a.d_temp = dT;
+    L.a.map_t = map_t;
+    L.a.contract = contract;
+    L.a.value = value;
+    L.a.map = map;
+    OHM_DISPATCH_W(L, (sv_affine_tangent_forward_kernel<W>));
+    stage_mark(1, L.stream);
+    stage_mark(2, L.stream);
+    if ((st = check_launch("sv tangent forward")) != OHM_OK) return st;
+    OHM_DISPATCH_W(L, (sv_affine_tangent_backward_kernel<W>));
+    stage_mark(3, L.stream);
+    stage_mark(4, L.stream);
+    return check_launch("sv tangent backward");
+}
+
 // Saigo-Vert affine: value, then (when any backward output is asked for) map
 // and parameter gradients.
 ohm_status sv_forward_impl(const float* scores, const int32_t* lengths, const int64_t* offsets,
@@ -1705,9 +1739,22 @@ extern "C" ohm_status ohm_nw_affine_forward(const float* scores, const int32_t* 
 
 extern "C" size_t ohm_sv_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
                                           int64_t total_cells) {
-    if (B < 0 || L1max < 0 || L2max < 0 || op < 0 || op > 1) return 0;
+    if (B < 0 || L1max < 0 || L2max < 0 || op < 0 || op > 2) return 0;
     return ws_layout(op, B, L1max, L2max, total_cells, TableSpec{kPlanesSv, 1},
-                     TableSpec{kPlanesSv, 1}).total;
+                     TableSpec{kPlanesSvTan, 1}).total;
+}
+
+extern "C" ohm_status ohm_sv_affine_tangent(const float* scores, const int32_t* lengths,
+                                            const int64_t* offsets, int32_t B, int32_t L1max,
+                                            int32_t L2max, float gap_open, float gap_ext,
+                                            float temp, const float* direction, float d_gap_open,
+                                            float d_gap_ext, float d_temp, float* map_tangent,
+                                            float* contract, float* value, float* map,
+                                            void* workspace, size_t workspace_bytes,
+                                            void* stream) {
+    return sv_tangent_impl(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
+                           direction, d_gap_open, d_gap_ext, d_temp, map_tangent, contract,
+                           value, map, workspace, workspace_bytes, stream);
 }
 
 extern "C" ohm_status ohm_sv_affine_forward(const float* scores, const int32_t* lengths,

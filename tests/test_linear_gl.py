@@ -125,9 +125,19 @@ def test_public_functions_and_gradients(fam):
     assert abs(g.grad.item() - float(d["grad_gap"].sum())) < 1e-3 * max(1.0, abs(g.grad.item()))
     out = getattr(ohm.ops, fam).forward(x.detach(), gap_score=gap, temperature=T, lengths=ln)
     assert set(out) == {"map", "value", "entropy"}
-    if fam == "sv":  # the Saigo-Vert map has no second-order kernel yet
-        with pytest.raises(NotImplementedError):
-            getattr(ohm, fam)(x, gap_score=gap, temperature=T, lengths=ln).sum().backward()
+    # the map is differentiable too (forward-over-reverse pass with both gap
+    # directions set): its VJP in the gap equals <G, dP/dgap> by central
+    # differences of the oracle
+    x2 = torch.from_numpy(d["scores"]).clone().requires_grad_()
+    g2 = torch.tensor(gap, requires_grad=True)
+    G = torch.from_numpy(np.random.default_rng(1).standard_normal(d["scores"].shape).astype(np.float32))
+    (getattr(ohm, fam)(x2, gap_score=g2, temperature=T, lengths=ln) * G).sum().backward()
+    fw = getattr(O, f"{fam}_forward")
+    h = 1e-5
+    fd = ((fw(d["scores"].astype(np.float64), gap + h, T, d["lengths"], prec="f64")[1] -
+           fw(d["scores"].astype(np.float64), gap - h, T, d["lengths"], prec="f64")[1]) /
+          (2 * h) * G.numpy()).sum()
+    assert abs(g2.grad.item() - fd) < 1e-3 * max(1.0, abs(fd))
 
 
 @pytest.mark.gpu

@@ -174,7 +174,8 @@ def test_library_exports_the_sv_entry_points():
     for name in ("ohm_sv_affine_workspace", "ohm_sv_affine_forward", "ohm_sv_affine_forward_cpu"):
         assert hasattr(_lib._lib, name)
     assert _lib._ws_sv_affine(1, 4, 100, 100, -1) > _lib._ws_sv_affine(0, 4, 100, 100, -1) > 0
-    assert _lib._ws_sv_affine(2, 4, 100, 100, -1) == 0
+    assert _lib._ws_sv_affine(2, 4, 100, 100, -1) > _lib._ws_sv_affine(1, 4, 100, 100, -1)
+    assert _lib._ws_sv_affine(3, 4, 100, 100, -1) == 0
 
 
 @pytest.mark.parametrize("path", SV, ids=IDS)
@@ -246,13 +247,6 @@ def test_invalid_lengths_raise():
     with pytest.raises((RuntimeError, ValueError)):
         ohm.sv_affine(x, gap_open_score=-2.0, gap_extend_score=-0.5,
                       lengths=torch.tensor([[-1, 5]], dtype=torch.int32))
-
-
-def test_second_order_derivatives_say_they_are_missing():
-    x = torch.randn(2, 4, 5, requires_grad=True)
-    mp = ohm.sv_affine(x, gap_open_score=-2.0, gap_extend_score=-0.5)
-    with pytest.raises(NotImplementedError):
-        mp.sum().backward()
 
 
 # --------------------------------------------------------------------------
@@ -341,3 +335,115 @@ def test_gpu_public_functions_run_on_the_hip_kernels():
     v.sum().backward()
     assert abs_err(x.grad.cpu().numpy(), d["map"]) < 1e-4
     assert _grad_err(v.detach().cpu().numpy(), d["value"]) < 1e-5
+
+
+def _tangent(s, ln, go, ge, T, device="cpu", direction=None, d=(0.0, 0.0, 0.0), contract=False):
+    st = torch.as_tensor(s, dtype=torch.float32).to(device).contiguous()
+    lnd = torch.as_tensor(ln, dtype=torch.int32).to(device).contiguous()
+    mt = torch.empty_like(st)
+    c = torch.empty(st.shape[0], 3, device=device) if contract else None
+    dv = None if direction is None else torch.as_tensor(direction, dtype=torch.float32).to(device)
+    _lib.sv_affine_tangent(st, lnd, None, go, ge, T, direction=dv, d_gap_open=d[0],
+                           d_gap_ext=d[1], d_temp=d[2], map_tangent=mt, contract=c)
+    return mt.cpu().numpy(), (None if c is None else c.cpu().numpy())
+
+
+def _fd_second_order(s, ln, go, ge, T, V, G, h=1e-5):
+    """Central differences of the FP64 oracle's map: H.V, dP/dgo, dP/dge,
+    dP/dT, and the parameter parts of the map VJP against G."""
+    def mp(ss=s, a=go, b=ge, t=T):
+        return O.sv_affine_forward(ss, a, b, t, ln, prec="f64")[1]
+    hv = (mp(s + h * V) - mp(s - h * V)) / (2 * h)
+    sens = [(mp(a=go + h) - mp(a=go - h)) / (2 * h), (mp(b=ge + h) - mp(b=ge - h)) / (2 * h),
+            (mp(t=T + h) - mp(t=T - h)) / (2 * h)]
+    con = np.stack([(x * G).sum(axis=(1, 2)) for x in sens], axis=1)
+    return hv, sens, con
+
+
+SECOND = [((3, 7, 9), [[7, 9], [4, 6], [1, 5]], -1.3, -0.4, 0.7),
+          ((2, 5, 4), [[5, 4], [0, 3]], 0.5, 0.2, 1.0)]
+
+
+@pytest.mark.parametrize("shape,ln,go,ge,T", SECOND)
+def test_host_second_order_matches_finite_differences(shape, ln, go, ge, T):
+    rng = np.random.default_rng(shape[1])
+    s, V, G = (rng.standard_normal(shape) for _ in range(3))
+    ln = np.array(ln, np.int32)
+    hv, sens, con = _fd_second_order(s.astype(np.float32).astype(np.float64), ln, go, ge, T, V, G)
+    s32 = s.astype(np.float32)
+    assert abs_err(_tangent(s32, ln, go, ge, T, direction=V)[0], hv) < 1e-5
+    for p in range(3):
+        d = [0.0, 0.0, 0.0]
+        d[p] = 1.0
+        assert abs_err(_tangent(s32, ln, go, ge, T, d=d)[0], sens[p]) < 1e-5
+    mt, c = _tangent(s32, ln, go, ge, T, direction=G, contract=True)
+    assert _grad_err(c, con) < 1e-4
+
+
+def test_map_derivatives_flow_through_autograd():
+    rng = np.random.default_rng(2)
+    s = rng.standard_normal((2, 6, 7)).astype(np.float32)
+    G = rng.standard_normal((2, 6, 7)).astype(np.float32)
+    x = torch.from_numpy(s).requires_grad_()
+    go = torch.tensor(-1.5, requires_grad=True)
+    mp = ohm.sv_affine(x, gap_open_score=go, gap_extend_score=-0.4, temperature=0.8)
+    (mp * torch.from_numpy(G)).sum().backward()
+    hv, sens, con = _fd_second_order(s.astype(np.float64), np.array([[6, 7]] * 2, np.int32),
+                                     -1.5, -0.4, 0.8, G, G)
+    assert abs_err(x.grad.numpy(), hv) < 1e-5
+    assert abs(go.grad.item() - con[:, 0].sum()) < 1e-4 * max(1.0, abs(go.grad.item()))
+    x2 = torch.from_numpy(s).requires_grad_()
+    ohm.sv(x2, gap_score=-0.7, temperature=0.8).sum().backward()
+    assert torch.isfinite(x2.grad).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("B,L1,L2,go,ge,T", [(3, 300, 257, -2.0, -0.5, 0.5),
+                                             (5, 70, 130, -1.0, -0.3, 1.0)])
+def test_gpu_second_order_on_multi_strip_pairs(B, L1, L2, go, ge, T):
+    """Against central differences of the FP64 oracle.  Second-order outputs
+    carry FP32 rounding that accumulates along the reverse recursion; the bar
+    is 2e-4 of the largest entry, as for the Needleman-Wunsch passes."""
+    rng = np.random.default_rng(L1 + L2)
+    s, V, G = (rng.standard_normal((B, L1, L2)).astype(np.float32) for _ in range(3))
+    ln = np.tile(np.array([L1, L2], np.int32), (B, 1))
+    ln[0] = [L1 - 41, L2 - 3]
+    ln[1] = [0, L2 - 7]
+    print(f"sv gpu tangent B={B} {L1}x{L2}", flush=True)
+    hv, sens, con = _fd_second_order(s.astype(np.float64), ln, go, ge, T, V, G, h=1e-4)
+    for kw, truth in ((dict(direction=V), hv), (dict(d=(1.0, 0.0, 0.0)), sens[0]),
+                      (dict(d=(0.0, 1.0, 0.0)), sens[1]), (dict(d=(0.0, 0.0, 1.0)), sens[2])):
+        g = _tangent(s, ln, go, ge, T, device="cuda", **kw)[0]
+        assert abs_err(g, truth) < 2e-4 * max(1.0, float(np.abs(truth).max()))
+    gm, gc = _tangent(s, ln, go, ge, T, device="cuda", direction=G, contract=True)
+    assert _grad_err(gc, con) < 2e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape,ln,go,ge,T", SECOND)
+def test_gpu_second_order_matches_finite_differences(shape, ln, go, ge, T):
+    rng = np.random.default_rng(shape[1])
+    s, V, G = (rng.standard_normal(shape) for _ in range(3))
+    ln = np.array(ln, np.int32)
+    s32 = s.astype(np.float32)
+    hv, sens, con = _fd_second_order(s32.astype(np.float64), ln, go, ge, T, V, G)
+    print(f"sv gpu tangent vs fd {shape}", flush=True)
+    assert abs_err(_tangent(s32, ln, go, ge, T, device="cuda", direction=V)[0], hv) < 1e-5
+    for p in range(3):
+        d = [0.0, 0.0, 0.0]
+        d[p] = 1.0
+        assert abs_err(_tangent(s32, ln, go, ge, T, device="cuda", d=d)[0], sens[p]) < 1e-5
+    _, c = _tangent(s32, ln, go, ge, T, device="cuda", direction=G, contract=True)
+    assert _grad_err(c, con) < 1e-4
+
+
+@pytest.mark.gpu
+def test_gpu_sv_map_backward_runs_on_the_hip_kernels():
+    rng = np.random.default_rng(8)
+    s = rng.standard_normal((4, 90, 80)).astype(np.float32)
+    G = rng.standard_normal((4, 90, 80)).astype(np.float32)
+    x = torch.from_numpy(s).cuda().requires_grad_()
+    mp = ohm.sv_affine(x, gap_open_score=-1.0, gap_extend_score=-0.3, temperature=0.6)
+    (mp * torch.from_numpy(G).cuda()).sum().backward()
+    h = _tangent(s, np.array([[90, 80]] * 4, np.int32), -1.0, -0.3, 0.6, direction=G)[0]
+    assert abs_err(x.grad.cpu().numpy(), h) < 1e-4
