@@ -395,3 +395,63 @@ def sv_forward(scores, gap, temp, lengths=None, impl="oracle", prec="f32"):
     """(value[B], map[B,L1,L2], grad_gap[B], grad_temp[B]) of the linear-gap
     Saigo-Vert alignment."""
     return _linear_forward("sv", scores, gap, temp, lengths, impl, prec)
+
+
+# Second order of the three-state global / Saigo-Vert models: the reference's
+# own kernels only (no restatement; the tests pin them, and the library,
+# against central differences of the FP64 restatement above).  Composites as
+# src/{nw,sv}_affine/torch_cpu.cpp: hvp = forward + HVP pass; the map's
+# parameter sensitivity = forward + backward (dS/dtheta) + parameter pass.
+
+def _ref_second_order_setup(family, scores, go, ge, temp, lengths):
+    lib = _ref_lib(f"{family}_affine")
+    s = _arr(scores, np.float32)
+    ln = _lens(s, lengths)
+    B, L1, L2 = s.shape
+    C = ctypes.c_float
+    asz = 3 * (L1 + 1) * (L2 + 1)
+    alpha = np.zeros((B, asz), np.float32)
+    part = np.zeros(B, np.float32)
+    getattr(lib, f"{family}_affine_forward_cpu")(_ptr(s), _ptr(alpha), _ptr(part), _ptr(ln), B,
+                                                 L1, L2, C(go), C(ge), C(temp))
+    return lib, s, ln, B, L1, L2, alpha, part, asz
+
+
+def three_state_hvp_ref(family, scores, V, gap_open, gap_ext, temp, lengths=None):
+    """H.V of the reference's ``{family}_affine_hvp_cpu`` (family "nw" or "sv")."""
+    C = ctypes.c_float
+    lib, s, ln, B, L1, L2, alpha, part, asz = _ref_second_order_setup(
+        family, scores, gap_open, gap_ext, temp, lengths)
+    v = _arr(V, np.float32)
+    ws = [np.zeros((B, asz), np.float32) for _ in range(3)]
+    dpart = np.zeros(B, np.float32)
+    H = np.zeros((B, L1, L2), np.float32)
+    if family == "nw":
+        lib.nw_affine_hvp_cpu(_ptr(alpha), _ptr(s), _ptr(part), _ptr(v), _ptr(ws[0]), _ptr(dpart),
+                              _ptr(ws[1]), _ptr(ws[2]), _ptr(H), _ptr(ln), B, L1, L2,
+                              C(gap_open), C(gap_ext), C(temp))
+    else:
+        lib.sv_affine_hvp_cpu(_ptr(alpha), _ptr(s), _ptr(part), _ptr(v), _ptr(ws[0]), _ptr(dpart),
+                              _ptr(ws[1]), _ptr(H), _ptr(ln), B, L1, L2, C(gap_open), C(gap_ext),
+                              C(temp))
+    return H
+
+
+def three_state_param_jacobian_ref(family, scores, ptype, gap_open, gap_ext, temp, lengths=None):
+    """d(map)/d(theta) of the reference's ``{family}_affine_param_grad_cpu``
+    (ptype 0 gap_open, 1 gap_ext, 2 temperature)."""
+    C = ctypes.c_float
+    lib, s, ln, B, L1, L2, alpha, part, asz = _ref_second_order_setup(
+        family, scores, gap_open, gap_ext, temp, lengths)
+    beta = np.zeros((B, asz), np.float32)
+    post = np.zeros((B, L1, L2), np.float32)
+    g = [np.zeros(B, np.float32) for _ in range(3)]
+    getattr(lib, f"{family}_affine_backward_cpu")(
+        _ptr(alpha), _ptr(s), _ptr(part), _ptr(beta), _ptr(post), _ptr(g[0]), _ptr(g[1]),
+        _ptr(g[2]), _ptr(ln), B, L1, L2, C(gap_open), C(gap_ext), C(temp))
+    ws = [np.zeros((B, asz), np.float32) for _ in range(3)]
+    dP = np.zeros((B, L1, L2), np.float32)
+    getattr(lib, f"{family}_affine_param_grad_cpu")(
+        _ptr(alpha), _ptr(s), _ptr(part), _ptr(g[ptype]), _ptr(ws[0]), _ptr(ws[1]), _ptr(ws[2]),
+        _ptr(dP), _ptr(ln), B, L1, L2, C(gap_open), C(gap_ext), C(temp), int(ptype))
+    return dP

@@ -100,6 +100,15 @@ def nw_affine_case(name, s, go, ge, T, lengths=None, family="nw"):
     fwd = O.nw_affine_forward if family == "nw" else O.sv_affine_forward
     v, m, go_, ge_, gt = fwd(s, go, ge, T, lengths, impl="ref")
     d.update(value=v, map=m, grad_open=go_, grad_ext=ge_, grad_temp=gt)
+    if B * L1 * L2 <= 8192:
+        # second order: the reference's HVP along V (N(0,1), seed 7 + B) and
+        # its parameter sensitivities of the map
+        V = torch.randn((B, L1, L2), generator=torch.Generator().manual_seed(7 + B),
+                        dtype=torch.float32).numpy()
+        d["V"] = V
+        d["hvp"] = O.three_state_hvp_ref(family, s, V, go, ge, T, lengths)
+        for p, n in enumerate(("gap_open", "gap_ext", "temp")):
+            d[f"sens_{n}"] = O.three_state_param_jacobian_ref(family, s, p, go, ge, T, lengths)
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **d)
     return d
 

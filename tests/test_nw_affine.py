@@ -439,3 +439,42 @@ def test_gpu_nw_map_backward_runs_on_the_hip_kernels():
     (mp * torch.from_numpy(G).cuda()).sum().backward()
     h = _tangent(s, np.array([[90, 80]] * 4, np.int32), -1.0, -0.3, 0.6, direction=G)[0]
     assert abs_err(x.grad.cpu().numpy(), h) < 1e-4
+
+
+SECOND_GOLDEN = [p for p in NW if "hvp" in load(p)]
+
+
+def _second_order_close(got, ref, truth, tol):
+    """Within tol of the reference's FP32 output, or at most twice as far
+    from the FP64 central-difference value as that output is."""
+    if abs_err(got, ref) < tol:
+        return True
+    t = truth()
+    return abs_err(got, t) <= max(tol, 2.0 * abs_err(ref, t))
+
+
+def _check_second_order_golden(d, device):
+    go, ge, T = _params(d)
+    s, ln, V = d["scores"], d["lengths"], d["V"]
+    f64 = lambda: _fd_second_order(s.astype(np.float64), ln, go, ge, T, V, V)  # noqa: E731
+    tol = lambda r: 1e-4 * max(1.0, float(np.abs(r).max()))  # noqa: E731
+    got = _tangent(s, ln, go, ge, T, device=device, direction=V)[0]
+    assert _second_order_close(got, d["hvp"], lambda: f64()[0], tol(d["hvp"]))
+    for p, n in enumerate(("gap_open", "gap_ext", "temp")):
+        dd = [0.0, 0.0, 0.0]
+        dd[p] = 1.0
+        got = _tangent(s, ln, go, ge, T, device=device, d=dd)[0]
+        ref = d[f"sens_{n}"]
+        assert _second_order_close(got, ref, lambda p=p: f64()[1][p], tol(ref)), n
+
+
+@pytest.mark.parametrize("path", SECOND_GOLDEN, ids=lambda p: p.split("/")[-1])
+def test_host_second_order_matches_reference_golden(path):
+    _check_second_order_golden(load(path), "cpu")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", SECOND_GOLDEN, ids=lambda p: p.split("/")[-1])
+def test_gpu_second_order_matches_reference_golden(path):
+    print(f"nw gpu second order golden {path.split('/')[-1]}", flush=True)
+    _check_second_order_golden(load(path), "cuda")
