@@ -302,12 +302,17 @@ def kernel_durations(torch, lib, hip, launch, steps):
 # --------------------------------------------------------------------------
 
 
-def cpu_baseline(torch, pairs, params, seconds):
+def cpu_baseline(torch, pairs, params, seconds, family="sw"):
+    """The reference's CPU kernels (oracle/_ref; the restatement when they are
+    not built) on the first pairs of the batch, about `seconds` of work.
+    family: "sw" (Smith-Waterman affine), "nw" or "sv" (affine)."""
     import numpy as np
 
     from oracle import oracle as O
 
-    kind = "reference" if O.ref_available() else "port"
+    run = {"sw": O.sw_affine_forward, "nw": O.nw_affine_forward, "sv": O.sv_affine_forward}[family]
+    built = {"sw": O.ref_available, "nw": O.nw_ref_available, "sv": O.sv_ref_available}[family]
+    kind = "reference" if built() else "port"
     threads = min(16, os.cpu_count() or 1) if kind == "reference" else 1
     old = torch.get_num_threads()
     torch.set_num_threads(threads)
@@ -326,7 +331,7 @@ def cpu_baseline(torch, pairs, params, seconds):
                 pad[i, :p.shape[0], :p.shape[1]] = p
                 ln[i] = p.shape
             t0 = time.perf_counter()
-            O.sw_affine_forward(pad, go, ge, T, ln, impl="ref" if kind == "reference" else "oracle")
+            run(pad, go, ge, T, ln, impl="ref" if kind == "reference" else "oracle")
             elapsed += time.perf_counter() - t0
             cells += int((ln[:, 0].astype(np.int64) * ln[:, 1]).sum())
             done += len(part)
@@ -475,6 +480,29 @@ def nw_workload(torch, args, dev, hip, family="nw"):
                      "bytes_per_cell": int(byts[dom] / cells)},
         "cpu_baseline": None,
     }
+    # the HVP along an N(0,1) direction (forward-over-reverse pass) on the same batch
+    tan = _lib.nw_affine_tangent if family == "nw" else _lib.sv_affine_tangent
+    wsf = _lib._ws_nw_affine if family == "nw" else _lib._ws_sv_affine
+    wst = torch.empty(int(wsf(2, B, L1max, L2max, flat.numel())), dtype=torch.uint8, device=dev)
+    V = torch.randn(flat.shape, generator=torch.Generator().manual_seed(5)).to(dev)
+    mt = torch.empty_like(flat)
+
+    def hvp():
+        tan(flat, lengths, offsets, *params, direction=V, map_tangent=mt, L1max=L1max,
+            L2max=L2max, workspace=wst)
+
+    hvp()
+    torch.cuda.synchronize()
+    hms, _ = timed(torch, hvp, max(3, args.steps // 2))
+    del wst
+    log(f"{family} affine HVP: {hms:.3f} ms, {cells / (hms * 1e-3) / 1e9:.1f} GCUPS")
+    line["config"]["hvp_ms"] = round(hms, 4)
+    if not args.no_cpu:
+        log("CPU baseline (about %.0f s)" % args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(torch, [p.numpy() for p in pairs], params,
+                                            args.cpu_seconds, family)
+        log(f"CPU reference: {line['cpu_baseline']['value']} GCUPS on "
+            f"{line['cpu_baseline']['cores']} threads")
     print(json.dumps(line), flush=True)
 
 
