@@ -16,7 +16,8 @@ __version__ = "0.1.0"
 from . import _ops as _ops  # registers torch.ops.orihime.* (loads libohm_sw.so)
 from .sw import sw, sw_affine, sw_affine_entropy, sw_affine_value, sw_entropy, sw_value
 from .nw import nw, nw_affine, nw_affine_entropy, nw_affine_value, nw_entropy, nw_value
-from .sv import sv, sv_affine, sv_affine_entropy, sv_affine_value, sv_entropy, sv_value
+from .sv import (sv, sv_affine, sv_affine_entropy, sv_affine_value, sv_entropy, sv_linear,
+                 sv_linear_entropy, sv_linear_value, sv_value)
 from . import nn as nn
 from . import raw as raw
 from . import ops as ops
@@ -42,6 +43,9 @@ __all__ = [
     "sv",
     "sv_value",
     "sv_entropy",
+    "sv_linear",
+    "sv_linear_value",
+    "sv_linear_entropy",
     "sv_affine",
     "sv_affine_value",
     "sv_affine_entropy",

@@ -786,9 +786,12 @@ sw_affine_marginals_grad_gap_ext = orihime.sw_affine_marginals_grad_gap_ext
 sw_affine_marginals_grad_temp = orihime.sw_affine_marginals_grad_temp
 sw_affine_value_only = orihime.sw_affine_value_only
 
-for _fam in ("nw_affine", "sv_affine", "nw", "sv_linear"):
+for _fam, _fields in (("nw_affine", ("gap_open", "gap_ext", "temp")),
+                      ("sv_affine", ("gap_open", "gap_ext", "temp")),
+                      ("nw", ("gap", "temp")), ("sv_linear", ("gap", "temp"))):
     for _op in (f"{_fam}_forward", f"{_fam}_forward_t", f"{_fam}_value_grad_params",
-                f"{_fam}_value_only"):
+                f"{_fam}_value_only", f"{_fam}_marginals_hvp",
+                *(f"{_fam}_marginals_grad_{f}" for f in _fields)):
         torch.library.register_vmap(f"orihime::{_op}", _make_batch_folding_vmap_rule(_op), lib=_LIB)
 
 nw_affine_forward = orihime.nw_affine_forward

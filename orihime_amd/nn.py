@@ -1,4 +1,5 @@
-"""Stateful SW modules with learnable parameters (orihime/nn.py:51-265)."""
+"""Stateful alignment modules with learnable parameters (orihime/nn.py:51-390):
+Smith-Waterman, Saigo-Vert and Needleman-Wunsch, linear and affine gaps."""
 
 from __future__ import annotations
 
@@ -9,6 +10,8 @@ import torch
 from torch import Tensor, nn
 
 from .operator import _validate_scalar_parameter_shape, _validate_temperature
+from .nw import nw, nw_affine, nw_affine_entropy, nw_affine_value, nw_entropy, nw_value
+from .sv import sv, sv_affine, sv_affine_entropy, sv_affine_value, sv_entropy, sv_value
 from .sw import sw, sw_affine, sw_affine_entropy, sw_affine_value, sw_entropy, sw_value
 
 ParamValue = float | Tensor
@@ -123,4 +126,63 @@ class SmithWatermanAffine(_OperatorModule):
                          dtype=dtype)
 
 
-__all__ = ["SmithWaterman", "SmithWatermanAffine"]
+class _Linear(_OperatorModule):
+    _param_names = ("gap_score", "temperature")
+    _functions: ClassVar[tuple] = ()
+
+    def __init__(self, *, gap_score: float | Tensor = 0.0, temperature: float | Tensor = 1.0,
+                 learnable: tuple[str, ...] = (), device=None, dtype=None) -> None:
+        m, v, e = self._functions
+        super().__init__(params={"gap_score": gap_score, "temperature": temperature},
+                         map_function=m, value_function=v, entropy_function=e,
+                         learnable=learnable, device=device, dtype=dtype)
+
+
+class _Affine(_OperatorModule):
+    _param_names = ("gap_open_score", "gap_extend_score", "temperature")
+    _functions: ClassVar[tuple] = ()
+
+    def __init__(self, *, gap_open_score: float | Tensor = 0.0,
+                 gap_extend_score: float | Tensor = 0.0, temperature: float | Tensor = 1.0,
+                 learnable: tuple[str, ...] = (), device=None, dtype=None) -> None:
+        m, v, e = self._functions
+        super().__init__(params={"gap_open_score": gap_open_score,
+                                 "gap_extend_score": gap_extend_score, "temperature": temperature},
+                         map_function=m, value_function=v, entropy_function=e,
+                         learnable=learnable, device=device, dtype=dtype)
+
+
+class SaigoVertLinear(_Linear):
+    """Canonical linear-gap Saigo-Vert layer."""
+
+    _functions = (staticmethod(sv), staticmethod(sv_value), staticmethod(sv_entropy))
+
+
+class SaigoVertAffine(_Affine):
+    """Canonical affine-gap Saigo-Vert layer."""
+
+    _functions = (staticmethod(sv_affine), staticmethod(sv_affine_value),
+                  staticmethod(sv_affine_entropy))
+
+
+class NeedlemanWunsch(_Linear):
+    """Score-native soft Needleman-Wunsch global alignment."""
+
+    _functions = (staticmethod(nw), staticmethod(nw_value), staticmethod(nw_entropy))
+
+
+class NeedlemanWunschAffine(_Affine):
+    """Stateful affine Needleman-Wunsch layer."""
+
+    _functions = (staticmethod(nw_affine), staticmethod(nw_affine_value),
+                  staticmethod(nw_affine_entropy))
+
+
+SVLinear = SaigoVertLinear
+SVAffine = SaigoVertAffine
+NW = NeedlemanWunsch
+NWAffine = NeedlemanWunschAffine
+
+__all__ = ["SmithWaterman", "SmithWatermanAffine", "SaigoVertLinear", "SVLinear",
+           "SaigoVertAffine", "SVAffine", "NeedlemanWunsch", "NW", "NeedlemanWunschAffine",
+           "NWAffine"]

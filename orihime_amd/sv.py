@@ -114,5 +114,10 @@ def sv_affine_entropy(pair_scores: Tensor, *, gap_open_score: float | Tensor = 0
                                        temperature, lengths=lengths, mask=mask, dtype=dtype)
 
 
-__all__ = ["sv", "sv_value", "sv_entropy", "sv_affine", "sv_affine_value",
-           "sv_affine_entropy"]
+# The reference keeps the native family name as aliases (orihime/sv.py:117-122).
+sv_linear = sv
+sv_linear_value = sv_value
+sv_linear_entropy = sv_entropy
+
+__all__ = ["sv", "sv_value", "sv_entropy", "sv_linear", "sv_linear_value", "sv_linear_entropy",
+           "sv_affine", "sv_affine_value", "sv_affine_entropy"]

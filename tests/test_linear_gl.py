@@ -170,3 +170,27 @@ def test_gpu_linear_equals_affine_call_with_one_gap(fam):
     ref = getattr(O, f"{fam}_forward")(s, -1.2, 0.5, ln, prec="f64")
     assert _err(lin[0], ref[0]) < 1e-5 and abs_err(lin[1], ref[1]) < 1e-4
     assert _err(lin[2], ref[2]) < 1e-4
+
+
+def test_nn_modules_and_vmap_cover_the_nw_and_sv_families():
+    from orihime_amd import nn as onn
+
+    rng = np.random.default_rng(6)
+    x = torch.from_numpy(rng.standard_normal((2, 6, 7)).astype(np.float32))
+    for cls, fam, kw in ((onn.NW, "nw", {"gap_score": -0.5}),
+                         (onn.SVLinear, "sv", {"gap_score": -0.5}),
+                         (onn.NWAffine, "nw_affine", {"gap_open_score": -1.0, "gap_extend_score": -0.3}),
+                         (onn.SVAffine, "sv_affine", {"gap_open_score": -1.0, "gap_extend_score": -0.3})):
+        m = cls(**kw, temperature=0.7, learnable=("temperature",))
+        out = m(x)
+        ref = getattr(ohm, fam)(x, **kw, temperature=0.7)
+        assert torch.allclose(out, ref)
+        out.sum().backward()
+        assert m.temperature.grad is not None and torch.isfinite(m.temperature.grad).all()
+    v = torch.from_numpy(rng.standard_normal((3, 2, 6, 7)).astype(np.float32))
+    batched = torch.vmap(lambda vv: torch.ops.orihime.nw_affine_marginals_hvp(
+        x, vv, -1.0, -0.3, 0.7, None))(v)
+    for k in range(3):
+        one = torch.ops.orihime.nw_affine_marginals_hvp(x, v[k], -1.0, -0.3, 0.7, None)
+        assert torch.allclose(batched[k], one, atol=1e-6)
+    assert ohm.sv_linear is ohm.sv
