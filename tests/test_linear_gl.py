@@ -194,3 +194,28 @@ def test_nn_modules_and_vmap_cover_the_nw_and_sv_families():
         one = torch.ops.orihime.nw_affine_marginals_hvp(x, v[k], -1.0, -0.3, 0.7, None)
         assert torch.allclose(batched[k], one, atol=1e-6)
     assert ohm.sv_linear is ohm.sv
+
+
+def test_raw_tier_vjps_for_the_nw_and_sv_families():
+    rng = np.random.default_rng(9)
+    x = torch.from_numpy(rng.standard_normal((2, 5, 6)).astype(np.float32))
+    G = torch.from_numpy(rng.standard_normal((2, 5, 6)).astype(np.float32))
+    x64 = x.numpy().astype(np.float64)
+    h = 1e-5
+    for fam in ("nw_affine", "sv_affine"):
+        out = getattr(ohm.raw, fam).vjp(x, wrt=("gap_open_score", "temperature"), cotangent=G,
+                                        gap_open_score=-1.0, gap_extend_score=-0.3,
+                                        temperature=0.7)
+        f = getattr(O, f"{fam}_forward")
+        fd_go = ((f(x64, -1.0 + h, -0.3, 0.7, prec="f64")[1] -
+                  f(x64, -1.0 - h, -0.3, 0.7, prec="f64")[1]) / (2 * h) * G.numpy()).sum()
+        fd_t = ((f(x64, -1.0, -0.3, 0.7 + h, prec="f64")[1] -
+                 f(x64, -1.0, -0.3, 0.7 - h, prec="f64")[1]) / (2 * h) * G.numpy()).sum()
+        assert abs(out["gap_open_score"].item() - fd_go) < 1e-4 * max(1.0, abs(fd_go))
+        assert abs(out["temperature"].item() - fd_t) < 1e-4 * max(1.0, abs(fd_t))
+    for fam, f in (("nw", O.nw_forward), ("sv_linear", O.sv_forward)):
+        got = getattr(ohm.raw, fam).vjp_one(x, wrt="gap_score", cotangent=G, gap_score=-0.5,
+                                            temperature=0.7)
+        fd = ((f(x64, -0.5 + h, 0.7, prec="f64")[1] - f(x64, -0.5 - h, 0.7, prec="f64")[1]) /
+              (2 * h) * G.numpy()).sum()
+        assert abs(got.item() - fd) < 1e-4 * max(1.0, abs(fd))
