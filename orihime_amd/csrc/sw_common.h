@@ -138,7 +138,8 @@ inline WsLayout ws_layout(int op, int64_t B, int64_t L1max, int64_t L2max, int64
     w.data = o;
     const TableSpec none = {0, 1};
     const TableSpec t = op == 0 ? none : (op == 1 ? fwd : tan);
-    int64_t elems = table_bound(N, B, L1max, L2max, t) + bnd_bound(N, B, L2max);
+    // + 64 floats per pair: each pair's region is rounded up to 256 bytes
+    int64_t elems = table_bound(N, B, L1max, L2max, t) + bnd_bound(N, B, L2max) + 64 * B;
     w.total = align256(o + (size_t)elems * sizeof(float));
     return w;
 }

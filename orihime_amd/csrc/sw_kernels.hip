@@ -395,7 +395,10 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
             ts = (int64_t)ns * ((L2 + 63 + tstride - 1) / tstride) * kWave * planes;
             bs = (int64_t)ns * L2 * kBndStride;
         }
-        s_t[tid] = ts + bs;
+        // each pair's region starts on a 256-byte boundary, so the 256-byte
+        // table rows of its strips fill whole cache lines
+        const int64_t tot = (ts + bs + 63) / 64 * 64;
+        s_t[tid] = tot;
         __syncthreads();
         for (int d = 1; d < kPlanThreads; d <<= 1) {
             int64_t at = tid >= d ? s_t[tid - d] : 0;
@@ -405,7 +408,7 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
         }
         if (b < B) {
             PairDesc pd;
-            pd.t_off = s_carry[0] + s_t[tid] - (ts + bs);
+            pd.t_off = s_carry[0] + s_t[tid] - tot;
             pd.b_off = pd.t_off + ts;
             pd.s_off = offsets ? offsets[b] : (int64_t)b * L1max * L2max;
             pd.ld = offsets ? L2 : L2max;
