@@ -1223,6 +1223,15 @@ struct Launch {
     hipStream_t stream;
 };
 
+// Operations for setup(): which table the plan kernel lays out.
+constexpr int kOpValue = 0;      // value only (no table)
+constexpr int kOpForward = 1;    // Smith-Waterman forward + reverse
+constexpr int kOpTangent = 2;    // Smith-Waterman tangent passes
+constexpr int kOpNw = 3;         // Needleman-Wunsch forward + reverse
+constexpr int kOpSv = 4;         // Saigo-Vert forward + reverse
+constexpr int kOpNwTangent = 5;  // Needleman-Wunsch tangent passes
+constexpr int kOpSvTangent = 6;  // Saigo-Vert tangent passes
+
 ohm_status setup(int op, bool affine, const float* scores, const int32_t* lengths,
                  const int64_t* offsets, int B, int L1max, int L2max, float go, float ge,
                  float temp, void* ws, size_t ws_bytes, void* stream, Launch& L) {
@@ -1234,15 +1243,14 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     if (offsets && !lengths) return fail(OHM_EINVAL, "the packed layout needs lengths");
     if ((int64_t)((L1max + kStrip - 1) / kStrip) * 4 > 60 * 1024)
         return fail(OHM_EINVAL, "L1 too long for one launch (max 983040 rows)");
-    // op 3 / 4: the Needleman-Wunsch / Saigo-Vert forward + reverse (full tables)
-    // op 5: the Needleman-Wunsch tangent passes
-    const TableSpec tf = op == 3 ? TableSpec{kPlanesNw, 1}
-                         : op == 4 ? TableSpec{kPlanesSv, 1}
-                         : op == 5 ? TableSpec{kPlanesNwTan, 1}
-                         : op == 6 ? TableSpec{kPlanesSvTan, 1}
+    // the Needleman-Wunsch and Saigo-Vert passes keep full tables
+    const TableSpec tf = op == kOpNw ? TableSpec{kPlanesNw, 1}
+                         : op == kOpSv ? TableSpec{kPlanesSv, 1}
+                         : op == kOpNwTangent ? TableSpec{kPlanesNwTan, 1}
+                         : op == kOpSvTangent ? TableSpec{kPlanesSvTan, 1}
                          : affine ? TableSpec{kPlanesAffineCkpt, kCkpt} : TableSpec{kPlanesLinearFwd, 1};
     const TableSpec tt = {affine ? kPlanesAffineTan : kPlanesLinearTan, 1};
-    WsLayout w = ws_layout(op >= 3 ? 1 : op, B, L1max, L2max, -1, tf, tt);
+    WsLayout w = ws_layout(op >= kOpNw ? kOpForward : op, B, L1max, L2max, -1, tf, tt);
     if (!ws) return fail(OHM_EWORKSPACE, "workspace is null");
     if (ws_bytes < w.data + 256) return fail(OHM_EWORKSPACE, "workspace too small");
     if (!offsets && ws_bytes < w.total)
@@ -1276,7 +1284,7 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     L.geo = choose_geometry(B, L1max);
     L.lds = (size_t)a.max_strips * sizeof(int);
     L.stream = static_cast<hipStream_t>(stream);
-    const TableSpec ts = op == 0 ? TableSpec{0, 1} : (op == 2 ? tt : tf);
+    const TableSpec ts = op == kOpValue ? TableSpec{0, 1} : (op == kOpTangent ? tt : tf);
     const int64_t cap = (int64_t)((ws_bytes - w.data) / sizeof(float));
     if (B <= kGridMaxPairs) a.grid = reinterpret_cast<int*>(base + w.grid);
     hipLaunchKernelGGL(sw_plan_kernel, dim3(1), dim3(kPlanThreads), 0, L.stream, lengths, offsets,
@@ -1385,7 +1393,7 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     if (B > 0 && !value) return fail(OHM_EINVAL, "value is null");
     const bool need_bwd = map || g0 || g1 || g2;
     Launch L;
-    ohm_status st = setup(need_bwd ? 1 : 0, affine, scores, lengths, offsets, B, L1max, L2max, go,
+    ohm_status st = setup(need_bwd ? kOpForward : kOpValue, affine, scores, lengths, offsets, B, L1max, L2max, go,
                           ge, temp, ws, ws_bytes, stream, L);
     if (st != OHM_OK || B == 0) return st;
     L.a.value = value;
@@ -1471,7 +1479,7 @@ ohm_status nw_forward_impl(const float* scores, const int32_t* lengths, const in
     const bool need_bwd = map || g0 || g1 || g2;
     if (need_bwd && !map) return fail(OHM_EINVAL, "the gradients need the map buffer");
     Launch L;
-    ohm_status st = setup(need_bwd ? 3 : 0, true, scores, lengths, offsets, B, L1max, L2max, go,
+    ohm_status st = setup(need_bwd ? kOpNw : kOpValue, true, scores, lengths, offsets, B, L1max, L2max, go,
                           ge, temp, ws, ws_bytes, stream, L);
     if (st != OHM_OK || B == 0) return st;
     L.a.value = value;
@@ -1502,7 +1510,7 @@ ohm_status nw_tangent_impl(const float* scores, const int32_t* lengths, const in
     if (contract && (dgo != 0.0f || dge != 0.0f || dT != 0.0f))
         return fail(OHM_EINVAL, "contract needs a pure score direction");
     Launch L;
-    ohm_status st = setup(5, true, scores, lengths, offsets, B, L1max, L2max, go, ge, temp, ws,
+    ohm_status st = setup(kOpNwTangent, true, scores, lengths, offsets, B, L1max, L2max, go, ge, temp, ws,
                           ws_bytes, stream, L);
     if (st != OHM_OK || B == 0) return st;
     L.a.dir = dir;
@@ -1534,7 +1542,7 @@ ohm_status sv_tangent_impl(const float* scores, const int32_t* lengths, const in
     if (contract && (dgo != 0.0f || dge != 0.0f || dT != 0.0f))
         return fail(OHM_EINVAL, "contract needs a pure score direction");
     Launch L;
-    ohm_status st = setup(6, true, scores, lengths, offsets, B, L1max, L2max, go, ge, temp, ws,
+    ohm_status st = setup(kOpSvTangent, true, scores, lengths, offsets, B, L1max, L2max, go, ge, temp, ws,
                           ws_bytes, stream, L);
     if (st != OHM_OK || B == 0) return st;
     L.a.dir = dir;
@@ -1565,7 +1573,7 @@ ohm_status sv_forward_impl(const float* scores, const int32_t* lengths, const in
     const bool need_bwd = map || g0 || g1 || g2;
     if (need_bwd && !map) return fail(OHM_EINVAL, "the gradients need the map buffer");
     Launch L;
-    ohm_status st = setup(need_bwd ? 4 : 0, true, scores, lengths, offsets, B, L1max, L2max, go,
+    ohm_status st = setup(need_bwd ? kOpSv : kOpValue, true, scores, lengths, offsets, B, L1max, L2max, go,
                           ge, temp, ws, ws_bytes, stream, L);
     if (st != OHM_OK || B == 0) return st;
     L.a.value = value;
@@ -1595,7 +1603,7 @@ ohm_status tangent_impl(bool affine, const float* scores, const int32_t* lengths
     if (contract && (dgo != 0.0f || dge != 0.0f || dT != 0.0f))
         return fail(OHM_EINVAL, "contract needs a pure score direction");
     Launch L;
-    ohm_status st = setup(2, affine, scores, lengths, offsets, B, L1max, L2max, go, ge, temp, ws,
+    ohm_status st = setup(kOpTangent, affine, scores, lengths, offsets, B, L1max, L2max, go, ge, temp, ws,
                           ws_bytes, stream, L);
     if (st != OHM_OK || B == 0) return st;
     L.a.dir = dir;
