@@ -219,3 +219,20 @@ def test_raw_tier_vjps_for_the_nw_and_sv_families():
         fd = ((f(x64, -0.5 + h, 0.7, prec="f64")[1] - f(x64, -0.5 - h, 0.7, prec="f64")[1]) /
               (2 * h) * G.numpy()).sum()
         assert abs(got.item() - fd) < 1e-4 * max(1.0, abs(fd))
+
+
+def test_masks_compile_and_autocast_for_the_nw_and_sv_families():
+    rng = np.random.default_rng(13)
+    x = torch.from_numpy(rng.standard_normal((2, 6, 7)).astype(np.float32))
+    m = torch.zeros_like(x, dtype=torch.bool)
+    m[:, 2, 3] = True
+    kw_aff = dict(gap_open_score=-1.0, gap_extend_score=-0.3, temperature=0.7)
+    for f, kw in ((ohm.nw_affine, kw_aff), (ohm.sv_affine, kw_aff),
+                  (ohm.nw, dict(gap_score=-0.5, temperature=0.7)),
+                  (ohm.sv, dict(gap_score=-0.5, temperature=0.7))):
+        out = f(x, mask=m, **kw)
+        assert out[m].abs().max() == 0
+        compiled = torch.compile(lambda s, f=f, kw=kw: f(s, **kw), backend="aot_eager")
+        assert torch.allclose(compiled(x), f(x, **kw))
+    with torch.autocast("cpu", dtype=torch.bfloat16):
+        assert ohm.nw(x.bfloat16(), gap_score=-0.5).dtype == torch.float32
