@@ -224,10 +224,13 @@ DEV void load16(float (&dst)[16], const float* row, int c0, int len, bool ok) {
             dst[4 * q + 0] = v.x; dst[4 * q + 1] = v.y; dst[4 * q + 2] = v.z; dst[4 * q + 3] = v.w;
         }
     } else {
+        // one window base, element u at an immediate offset (a per-element
+        // 64-bit row index would hold 16 address pairs live)
+        const float* p = row + c0;
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             int c = c0 + u;
-            dst[u] = (ok && c >= 0 && c < len) ? row[c] : 0.0f;
+            dst[u] = (ok && c >= 0 && c < len) ? p[u] : 0.0f;
         }
     }
 }
@@ -240,10 +243,11 @@ DEV void store16(float* row, int c0, int len, bool ok, const float (&src)[16]) {
             *reinterpret_cast<f4u*>(row + c0 + 4 * q) = v;
         }
     } else {
+        float* p = row + c0;
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             int c = c0 + u;
-            if (ok && c >= 0 && c < len) row[c] = src[u];
+            if (ok && c >= 0 && c < len) p[u] = src[u];
         }
     }
 }
@@ -257,10 +261,11 @@ DEV void load8(float (&dst)[8], const float* row, int c0, int len, bool ok) {
             dst[4 * q + 0] = v.x; dst[4 * q + 1] = v.y; dst[4 * q + 2] = v.z; dst[4 * q + 3] = v.w;
         }
     } else {
+        const float* p = row + c0;
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             int c = c0 + u;
-            dst[u] = (ok && c >= 0 && c < len) ? row[c] : 0.0f;
+            dst[u] = (ok && c >= 0 && c < len) ? p[u] : 0.0f;
         }
     }
 }
@@ -273,10 +278,11 @@ DEV void store8(float* row, int c0, int len, bool ok, const float (&src)[8]) {
             *reinterpret_cast<f4u*>(row + c0 + 4 * q) = v;
         }
     } else {
+        float* p = row + c0;
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             int c = c0 + u;
-            if (ok && c >= 0 && c < len) row[c] = src[u];
+            if (ok && c >= 0 && c < len) p[u] = src[u];
         }
     }
 }
