@@ -83,6 +83,88 @@ def _check_lengths(lengths: Optional[Tensor], scores: Tensor) -> Optional[Tensor
     return lengths
 
 
+# Shape guards of the reference bindings.  The reference keeps (L1+1) x (L2+1)
+# DP tables (three planes for the affine models) indexed with int32 and
+# refuses, before allocating anything, shapes whose table would overflow that
+# index (src/sw_affine/torch_cpu.cpp:39-62, torch_hip.cpp:44-67;
+# src/sw/torch_cpu.cpp:38-62, torch_hip.cpp:40-66; src/nw/torch_cpu.cpp:43-61;
+# src/nw_affine/torch_cpu.cpp:32-62, torch_hip.cpp:32-80;
+# src/sv_affine/torch_cpu.cpp:39-62, torch_hip.cpp:36-66;
+# src/sv_linear/torch_cpu.cpp:33-60, torch_hip.cpp:35-62).  These sweeps keep
+# no such table, but a drop-in refuses the same shapes with the same messages,
+# so a caller sees the same contract.  Zero-storage tensors such as
+# (1, 0, 715827882) reach these checks without any allocation.
+_I32 = 2**31 - 1
+
+
+def _guard_shape(family: str, scores: Tensor) -> None:
+    B, L1, L2 = (int(x) for x in scores.shape)
+    rows, cols = L1 + 1, L2 + 1
+    cells = rows * cols
+    gpu = scores.device.type != "cpu"
+    for n, what in ((B, "batch"), (L1, "L1"), (L2, "L2")):
+        if n > _I32:
+            raise RuntimeError(f"scores {what} dimension is too large for {family}")
+    if family in ("sw_affine", "sv_affine"):
+        if gpu:
+            api = "CUDA" if family == "sw_affine" else "HIP"
+            if L1 > _I32 - L2:
+                raise RuntimeError(
+                    f"{family} dimensions too large: L1 + L2 exceeds {api} int index range")
+            if 3 * cells > _I32:
+                raise RuntimeError(f"{family} DP table too large: 3 * (L1 + 1) * (L2 + 1) "
+                                   f"exceeds {api} int index range")
+        elif 3 * cells > _I32:
+            raise RuntimeError(
+                f"{family} CPU DP table is too large: 3 * (L1 + 1) * (L2 + 1) = {3 * cells}")
+    elif family == "nw_affine":
+        if gpu:
+            if L1 > _I32 - L2:
+                raise RuntimeError(f"scores dimensions L1 + L2 must fit in int32, got {L1} + {L2}")
+            if 3 * cells > _I32:
+                raise RuntimeError(
+                    f"nw_affine alpha workspace size must fit in int32, got {3 * cells}")
+        elif 3 * cells > _I32:
+            raise RuntimeError(f"nw_affine CPU DP workspace is too large: 3 * (L1 + 1) * "
+                               f"(L2 + 1) = {3 * cells} exceeds {_I32}")
+    elif family == "sw":
+        if cells > _I32:
+            if gpu:
+                raise RuntimeError(f"SW CUDA DP table is too large for 32-bit kernel indexing: "
+                                   f"(L1 + 1) * (L2 + 1) = {cells} exceeds {_I32}")
+            raise RuntimeError(f"SW CPU DP table size exceeds supported int32 range: "
+                               f"(L1 + 1) * (L2 + 1) must be <= {_I32}, got {rows} * {cols}")
+    elif family == "sv_linear":
+        if cells > _I32 // 3:
+            if gpu:
+                raise RuntimeError(f"SV linear HIP DP table is too large for 32-bit kernel "
+                                   f"indexing: 3 * (L1 + 1) * (L2 + 1) = {3 * cells} exceeds {_I32}")
+            raise RuntimeError(f"SV linear CPU DP table size exceeds supported int32 range: "
+                               f"3 * (L1 + 1) * (L2 + 1) must be <= {_I32}, got {rows} * {cols}")
+    elif family == "nw" and not gpu and cells > _I32:
+        raise RuntimeError(f"NW alpha workspace size is too large: ({L1} + 1) * ({L2} + 1) "
+                           f"exceeds {_I32}")
+
+
+def _family_of(schema_name: str) -> str:
+    name = schema_name[5:] if schema_name.startswith("soft_") else schema_name
+    for fam in ("sw_affine", "nw_affine", "sv_affine", "sv_linear"):
+        if name.startswith(fam):
+            return fam
+    return "sw" if name.startswith("sw") else "nw"
+
+
+def _guarded(schema_name: str, impl):
+    family = _family_of(schema_name)
+
+    def run(scores, *args):
+        if isinstance(scores, Tensor) and scores.dim() == 3:
+            _guard_shape(family, scores)
+        return impl(scores, *args)
+
+    return run
+
+
 def _scalar(t: Tensor, name: str) -> float:
     if t.numel() != 1:
         raise RuntimeError(f"{name} must be a scalar tensor")
@@ -610,8 +692,8 @@ def _fake_for(name: str, schema: str):
 
 for _name, (_schema, _impl) in _SCHEMAS.items():
     _LIB.define(_name + _schema)
-    _LIB.impl(_name, _impl, "CPU")
-    _LIB.impl(_name, _impl, "CUDA")
+    _LIB.impl(_name, _guarded(_name, _impl), "CPU")
+    _LIB.impl(_name, _guarded(_name, _impl), "CUDA")
     torch.library.register_fake(f"orihime::{_name}", _fake_for(_name, _schema), lib=_LIB)
 
 orihime = torch.ops.orihime

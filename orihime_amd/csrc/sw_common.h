@@ -49,6 +49,8 @@ struct PairDesc {
     int32_t ld;     // row stride of scores / map
     int32_t L1, L2; // active lengths
     int32_t ns;     // strips = ceil(L1 / 64) (0 when the pair has no cells)
+    int32_t lost;   // 1: the workspace had no room for the pair (it runs as an
+                    // empty pair and its outputs are overwritten with NaN)
 };
 
 // Floats per table step (one column of a strip) for each pass: the table of
@@ -99,6 +101,7 @@ constexpr int kClaimGridFwd = 10;   // strip tickets of the grid-scheduled sweep
 constexpr int kClaimGridRev = 11;
 constexpr int kClaimGridStall = 12;  // set when a grid hand-off wait exceeded its bound
 constexpr int kClaimFused = 13;      // pair claims of the fused forward + reverse launch
+constexpr int kClaimWsShort = 14;    // set by the plan kernel when some pair did not fit the workspace
 
 // Grid scheduling (strips of a pair on waves of any workgroup, hand-off
 // through global memory) for batches of at most kGridMaxPairs pairs.  Its

@@ -805,7 +805,9 @@ ohm_status check(const Problem& P) {
     if (P.B < 0 || P.L1max < 0 || P.L2max < 0) return cpu_fail(OHM_EINVAL, "negative batch size or length");
     if (!(P.T > 0.0f) || !std::isfinite(P.T))
         return cpu_fail(OHM_EINVAL, "temperature must be finite and strictly positive");
-    if (P.B > 0 && !P.scores) return cpu_fail(OHM_EINVAL, "scores is null");
+    // a batch with an empty dimension has no cells: its scores may be null
+    if (P.B > 0 && P.L1max > 0 && P.L2max > 0 && !P.scores)
+        return cpu_fail(OHM_EINVAL, "scores is null");
     if (P.offsets && !P.lengths) return cpu_fail(OHM_EINVAL, "the packed layout needs lengths");
     for (int b = 0; P.lengths && b < P.B; ++b) {
         int l1 = P.lengths[2 * b], l2 = P.lengths[2 * b + 1];
@@ -864,7 +866,8 @@ extern "C" ohm_status ohm_nw_affine_forward_cpu(const float* scores, const int32
     if (!std::isfinite(gap_open) || !std::isfinite(gap_ext))
         return cpu_fail(OHM_EINVAL, "gap scores must be finite");
     const bool bwd = map || grad_open || grad_ext || grad_temp;
-    if (bwd && !map) return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
+    if (bwd && !map && L1max > 0 && L2max > 0)
+        return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
     parallel_pairs(B, threads, [&](int b) { nw_pair(P, b, bwd); });
     return OHM_OK;
 }
@@ -882,7 +885,8 @@ extern "C" ohm_status ohm_sv_affine_forward_cpu(const float* scores, const int32
     if (!std::isfinite(gap_open) || !std::isfinite(gap_ext))
         return cpu_fail(OHM_EINVAL, "gap scores must be finite");
     const bool bwd = map || grad_open || grad_ext || grad_temp;
-    if (bwd && !map) return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
+    if (bwd && !map && L1max > 0 && L2max > 0)
+        return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
     parallel_pairs(B, threads, [&](int b) { sv_pair(P, b, bwd); });
     return OHM_OK;
 }
@@ -898,7 +902,8 @@ extern "C" ohm_status ohm_nw_forward_cpu(const float* scores, const int32_t* len
     if (st != OHM_OK) return st;
     if (!std::isfinite(gap)) return cpu_fail(OHM_EINVAL, "gap scores must be finite");
     const bool bwd = map || grad_gap || grad_temp;
-    if (bwd && !map) return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
+    if (bwd && !map && L1max > 0 && L2max > 0)
+        return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
     parallel_pairs(B, threads, [&](int b) { nw_pair(P, b, bwd); });
     return OHM_OK;
 }
@@ -914,7 +919,8 @@ extern "C" ohm_status ohm_sv_forward_cpu(const float* scores, const int32_t* len
     if (st != OHM_OK) return st;
     if (!std::isfinite(gap)) return cpu_fail(OHM_EINVAL, "gap scores must be finite");
     const bool bwd = map || grad_gap || grad_temp;
-    if (bwd && !map) return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
+    if (bwd && !map && L1max > 0 && L2max > 0)
+        return cpu_fail(OHM_EINVAL, "the gradients need the map buffer");
     parallel_pairs(B, threads, [&](int b) { sv_pair(P, b, bwd); });
     return OHM_OK;
 }

@@ -373,8 +373,8 @@ constexpr int kSortMax = 2048;
 __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
     const int32_t* __restrict__ lengths, const int64_t* __restrict__ offsets, int B, int L1max,
     int L2max, int planes, int tstride, int64_t cap, PairDesc* __restrict__ desc,
-    int32_t* __restrict__ order, int* __restrict__ claims, float* __restrict__ value,
-    int* __restrict__ grid, int grid_ints) {
+    int32_t* __restrict__ order, int* __restrict__ claims, int* __restrict__ grid,
+    int grid_ints) {
     __shared__ int64_t s_t[kPlanThreads];
     __shared__ int64_t s_carry[1];
     __shared__ float s_key[kSortMax];
@@ -421,13 +421,17 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
             pd.L1 = L1;
             pd.L2 = L2;
             pd.ns = ns;
+            pd.lost = 0;
             if (pd.b_off + bs > cap) {
-                // Workspace sized for fewer cells than requested: skip the pair
-                // and make the failure visible in its value.
+                // Workspace sized for fewer cells than the lengths ask for (a
+                // packed-layout call): the pair runs as an empty pair and
+                // lost_pairs_kernel, launched at the end of the call, turns its
+                // outputs into NaN.  Every writer stores the same word.
                 pd.ns = 0;
                 pd.L1 = 0;
                 pd.L2 = 0;
-                if (value) value[b] = __builtin_bit_cast(float, 0x7fc00000u);  // quiet NaN
+                pd.lost = 1;
+                claims[kClaimWsShort] = 1;
             }
             desc[b] = pd;
             if (sort) {
@@ -484,6 +488,49 @@ DEV void zero_padding(float* out, const PairDesc& d, int L1max, int L2max, int t
     // rows [L1, L1max): everything
     for (int64_t q = tid; q < (int64_t)(L1max - L1) * L2max; q += nthr)
         out[base + (int64_t)L1 * L2max + q] = 0.0f;
+}
+
+// Outputs of one call, for lost_pairs_kernel.
+struct CallOut {
+    float* value;
+    float* g[3];
+    float* contract;  // [B * nc]
+    int nc;
+    float* map;
+    float* map_t;
+};
+
+// End of a packed-layout call whose workspace was sized for fewer cells than
+// the lengths ask for: the plan kernel ran the pairs that did not fit as empty
+// pairs and raised claims[kClaimWsShort]; this one-block launch overwrites
+// their value, gradients, contractions and every map / map-tangent cell with
+// NaN, so the failure cannot pass for an empty alignment.  It returns at once
+// when the word is clear.
+__global__ __launch_bounds__(256) void lost_pairs_kernel(const int* __restrict__ claims,
+                                                         const PairDesc* __restrict__ desc,
+                                                         const int32_t* __restrict__ lengths,
+                                                         const int64_t* __restrict__ offsets,
+                                                         int B, CallOut o) {
+    if (__builtin_amdgcn_readfirstlane(claims[kClaimWsShort]) == 0) return;
+    const float nan = __builtin_nanf("");
+    const int tid = tid_x();
+    for (int b = tid; b < B; b += 256) {
+        if (!desc[b].lost) continue;
+        if (o.value) o.value[b] = nan;
+        for (int q = 0; q < 3; ++q)
+            if (o.g[q]) o.g[q][b] = nan;
+        if (o.contract)
+            for (int q = 0; q < o.nc; ++q) o.contract[(int64_t)o.nc * b + q] = nan;
+    }
+    if (!o.map && !o.map_t) return;
+    for (int b = 0; b < B; ++b) {
+        if (!desc[b].lost) continue;
+        const int64_t n = (int64_t)max(lengths[2 * b], 0) * max(lengths[2 * b + 1], 0);
+        for (int64_t q = tid; q < n; q += 256) {
+            if (o.map) o.map[offsets[b] + q] = nan;
+            if (o.map_t) o.map_t[offsets[b] + q] = nan;
+        }
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -1164,6 +1211,22 @@ ohm_status check_launch(const char* what) {
     return OHM_OK;
 }
 
+// Tail of every packed-layout GPU call: lost_pairs_kernel on the claims and
+// descriptors that setup() laid out at the front of the workspace.  (Padded
+// calls need none: setup() requires the padded bound, so every pair fits.)
+ohm_status finish_lost(const int32_t* lengths, const int64_t* offsets, int B, int L1max,
+                       void* ws, const CallOut& o, void* stream) {
+    if (!offsets || B == 0) return OHM_OK;
+    size_t claims, desc, order, stil, tgt, pst, grid;
+    ws_fixed(B, L1max > 0 ? (L1max + kStrip - 1) / kStrip : 1, &claims, &desc, &order, &stil, &tgt,
+             &pst, &grid);
+    char* base = static_cast<char*>(ws);
+    hipLaunchKernelGGL(lost_pairs_kernel, dim3(1), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       reinterpret_cast<const int*>(base + claims),
+                       reinterpret_cast<const PairDesc*>(base + desc), lengths, offsets, B, o);
+    return check_launch("lost-pairs kernel");
+}
+
 int device_cus() {
     static std::mutex mu;
     static int cached[64] = {0};
@@ -1245,7 +1308,8 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     if (!(temp > 0.0f) || !std::isfinite(temp))
         return fail(OHM_EINVAL, "temperature must be finite and strictly positive");
     if (!std::isfinite(go) || !std::isfinite(ge)) return fail(OHM_EINVAL, "gap scores must be finite");
-    if (B > 0 && !scores) return fail(OHM_EINVAL, "scores is null");
+    // a batch with an empty dimension has no cells: its scores may be null
+    if (B > 0 && L1max > 0 && L2max > 0 && !scores) return fail(OHM_EINVAL, "scores is null");
     if (offsets && !lengths) return fail(OHM_EINVAL, "the packed layout needs lengths");
     if ((int64_t)((L1max + kStrip - 1) / kStrip) * 4 > 60 * 1024)
         return fail(OHM_EINVAL, "L1 too long for one launch (max 983040 rows)");
@@ -1295,7 +1359,7 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     if (B <= kGridMaxPairs) a.grid = reinterpret_cast<int*>(base + w.grid);
     hipLaunchKernelGGL(sw_plan_kernel, dim3(1), dim3(kPlanThreads), 0, L.stream, lengths, offsets,
                        B, L1max, L2max, ts.planes, ts.stride, cap, const_cast<PairDesc*>(a.desc),
-                       const_cast<int32_t*>(a.order), a.claims, (float*)nullptr, a.grid,
+                       const_cast<int32_t*>(a.order), a.claims, a.grid,
                        a.grid ? (int)grid_zero_ints(B, a.max_strips) : 0);
     stage_mark(0, L.stream);
     return check_launch("plan kernel");
@@ -1483,7 +1547,8 @@ ohm_status nw_forward_impl(const float* scores, const int32_t* lengths, const in
                            size_t ws_bytes, void* stream, bool gap_sum = false) {
     if (B > 0 && !value) return fail(OHM_EINVAL, "value is null");
     const bool need_bwd = map || g0 || g1 || g2;
-    if (need_bwd && !map) return fail(OHM_EINVAL, "the gradients need the map buffer");
+    if (need_bwd && !map && L1max > 0 && L2max > 0)  // (a batch without cells has no map)
+        return fail(OHM_EINVAL, "the gradients need the map buffer");
     Launch L;
     ohm_status st = setup(need_bwd ? kOpNw : kOpValue, true, scores, lengths, offsets, B, L1max, L2max, go,
                           ge, temp, ws, ws_bytes, stream, L);
@@ -1577,7 +1642,8 @@ ohm_status sv_forward_impl(const float* scores, const int32_t* lengths, const in
                            size_t ws_bytes, void* stream, bool gap_sum = false) {
     if (B > 0 && !value) return fail(OHM_EINVAL, "value is null");
     const bool need_bwd = map || g0 || g1 || g2;
-    if (need_bwd && !map) return fail(OHM_EINVAL, "the gradients need the map buffer");
+    if (need_bwd && !map && L1max > 0 && L2max > 0)  // (a batch without cells has no map)
+        return fail(OHM_EINVAL, "the gradients need the map buffer");
     Launch L;
     ohm_status st = setup(need_bwd ? kOpSv : kOpValue, true, scores, lengths, offsets, B, L1max, L2max, go,
                           ge, temp, ws, ws_bytes, stream, L);
@@ -1629,11 +1695,11 @@ ohm_status tangent_impl(bool affine, const float* scores, const int32_t* lengths
         const size_t lds = grid_cum_bytes(L.a.max_strips);
         if (grid && (st = launch_grid_fill(L)) != OHM_OK) return st;
         if (mode == kTanParam) {
-            if (grid) st = launch_grid(sw_affine_scaled_hvp_forward_grid<kTanParam>, lds, L);
-            else OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_forward_kernel<W, kTanParam>));
+            if (grid) st = launch_grid(sw_affine_param_grad_scaled_forward_grid, lds, L);
+            else OHM_DISPATCH_W(L, (sw_affine_param_grad_scaled_forward_kernel<W>));
         } else {
-            if (grid) st = launch_grid(sw_affine_scaled_hvp_forward_grid<kTanScore>, lds, L);
-            else OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_forward_kernel<W, kTanScore>));
+            if (grid) st = launch_grid(sw_affine_hvp_scaled_forward_grid, lds, L);
+            else OHM_DISPATCH_W(L, (sw_affine_hvp_scaled_forward_kernel<W>));
         }
         if (st != OHM_OK) return st;
         stage_mark(1, L.stream);
@@ -1644,14 +1710,14 @@ ohm_status tangent_impl(bool affine, const float* scores, const int32_t* lengths
         L.a.sel = 0;
         if (grid && (st = launch_grid_fill(L)) != OHM_OK) return st;
         if (mode == kTanParam) {
-            if (grid) st = launch_grid(sw_affine_scaled_hvp_backward_grid<kTanParam>, lds, L);
-            else OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_backward_kernel<W, kTanParam>));
+            if (grid) st = launch_grid(sw_affine_param_grad_scaled_backward_grid, lds, L);
+            else OHM_DISPATCH_W(L, (sw_affine_param_grad_scaled_backward_kernel<W>));
         } else if (mode == kTanContract) {
-            if (grid) st = launch_grid(sw_affine_scaled_hvp_backward_grid<kTanContract>, lds, L);
-            else OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_backward_kernel<W, kTanContract>));
+            if (grid) st = launch_grid(sw_affine_param_grad_contract_backward_grid, lds, L);
+            else OHM_DISPATCH_W(L, (sw_affine_param_grad_contract_backward_kernel<W>));
         } else {
-            if (grid) st = launch_grid(sw_affine_scaled_hvp_backward_grid<kTanScore>, lds, L);
-            else OHM_DISPATCH_W(L, (sw_affine_scaled_hvp_backward_kernel<W, kTanScore>));
+            if (grid) st = launch_grid(sw_affine_hvp_scaled_backward_grid, lds, L);
+            else OHM_DISPATCH_W(L, (sw_affine_hvp_scaled_backward_kernel<W>));
         }
         if (st != OHM_OK) return st;
         stage_mark(3, L.stream);
@@ -1737,9 +1803,11 @@ extern "C" ohm_status ohm_nw_affine_tangent(const float* scores, const int32_t* 
                                             float* contract, float* value, float* map,
                                             void* workspace, size_t workspace_bytes,
                                             void* stream) {
-    return nw_tangent_impl(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
+    ohm_status st = nw_tangent_impl(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
                            direction, d_gap_open, d_gap_ext, d_temp, map_tangent, contract,
                            value, map, workspace, workspace_bytes, stream);
+    if (st != OHM_OK) return st;
+    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 3, map, map_tangent}, stream);
 }
 
 extern "C" ohm_status ohm_nw_affine_forward(const float* scores, const int32_t* lengths,
@@ -1749,9 +1817,11 @@ extern "C" ohm_status ohm_nw_affine_forward(const float* scores, const int32_t* 
                                             float* grad_open, float* grad_ext, float* grad_temp,
                                             void* workspace, size_t workspace_bytes,
                                             void* stream) {
-    return nw_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
+    ohm_status st = nw_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
                            value, map, grad_open, grad_ext, grad_temp, workspace,
                            workspace_bytes, stream);
+    if (st != OHM_OK) return st;
+    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {grad_open, grad_ext, grad_temp}, nullptr, 0, map, nullptr}, stream);
 }
 
 extern "C" size_t ohm_sv_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
@@ -1769,9 +1839,11 @@ extern "C" ohm_status ohm_sv_affine_tangent(const float* scores, const int32_t* 
                                             float* contract, float* value, float* map,
                                             void* workspace, size_t workspace_bytes,
                                             void* stream) {
-    return sv_tangent_impl(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
+    ohm_status st = sv_tangent_impl(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
                            direction, d_gap_open, d_gap_ext, d_temp, map_tangent, contract,
                            value, map, workspace, workspace_bytes, stream);
+    if (st != OHM_OK) return st;
+    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 3, map, map_tangent}, stream);
 }
 
 extern "C" ohm_status ohm_sv_affine_forward(const float* scores, const int32_t* lengths,
@@ -1781,9 +1853,11 @@ extern "C" ohm_status ohm_sv_affine_forward(const float* scores, const int32_t* 
                                             float* grad_open, float* grad_ext, float* grad_temp,
                                             void* workspace, size_t workspace_bytes,
                                             void* stream) {
-    return sv_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
+    ohm_status st = sv_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
                            value, map, grad_open, grad_ext, grad_temp, workspace,
                            workspace_bytes, stream);
+    if (st != OHM_OK) return st;
+    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {grad_open, grad_ext, grad_temp}, nullptr, 0, map, nullptr}, stream);
 }
 
 // Linear-gap Needleman-Wunsch and Saigo-Vert: the affine sweeps with one gap
@@ -1804,8 +1878,10 @@ extern "C" ohm_status ohm_nw_forward(const float* scores, const int32_t* lengths
                                      int32_t L2max, float gap, float temp, float* value,
                                      float* map, float* grad_gap, float* grad_temp,
                                      void* workspace, size_t workspace_bytes, void* stream) {
-    return nw_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap, gap, temp, value, map,
+    ohm_status st = nw_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap, gap, temp, value, map,
                            grad_gap, nullptr, grad_temp, workspace, workspace_bytes, stream, true);
+    if (st != OHM_OK) return st;
+    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {grad_gap, grad_temp, nullptr}, nullptr, 0, map, nullptr}, stream);
 }
 
 extern "C" ohm_status ohm_sv_forward(const float* scores, const int32_t* lengths,
@@ -1813,8 +1889,10 @@ extern "C" ohm_status ohm_sv_forward(const float* scores, const int32_t* lengths
                                      int32_t L2max, float gap, float temp, float* value,
                                      float* map, float* grad_gap, float* grad_temp,
                                      void* workspace, size_t workspace_bytes, void* stream) {
-    return sv_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap, gap, temp, value, map,
+    ohm_status st = sv_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap, gap, temp, value, map,
                            grad_gap, nullptr, grad_temp, workspace, workspace_bytes, stream, true);
+    if (st != OHM_OK) return st;
+    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {grad_gap, grad_temp, nullptr}, nullptr, 0, map, nullptr}, stream);
 }
 
 extern "C" size_t ohm_sw_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
@@ -1831,9 +1909,11 @@ extern "C" ohm_status ohm_sw_affine_forward(const float* scores, const int32_t* 
                                             float* grad_open, float* grad_ext, float* grad_temp,
                                             void* workspace, size_t workspace_bytes,
                                             void* stream) {
-    return forward_impl(true, scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
+    ohm_status st = forward_impl(true, scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
                         value, map, grad_open, grad_ext, grad_temp, workspace, workspace_bytes,
                         stream);
+    if (st != OHM_OK) return st;
+    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {grad_open, grad_ext, grad_temp}, nullptr, 0, map, nullptr}, stream);
 }
 
 extern "C" ohm_status ohm_sw_affine_tangent(const float* scores, const int32_t* lengths,
@@ -1844,9 +1924,11 @@ extern "C" ohm_status ohm_sw_affine_tangent(const float* scores, const int32_t* 
                                             float* contract, float* value, float* map,
                                             void* workspace, size_t workspace_bytes,
                                             void* stream) {
-    return tangent_impl(true, scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
+    ohm_status st = tangent_impl(true, scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
                         direction, d_gap_open, d_gap_ext, d_temp, map_tangent, contract, value,
                         map, workspace, workspace_bytes, stream);
+    if (st != OHM_OK) return st;
+    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 3, map, map_tangent}, stream);
 }
 
 extern "C" ohm_status ohm_sw_forward(const float* scores, const int32_t* lengths,
@@ -1854,8 +1936,10 @@ extern "C" ohm_status ohm_sw_forward(const float* scores, const int32_t* lengths
                                      int32_t L2max, float gap, float temp, float* value,
                                      float* map, float* grad_gap, float* grad_temp,
                                      void* workspace, size_t workspace_bytes, void* stream) {
-    return forward_impl(false, scores, lengths, offsets, B, L1max, L2max, gap, 0.0f, temp, value,
+    ohm_status st = forward_impl(false, scores, lengths, offsets, B, L1max, L2max, gap, 0.0f, temp, value,
                         map, grad_gap, grad_temp, nullptr, workspace, workspace_bytes, stream);
+    if (st != OHM_OK) return st;
+    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {grad_gap, grad_temp, nullptr}, nullptr, 0, map, nullptr}, stream);
 }
 
 extern "C" ohm_status ohm_sw_tangent(const float* scores, const int32_t* lengths,
@@ -1864,7 +1948,9 @@ extern "C" ohm_status ohm_sw_tangent(const float* scores, const int32_t* lengths
                                      float d_gap, float d_temp, float* map_tangent,
                                      float* contract, float* value, float* map, void* workspace,
                                      size_t workspace_bytes, void* stream) {
-    return tangent_impl(false, scores, lengths, offsets, B, L1max, L2max, gap, 0.0f, temp,
+    ohm_status st = tangent_impl(false, scores, lengths, offsets, B, L1max, L2max, gap, 0.0f, temp,
                         direction, d_gap, 0.0f, d_temp, map_tangent, contract, value, map,
                         workspace, workspace_bytes, stream);
+    if (st != OHM_OK) return st;
+    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 2, map, map_tangent}, stream);
 }

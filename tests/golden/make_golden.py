@@ -223,6 +223,18 @@ def main():
         s = _draw(140, (5, 40, 36), extra=0)[0]
         ln = np.array([[40, 36], [17, 29], [0, 6], [33, 0], [1, 1]], np.int32)
         linear_gl_case(f"{fam}_linear_ragged_b5_40x36", s, -1.0, 0.7, ln, family=fam)
+    # Empty dimensions in the tensor's shape (not through lengths): the
+    # reference accepts (B, 0, N) and (B, M, 0) and returns the empty
+    # alignment (0; for global alignment the gap run over the other side).
+    for shape in ((2, 0, 5), (2, 5, 0)):
+        tag = "x".join(str(n) for n in shape)
+        z = np.zeros(shape, np.float32)
+        affine_case(f"affine_empty_{tag}", z, z, z, -2.0, -0.5, 1.0)
+        linear_case(f"linear_empty_{tag}", z, z, z, -1.0, 1.0)
+        nw_affine_case(f"nw_affine_empty_{tag}", z, -2.0, -0.5, 1.0)
+        nw_affine_case(f"sv_affine_empty_{tag}", z, -2.0, -0.5, 1.0, family="sv")
+        for fam in ("nw", "sv"):
+            linear_gl_case(f"{fam}_linear_empty_{tag}", z, -1.0, 1.0, family=fam)
     print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
 
 

@@ -111,7 +111,7 @@ def test_affine_cpu_path_matches_golden(path):
         s64 = d["scores"].astype(np.float64)
         lnn = d["lengths"]
         # second-order outputs scale like 1/T: tolerance relative to their size
-        tol = lambda r: 1e-4 * max(1.0, np.abs(r).max())
+        tol = lambda r: 1e-4 * max(1.0, np.abs(r).max(initial=0.0))
         got = torch.ops.orihime.sw_affine_marginals_hvp(s, V, go, ge, T, ln)
         assert close_or_nearer_truth(got, d["hvp"], lambda: O.sw_affine_hvp(s64, d["V"], go, ge, T, lnn, prec="f64"), tol(d["hvp"]))
         for p, n in enumerate(("gap_open", "gap_ext", "temp")):
@@ -139,7 +139,7 @@ def test_linear_cpu_path_matches_golden(path):
     assert np.all(np.abs(b.numpy() - d["grad_temp"]) <= entropy_tol(d["value"], T) * np.maximum(1, np.abs(d["grad_temp"])))
     if "hvp" in d:
         s64, lnn = d["scores"].astype(np.float64), d["lengths"]
-        tol = lambda r: 1e-4 * max(1.0, np.abs(r).max())
+        tol = lambda r: 1e-4 * max(1.0, np.abs(r).max(initial=0.0))
         got = torch.ops.orihime.sw_marginals_hvp(s, _t(d["V"]), g, T, ln)
         assert close_or_nearer_truth(got, d["hvp"], lambda: O.sw_hvp(s64, d["V"], g, T, lnn, prec="f64"), tol(d["hvp"]))
         for p, n in enumerate(("gap", "temp")):

@@ -414,7 +414,7 @@ def test_gpu_second_order_on_multi_strip_pairs(B, L1, L2, go, ge, T):
     for kw, truth in ((dict(direction=V), hv), (dict(d=(1.0, 0.0, 0.0)), sens[0]),
                       (dict(d=(0.0, 1.0, 0.0)), sens[1]), (dict(d=(0.0, 0.0, 1.0)), sens[2])):
         g = _tangent(s, ln, go, ge, T, device="cuda", **kw)[0]
-        assert abs_err(g, truth) < 2e-4 * max(1.0, float(np.abs(truth).max()))
+        assert abs_err(g, truth) < 2e-4 * max(1.0, float(np.abs(truth).max(initial=0.0)))
     gm, gc = _tangent(s, ln, go, ge, T, device="cuda", direction=G, contract=True)
     assert _grad_err(gc, con) < 2e-3
 
@@ -465,7 +465,7 @@ def _check_second_order_golden(d, device):
     go, ge, T = _params(d)
     s, ln, V = d["scores"], d["lengths"], d["V"]
     f64 = lambda: _fd_second_order(s.astype(np.float64), ln, go, ge, T, V, V)  # noqa: E731
-    tol = lambda r: 1e-4 * max(1.0, float(np.abs(r).max()))  # noqa: E731
+    tol = lambda r: 1e-4 * max(1.0, float(np.abs(r).max(initial=0.0)))  # noqa: E731
     got = _tangent(s, ln, go, ge, T, device=device, direction=V)[0]
     assert _second_order_close(got, d["hvp"], lambda: f64()[0], tol(d["hvp"]))
     for p, n in enumerate(("gap_open", "gap_ext", "temp")):
