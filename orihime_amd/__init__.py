@@ -22,9 +22,9 @@ from . import nn as nn
 from . import raw as raw
 from . import ops as ops
 from . import ragged as ragged
-from ._public_ops import install_public_operations as _install_public_operations
+from .families import install_operations as _install_operations
 
-_install_public_operations(ops)
+_install_operations(ops)
 
 __all__ = [
     "__version__",

@@ -381,7 +381,8 @@ def soft_sw_param_jacobian_impl(scores, param_type: int, gap: float, temperature
 # --------------------------------------------------------------------------
 # Needleman-Wunsch and Saigo-Vert, affine gap (src/nw_affine/registry.cpp:33-46,
 # src/sv_affine/registry.cpp:29-44): forward + reverse sweeps for value, map
-# and parameter gradients; the second-order schemas have no kernel yet.
+# and parameter gradients; the second-order schemas run the families'
+# forward-over-reverse tangent sweeps.
 # --------------------------------------------------------------------------
 
 
@@ -491,25 +492,15 @@ def _three_state_family(family: str, call, value_first: bool):
         value, _, _ = run(scores, gap_open, gap_ext, temp, lengths, want_map=False)
         return value
 
-    def second_order(what: str):
-        def impl(scores, *args):
-            raise NotImplementedError(
-                f"{family}_affine {what} (second-order derivatives of the map) has no kernel in "
-                "orihime_amd yet; the value, map and parameter gradients are available")
-
-        return impl
-
-    return forward, forward_t, value_grad_params, with_grads, value_only, second_order
+    return forward, forward_t, value_grad_params, with_grads, value_only
 
 
-def _three_state_schemas(family: str, call, value_first: bool, tangent=None) -> dict:
-    fwd, fwd_t, vgp, wg, vo, so = _three_state_family(family, call, value_first)
-    if tangent is not None:
-        hvp, sens, mb, jac = _second_order(tangent, linear=False)
-        so2 = {"marginals_backward": mb, "marginals_hvp": hvp, "marginals_grad_gap_open": sens(0),
-               "marginals_grad_gap_ext": sens(1), "marginals_grad_temp": sens(2), "hvp": hvp,
-               "param_jacobian": jac, "backward_full": mb}
-        so = so2.__getitem__
+def _three_state_schemas(family: str, call, value_first: bool, tangent) -> dict:
+    fwd, fwd_t, vgp, wg, vo = _three_state_family(family, call, value_first)
+    hvp, sens, mb, jac = _second_order(tangent, linear=False)
+    so = {"marginals_backward": mb, "marginals_hvp": hvp, "marginals_grad_gap_open": sens(0),
+          "marginals_grad_gap_ext": sens(1), "marginals_grad_temp": sens(2), "hvp": hvp,
+          "param_jacobian": jac, "backward_full": mb}.__getitem__
     f = family
     S = "(Tensor scores, float gap_open, float gap_ext, float temp, Tensor? lengths)"
     L = "(Tensor scores, float gap_open, float gap_ext, float temperature, Tensor? lengths)"
@@ -533,7 +524,7 @@ def _three_state_schemas(family: str, call, value_first: bool, tangent=None) -> 
     }
 
 
-def _linear_family_schemas(ns: str, soft: str, call, model: str, tangent=None) -> dict:
+def _linear_family_schemas(ns: str, soft: str, call, model: str, tangent) -> dict:
     """First-order schemas of a linear-gap family (src/nw/registry.cpp:35-49,
     src/sv_linear/registry.cpp): the affine sweeps with one gap score in both
     roles, dS/dgap = dS/dgap_open + dS/dgap_ext (summed in the kernel)."""
@@ -570,19 +561,10 @@ def _linear_family_schemas(ns: str, soft: str, call, model: str, tangent=None) -
         value, _, _ = run(scores, gap, temp, lengths, want_map=False)
         return value
 
-    def second_order(what: str):
-        def impl(scores, *args):
-            raise NotImplementedError(
-                f"{model} {what} (second-order derivatives of the map) has no kernel in "
-                "orihime_amd yet; the value, map and parameter gradients are available")
-
-        return impl
-
-    if tangent is not None:
-        hvp, sens, mb, jac = _second_order(tangent, linear=True)
-        second_order = {"marginals_backward": mb, "marginals_hvp": hvp,
-                        "marginals_grad_gap": sens(0), "marginals_grad_temp": sens(1),
-                        "hvp": hvp, "param_jacobian": jac, "backward_full": mb}.__getitem__
+    hvp, sens, mb, jac = _second_order(tangent, linear=True)
+    second_order = {"marginals_backward": mb, "marginals_hvp": hvp,
+                    "marginals_grad_gap": sens(0), "marginals_grad_temp": sens(1),
+                    "hvp": hvp, "param_jacobian": jac, "backward_full": mb}.__getitem__
 
     S = "(Tensor scores, float gap, float temp, Tensor? lengths)"
     L = "(Tensor scores, float gap, float temperature, Tensor? lengths)"
@@ -706,6 +688,9 @@ orihime = torch.ops.orihime
 
 
 def _forward_setup(ctx, inputs, output):
+    # an unused output's gradient arrives as None (not zeros), so a backward
+    # through the value alone runs no map VJP
+    ctx.set_materialize_grads(False)
     scores = inputs[0]
     ctx.save_for_backward(scores, output[1])
     ctx.inputs = inputs

@@ -1,188 +1,131 @@
-"""Stateful alignment modules with learnable parameters (orihime/nn.py:51-390):
-Smith-Waterman, Saigo-Vert and Needleman-Wunsch, linear and affine gaps."""
+"""``torch.nn`` layers for every alignment family, with per-parameter
+learnability (reference surface: orihime/nn.py:51-390; class names and
+constructor arguments are the reference's).
+
+A layer stores its family's scalar parameters (gap scores and temperature)
+as tensors: those named in ``learnable`` as ``nn.Parameter``s, the others as
+persistent buffers, so both travel with ``state_dict`` and ``.to()``.
+``layer(pair_scores, ...)`` returns the map, ``layer.value`` and
+``layer.entropy`` the per-pair observables; extra keyword arguments
+(``lengths``, ``mask``, ``dtype``) go to the public function unchanged.
+"""
 
 from __future__ import annotations
 
-from collections.abc import Callable, Mapping, Sequence
+from collections.abc import Sequence
+from inspect import Parameter, Signature
 from typing import Any, ClassVar
 
 import torch
 from torch import Tensor, nn
 
-from .operator import _validate_scalar_parameter_shape, _validate_temperature
-from .nw import nw, nw_affine, nw_affine_entropy, nw_affine_value, nw_entropy, nw_value
-from .sv import sv, sv_affine, sv_affine_entropy, sv_affine_value, sv_entropy, sv_value
-from .sw import sw, sw_affine, sw_affine_entropy, sw_affine_value, sw_entropy, sw_value
+from . import _checks
+from .families import FAMILIES, Family, public_functions
 
-ParamValue = float | Tensor
+_DEFAULT = {"temperature": 1.0}
 
 
-class _OperatorModule(nn.Module):
-    """Thin stateful wrapper around a map/value/entropy function triple.
-    Names listed in ``learnable`` become ``nn.Parameter``s; the other model
-    parameters are persistent buffers."""
+def _stored(value: float | Tensor, name: str, device, dtype) -> Tensor:
+    """A fresh floating tensor holding ``value`` (a number or a tensor)."""
+    if isinstance(value, Tensor):
+        t = value.detach().to(device=device, dtype=dtype if dtype is not None else (
+            value.dtype if value.is_floating_point() else torch.get_default_dtype())).clone()
+    elif isinstance(value, (int, float)) and not isinstance(value, bool):
+        t = torch.tensor(float(value), device=device, dtype=dtype or torch.get_default_dtype())
+    else:
+        raise TypeError("operator parameters must be numbers or tensors")
+    if not t.is_floating_point():
+        raise TypeError("operator parameters must have a floating-point dtype")
+    _checks.scalar_shape(t, name)
+    return t
 
-    _param_names: ClassVar[tuple[str, ...]] = ()
 
-    def __init__(self, *, params: Mapping[str, ParamValue], map_function: Callable[..., Tensor],
-                 value_function: Callable[..., Tensor], entropy_function: Callable[..., Tensor],
-                 learnable: Sequence[str] = (), device=None, dtype=None) -> None:
+def _learnable_names(learnable: Sequence[str], allowed: tuple[str, ...]) -> frozenset[str]:
+    if isinstance(learnable, (str, bytes)) or not isinstance(learnable, Sequence):
+        raise TypeError("learnable must be a sequence of parameter names")
+    names = tuple(learnable)
+    if any(not isinstance(n, str) or not n for n in names):
+        raise TypeError("learnable must contain non-empty strings")
+    if len(set(names)) < len(names):
+        raise ValueError("learnable must not contain duplicates")
+    unknown = sorted(set(names) - set(allowed))
+    if unknown:
+        raise ValueError("unknown learnable parameter(s): " + ", ".join(unknown))
+    return frozenset(names)
+
+
+class AlignmentLayer(nn.Module):
+    """A soft-alignment layer of one family (``family`` on the subclass)."""
+
+    family: ClassVar[Family]
+
+    def __init__(self, *, learnable: Sequence[str] = (), device=None, dtype=None,
+                 **values: float | Tensor) -> None:
         super().__init__()
-        if set(params) != set(self._param_names):
-            missing = set(self._param_names) - set(params)
-            unknown = set(params) - set(self._param_names)
-            details = []
-            if missing:
-                details.append("missing " + ", ".join(sorted(missing)))
-            if unknown:
-                details.append("unknown " + ", ".join(sorted(unknown)))
-            raise TypeError("invalid parameter mapping: " + "; ".join(details))
-        if isinstance(learnable, (str, bytes)):
-            raise TypeError("learnable must be a sequence of parameter names")
-        names = tuple(learnable)
-        if not all(isinstance(n, str) and n for n in names):
-            raise TypeError("learnable must contain non-empty strings")
-        if len(set(names)) != len(names):
-            raise ValueError("learnable must not contain duplicates")
-        unknown_l = set(names) - set(self._param_names)
-        if unknown_l:
-            raise ValueError("unknown learnable parameter(s): " + ", ".join(sorted(unknown_l)))
-        object.__setattr__(self, "_map_function", map_function)
-        object.__setattr__(self, "_value_function", value_function)
-        object.__setattr__(self, "_entropy_function", entropy_function)
-        for name in self._param_names:
-            t = self._parameter_tensor(params[name], device=device, dtype=dtype)
-            _validate_scalar_parameter_shape(t, name)
-            if name in names:
-                self.register_parameter(name, nn.Parameter(t))
+        names = self.family.parameters
+        for extra in values:
+            if extra not in names:
+                raise TypeError(
+                    f"{type(self).__name__}() got an unexpected keyword argument {extra!r}")
+        trained = _learnable_names(learnable, names)
+        for n in names:
+            t = _stored(values.get(n, _DEFAULT.get(n, 0.0)), n, device, dtype)
+            if n in trained:
+                self.register_parameter(n, nn.Parameter(t))
             else:
-                self.register_buffer(name, t, persistent=True)
-        self._validate_constraints()
+                self.register_buffer(n, t, persistent=True)
+        _checks.temperature(self.temperature)
 
-    @staticmethod
-    def _parameter_tensor(value: ParamValue, *, device, dtype) -> Tensor:
-        if isinstance(value, Tensor):
-            if not value.is_floating_point() and dtype is None:
-                dtype = torch.get_default_dtype()
-            t = value.detach().to(device=device, dtype=dtype).clone()
-        elif isinstance(value, (int, float)):
-            t = torch.tensor(value, device=device, dtype=dtype or torch.get_default_dtype())
-        else:
-            raise TypeError("operator parameters must be numbers or tensors")
-        if not t.is_floating_point():
-            raise TypeError("operator parameters must have a floating-point dtype")
-        return t
+    def _scoring(self) -> dict[str, Tensor]:
+        _checks.temperature(self.temperature)
+        return {n: getattr(self, n) for n in self.family.parameters}
 
-    def _validate_constraints(self) -> None:
-        temperature = getattr(self, "temperature", None)
-        if temperature is not None:
-            _validate_temperature(temperature)
+    def forward(self, pair_scores: Tensor, **options: Any) -> Tensor:
+        """Match-probability map, shaped like ``pair_scores``."""
+        return public_functions(self.family.name)[0](pair_scores, **self._scoring(), **options)
 
-    def _parameter_kwargs(self) -> dict[str, Tensor]:
-        self._validate_constraints()
-        return {name: getattr(self, name) for name in self._param_names}
+    def value(self, pair_scores: Tensor, **options: Any) -> Tensor:
+        """Soft alignment score per pair, ``[B]``."""
+        return public_functions(self.family.name)[1](pair_scores, **self._scoring(), **options)
 
-    def forward(self, *inputs: Tensor, **structural: Any) -> Tensor:
-        """Return the operator map."""
-        return self._map_function(*inputs, **self._parameter_kwargs(), **structural)
-
-    def value(self, *inputs: Tensor, **structural: Any) -> Tensor:
-        """Return the operator value."""
-        return self._value_function(*inputs, **self._parameter_kwargs(), **structural)
-
-    def entropy(self, *inputs: Tensor, **structural: Any) -> Tensor:
-        """Return the operator entropy."""
-        return self._entropy_function(*inputs, **self._parameter_kwargs(), **structural)
+    def entropy(self, pair_scores: Tensor, **options: Any) -> Tensor:
+        """Entropy of the alignment distribution per pair, ``[B]``."""
+        return public_functions(self.family.name)[2](pair_scores, **self._scoring(), **options)
 
     def extra_repr(self) -> str:
-        learnable = tuple(n for n in self._param_names if n in self._parameters)
-        return f"learnable={learnable!r}"
+        trained = tuple(n for n in self.family.parameters if n in self._parameters)
+        return f"learnable={trained!r}"
 
 
-class SmithWaterman(_OperatorModule):
-    """Stateful Smith-Waterman layer with per-parameter learnability."""
+def _layer_class(fam: Family) -> type[AlignmentLayer]:
+    kw = Parameter.KEYWORD_ONLY
+    sig = Signature([Parameter("self", Parameter.POSITIONAL_ONLY),
+                     *(Parameter(n, kw, default=_DEFAULT.get(n, 0.0), annotation=float | Tensor)
+                       for n in fam.parameters),
+                     Parameter("learnable", kw, default=(), annotation=tuple[str, ...]),
+                     Parameter("device", kw, default=None), Parameter("dtype", kw, default=None)])
 
-    _param_names = ("gap_score", "temperature")
+    def __init__(self, **kwargs):
+        AlignmentLayer.__init__(self, **kwargs)
 
-    def __init__(self, *, gap_score: float | Tensor = 0.0, temperature: float | Tensor = 1.0,
-                 learnable: tuple[str, ...] = (), device=None, dtype=None) -> None:
-        super().__init__(params={"gap_score": gap_score, "temperature": temperature},
-                         map_function=sw, value_function=sw_value, entropy_function=sw_entropy,
-                         learnable=learnable, device=device, dtype=dtype)
-
-
-class SmithWatermanAffine(_OperatorModule):
-    """Affine Smith-Waterman layer with independently learnable parameters."""
-
-    _param_names = ("gap_open_score", "gap_extend_score", "temperature")
-
-    def __init__(self, *, gap_open_score: float | Tensor = 0.0,
-                 gap_extend_score: float | Tensor = 0.0, temperature: float | Tensor = 1.0,
-                 learnable: tuple[str, ...] = (), device=None, dtype=None) -> None:
-        super().__init__(params={"gap_open_score": gap_open_score,
-                                 "gap_extend_score": gap_extend_score, "temperature": temperature},
-                         map_function=sw_affine, value_function=sw_affine_value,
-                         entropy_function=sw_affine_entropy, learnable=learnable, device=device,
-                         dtype=dtype)
+    __init__.__signature__ = sig
+    __init__.__qualname__ = f"{fam.layer}.__init__"
+    doc = (f"Soft {fam.title} alignment layer; learnable subset of "
+           f"{', '.join(fam.parameters)} chosen with ``learnable``.")
+    return type(fam.layer, (AlignmentLayer,), {"family": fam, "__init__": __init__,
+                                               "__doc__": doc, "__module__": __name__})
 
 
-class _Linear(_OperatorModule):
-    _param_names = ("gap_score", "temperature")
-    _functions: ClassVar[tuple] = ()
+_CLASSES = {fam.layer: _layer_class(fam) for fam in FAMILIES.values()}
+SmithWaterman = _CLASSES["SmithWaterman"]
+SmithWatermanAffine = _CLASSES["SmithWatermanAffine"]
+NeedlemanWunsch = _CLASSES["NeedlemanWunsch"]
+NeedlemanWunschAffine = _CLASSES["NeedlemanWunschAffine"]
+SaigoVertLinear = _CLASSES["SaigoVertLinear"]
+SaigoVertAffine = _CLASSES["SaigoVertAffine"]
+# short aliases the reference also exports
+NW, NWAffine, SVLinear, SVAffine = NeedlemanWunsch, NeedlemanWunschAffine, SaigoVertLinear, SaigoVertAffine
 
-    def __init__(self, *, gap_score: float | Tensor = 0.0, temperature: float | Tensor = 1.0,
-                 learnable: tuple[str, ...] = (), device=None, dtype=None) -> None:
-        m, v, e = self._functions
-        super().__init__(params={"gap_score": gap_score, "temperature": temperature},
-                         map_function=m, value_function=v, entropy_function=e,
-                         learnable=learnable, device=device, dtype=dtype)
-
-
-class _Affine(_OperatorModule):
-    _param_names = ("gap_open_score", "gap_extend_score", "temperature")
-    _functions: ClassVar[tuple] = ()
-
-    def __init__(self, *, gap_open_score: float | Tensor = 0.0,
-                 gap_extend_score: float | Tensor = 0.0, temperature: float | Tensor = 1.0,
-                 learnable: tuple[str, ...] = (), device=None, dtype=None) -> None:
-        m, v, e = self._functions
-        super().__init__(params={"gap_open_score": gap_open_score,
-                                 "gap_extend_score": gap_extend_score, "temperature": temperature},
-                         map_function=m, value_function=v, entropy_function=e,
-                         learnable=learnable, device=device, dtype=dtype)
-
-
-class SaigoVertLinear(_Linear):
-    """Canonical linear-gap Saigo-Vert layer."""
-
-    _functions = (staticmethod(sv), staticmethod(sv_value), staticmethod(sv_entropy))
-
-
-class SaigoVertAffine(_Affine):
-    """Canonical affine-gap Saigo-Vert layer."""
-
-    _functions = (staticmethod(sv_affine), staticmethod(sv_affine_value),
-                  staticmethod(sv_affine_entropy))
-
-
-class NeedlemanWunsch(_Linear):
-    """Score-native soft Needleman-Wunsch global alignment."""
-
-    _functions = (staticmethod(nw), staticmethod(nw_value), staticmethod(nw_entropy))
-
-
-class NeedlemanWunschAffine(_Affine):
-    """Stateful affine Needleman-Wunsch layer."""
-
-    _functions = (staticmethod(nw_affine), staticmethod(nw_affine_value),
-                  staticmethod(nw_affine_entropy))
-
-
-SVLinear = SaigoVertLinear
-SVAffine = SaigoVertAffine
-NW = NeedlemanWunsch
-NWAffine = NeedlemanWunschAffine
-
-__all__ = ["SmithWaterman", "SmithWatermanAffine", "SaigoVertLinear", "SVLinear",
+__all__ = ["AlignmentLayer", "SmithWaterman", "SmithWatermanAffine", "SaigoVertLinear", "SVLinear",
            "SaigoVertAffine", "SVAffine", "NeedlemanWunsch", "NW", "NeedlemanWunschAffine",
            "NWAffine"]

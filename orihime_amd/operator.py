@@ -1,12 +1,11 @@
 """Operator engine for the soft Smith-Waterman family.
 
-Restates the behaviour of the reference engine (orihime/operator.py) that
-the SW operators rely on: input validation and its messages
-(:1948-2074), the FP32 numerical domain |x| / T <= 80 (:249-331), mask
-handling with the finite exclusion sentinel (:334-386, :2178-2206),
-autocast / dtype handling (:958-984), and the value / map / entropy
-observables with their autograd rules (:1288-1716), the map VJP and the
-parameter sensitivities (:2312-2399).
+Behaviour of the reference engine (orihime/operator.py) that the alignment
+operators rely on: input validation and its messages (:1948-2074; the
+shared argument checks live in ``_checks``), mask handling with the finite
+exclusion sentinel (:2178-2206), autocast / dtype handling (:958-984), and
+the value / map / entropy observables with their autograd rules
+(:1288-1716), the map VJP and the parameter sensitivities (:2312-2399).
 
 Kernel work goes through the per-family adapters in ``ops/`` (named
 primitives of ``_ops``); this module never computes a DP itself.
@@ -14,7 +13,6 @@ primitives of ``_ops``); this module never computes a DP itself.
 
 from __future__ import annotations
 
-import math
 from collections.abc import Callable, Mapping, Sequence
 from dataclasses import dataclass, field
 from types import MappingProxyType
@@ -23,211 +21,16 @@ from typing import Any
 import torch
 from torch import Tensor
 
-Param = float | Tensor
+from . import _checks
+from ._checks import Param
 
-_MAX_ABS_SCORE_TEMPERATURE_RATIO = 80.0
-_MASK_SENTINEL_MAGNITUDE = 1.0e4
-# A masked cell sits (path_len * 80 + 200) temperature units below the
-# retained maximum, more than any path through the grid can gain, so its
-# Boltzmann weight underflows to exactly 0 at every in-domain scale.
-_MASK_EXCLUSION_RATIO = 200.0
-
-_PUBLIC_PARAMETER_DOCS = {
-    "pair_scores": "FP32 pair-score tensor shaped ``[B, L1, L2]``.",
-    "gap_score": "Linear-gap score as a Python number or FP32 scalar tensor.",
-    "gap_open_score": "Gap-open score as a Python number or FP32 scalar tensor.",
-    "gap_extend_score": "Gap-extension score as a Python number or FP32 scalar tensor.",
-    "temperature": "Finite positive temperature as a Python number or FP32 scalar tensor.",
-    "lengths": (
-        "Optional contiguous ``torch.int32`` active lengths on the input "
-        "device, shaped ``[B, 2]``."
-    ),
-    "mask": (
-        "Optional boolean tensor shaped like the primary input; ``True`` "
-        "excludes that DP cell."
-    ),
-    "dtype": (
-        "Optional compute override. Only ``torch.float32`` is accepted; use "
-        "it to cast floating inputs to the native accumulation dtype."
-    ),
-}
-
-
-def _document_public_function(function: Callable[..., Tensor]) -> Callable[..., Tensor]:
-    """Append the shared argument contract to a public function's docstring
-    (orihime/operator.py:91-135)."""
-    import inspect
-
-    names = tuple(inspect.signature(function).parameters)
-    missing = [n for n in names if n not in _PUBLIC_PARAMETER_DOCS]
-    if missing:
-        raise RuntimeError(f"missing public documentation for {function.__name__}: {missing}")
-    if function.__name__.endswith(("_value", "_entropy")):
-        result = "An FP32 tensor with one value per batch item, shaped ``[B]``."
-    else:
-        result = "An FP32 map tensor shaped like the primary tensor input."
-    args = "\n".join(f"    {n}: {_PUBLIC_PARAMETER_DOCS[n]}" for n in names)
-    function.__doc__ = (
-        (function.__doc__ or "").rstrip()
-        + "\n\nArgs:\n" + args
-        + "\n\nReturns:\n    " + result
-        + "\n\nRaises:\n"
-        + "    TypeError: An input has the wrong tensor kind, dtype, or scalar form.\n"
-        + "    ValueError: A shape, device, length, mask, or static option is invalid.\n"
-        + "    RuntimeError: Dynamic values violate the numerical or length domain."
-    )
-    return function
-
-
-# --------------------------------------------------------------------------
-# Validation helpers
-# --------------------------------------------------------------------------
-
-
-def _graph_safe_assert(condition: Tensor, message: str) -> None:
-    """Raise RuntimeError(message) unless every element of ``condition`` holds.
-
-    A device condition is reduced on the device and moved to the host, so an
-    invalid input raises a normal exception instead of a device assert.
-    """
-    if not isinstance(condition, Tensor) or condition.dtype != torch.bool:
-        raise TypeError("internal validation condition must be a bool tensor")
-    cond = condition
-    functorch = torch._C._functorch
-    while functorch.is_gradtrackingtensor(cond) or functorch.is_batchedtensor(cond):
-        cond = functorch.get_unwrapped(cond)
-    ok = cond.all()
-    if ok.device.type != "cpu":
-        ok = ok.to("cpu")
-    if not bool(ok):
-        raise RuntimeError(message)
-
-
-def _validate_scalar_parameter_shape(value: Param, name: str) -> None:
-    if not isinstance(value, Tensor):
-        return
-    if value.ndim == 0 or (value.ndim == 1 and value.shape[0] == 1):
-        return
-    raise ValueError(
-        f"{name} must be a scalar tensor with shape [] or [1], got shape {tuple(value.shape)}"
-    )
-
-
-def _validate_derivative_vector(
-    primary: Tensor,
-    vector: Tensor,
-    *,
-    name: str,
-    primary_name: str,
-    error_type: type[Exception] | None = RuntimeError,
-) -> Tensor:
-    """Strict FP32 derivative-vector contract (orihime/operator.py:159-207)."""
-    if not isinstance(primary, Tensor):
-        raise TypeError(f"{primary_name} must be a tensor")
-
-    def fail(message: str, default: type[Exception]) -> None:
-        raise (error_type or default)(message)
-
-    if not isinstance(vector, Tensor):
-        fail(f"{name} must be a tensor", TypeError)
-    if vector.shape != primary.shape:
-        fail(
-            f"{name} must have same shape as {primary_name}; "
-            f"got {tuple(vector.shape)} and {tuple(primary.shape)}",
-            ValueError,
-        )
-    if vector.dtype != torch.float32:
-        fail(f"{name} must have dtype torch.float32, got {vector.dtype}", TypeError)
-    if vector.device != primary.device:
-        fail(
-            f"{name} must be on same device as {primary_name}; "
-            f"got {vector.device} and {primary.device}",
-            ValueError,
-        )
-    if not vector.is_contiguous():
-        fail(f"{name} must be contiguous", ValueError)
-    return vector
-
-
-def _validate_temperature(value: Param) -> None:
-    _validate_scalar_parameter_shape(value, "temperature")
-    message = "temperature must be finite and strictly positive"
-    if isinstance(value, Tensor):
-        if not value.is_floating_point():
-            raise TypeError("temperature tensor must have a floating-point dtype")
-        _graph_safe_assert(torch.isfinite(value) & (value > 0), message)
-        return
-    try:
-        scalar = float(value)
-    except (TypeError, ValueError) as exc:
-        raise TypeError("temperature must be a real number or tensor") from exc
-    if not math.isfinite(scalar) or scalar <= 0:
-        raise ValueError(f"{message}, got {scalar}")
-
-
-def _temperature_on(value: Param, tensor: Tensor) -> float | Tensor:
-    if isinstance(value, Tensor):
-        return value.to(device=tensor.device, dtype=tensor.dtype)
-    return float(value)
-
-
-def _validate_numerical_domain(
-    input_name: str, value: Tensor, param_names: tuple[str, ...], params: tuple[Param, ...]
-) -> None:
-    """|finite score or parameter| / temperature <= 80; -inf is a mask
-    (orihime/operator.py:249-331, score-native orientation)."""
-    temperature = params[param_names.index("temp")]
-    _validate_temperature(temperature)
-    limit = _MAX_ABS_SCORE_TEMPERATURE_RATIO
-    message = (
-        "orihime's supported FP32 numerical domain requires every finite "
-        f"|score or cost| / temperature to be <= {limit:g}"
-    )
-    if value.is_floating_point():
-        _graph_safe_assert(
-            ~torch.isposinf(value),
-            f"{input_name} contains an unsupported infinity; use -inf score masks",
-        )
-        scaled = limit * _temperature_on(temperature, value)
-        ok = ~torch.isnan(value) & (
-            torch.isneginf(value) | (value == -_MASK_SENTINEL_MAGNITUDE) | (value.abs() <= scaled)
-        )
-        _graph_safe_assert(ok, f"{message}; {input_name} is out of domain")
-    for name, p in zip(param_names, params):
-        if name == "temp":
-            continue
-        _validate_scalar_parameter_shape(p, name)
-        if isinstance(p, Tensor):
-            if not p.is_floating_point():
-                raise TypeError(f"{name} tensor must have a floating-point dtype")
-            scaled = limit * _temperature_on(temperature, p)
-            _graph_safe_assert(
-                torch.isfinite(p) & (p.abs() <= scaled), f"{message}; {name} is out of domain"
-            )
-            continue
-        scalar = float(p)
-        if not math.isfinite(scalar):
-            raise ValueError(f"{message}; {name} must be finite")
-        if isinstance(temperature, Tensor):
-            _graph_safe_assert(limit * temperature >= abs(scalar), f"{message}; {name} is out of domain")
-        elif abs(scalar) > limit * float(temperature):
-            raise ValueError(f"{message}; {name} is out of domain")
-
-
-def _normalize_masked(value: Tensor, temperature: Param) -> Tensor:
-    """Replace -inf mask cells by retained_amax - ((L1+L2)*80 + 200)*T,
-    with amax and T detached (orihime/operator.py:334-386)."""
-    mask = torch.isneginf(value)
-    temp = _temperature_on(temperature, value)
-    if isinstance(temp, Tensor):
-        temp = temp.detach()
-    dims = tuple(range(1, value.ndim))
-    path_len = float(sum(value.shape[1:]))
-    offset = (path_len * _MAX_ABS_SCORE_TEMPERATURE_RATIO + _MASK_EXCLUSION_RATIO) * temp
-    retained = value.amax(dim=dims, keepdim=True)
-    retained = torch.where(torch.isfinite(retained), retained, retained.new_zeros(()))
-    sentinel = (retained.detach() - offset).to(dtype=value.dtype, device=value.device)
-    return torch.where(mask, sentinel, value)
+# the names the adapters and autograd Functions below use
+_validate_scalar_parameter_shape = _checks.scalar_shape
+_validate_temperature = _checks.temperature
+_validate_derivative_vector = _checks.derivative_vector
+_validate_numerical_domain = _checks.numerical_domain
+_normalize_masked = _checks.mask_to_sentinel
+_graph_safe_assert = _checks.require
 
 
 # --------------------------------------------------------------------------

@@ -5,8 +5,8 @@ One native call (``soft_nw_with_grads`` / ``soft_sv_linear_with_grads``)
 gives value, map and the per-pair gap and temperature gradients, so the
 backward role reuses them; value-only calls run the forward sweep alone.
 Derivatives of the map (HVP, parameter sensitivities, map VJP) run the
-Needleman-Wunsch forward-over-reverse pass (one gap direction moving both
-affine gap scores) or raise NotImplementedError (Saigo-Vert: no kernel yet).
+family's affine forward-over-reverse tangent sweeps with one gap direction
+moving both affine gap scores.
 """
 
 from __future__ import annotations

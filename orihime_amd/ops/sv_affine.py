@@ -6,8 +6,8 @@ gives value, map and the per-pair parameter gradients, so the backward role
 reuses them instead of running ``sv_affine_value_grad_params`` again
 (orihime/ops/sv_affine.py:221-235 does); value-only calls run the forward
 sweep alone.  Derivatives of the map (HVP, parameter sensitivities, map
-VJP) run the forward-over-reverse pass (Needleman-Wunsch) or raise
-NotImplementedError (Saigo-Vert: no kernel yet).
+VJP) run the Saigo-Vert forward-over-reverse tangent sweeps
+(``ohm_sv_affine_tangent``).
 """
 
 from __future__ import annotations

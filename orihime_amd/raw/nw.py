@@ -1,39 +1,11 @@
-"""Raw VJPs and kernel bindings for linear-gap Needleman-Wunsch (orihime/raw/nw.py)."""
+"""Raw tier of linear-gap Needleman-Wunsch (reference surface: orihime/raw/nw.py):
+the kernel bindings of ``ops/nw`` and parameter VJPs computed without an
+autograd graph."""
 
-from __future__ import annotations
-
-from torch import Tensor
-
+from ..families import raw_vjp
 from ..ops.nw import *  # noqa: F401,F403
-from ..ops.nw import __all__ as _kernel_all
-from ..nw import _nw_operator
-from ._base import VJPFields, _RawVJP
+from ..ops.nw import __all__ as _bindings
 
-vjp_fields: VJPFields = ("gap_score", "temperature")
-_INTERNAL = {"gap_score": "gap", "temperature": "temp"}
+vjp_fields, vjp_one, vjp = raw_vjp("nw")
 
-
-def _impl(pair_scores, *, wrt, cotangent, gap_score, temperature, lengths):
-    return _nw_operator._vjp_one(pair_scores, gap_score, temperature, grad_map=cotangent,
-                                 wrt=_INTERNAL[wrt], lengths=lengths)
-
-
-_raw = _RawVJP(vjp_fields=vjp_fields, vjp_one=_impl)
-
-
-def vjp_one(pair_scores: Tensor, *, wrt: str, cotangent: Tensor, gap_score: float | Tensor = 0.0,
-            temperature: float | Tensor = 1.0, lengths: Tensor | None = None) -> Tensor:
-    """Return one selected Needleman-Wunsch parameter VJP."""
-    return _raw.vjp_one(pair_scores, wrt=wrt, cotangent=cotangent, gap_score=gap_score,
-                        temperature=temperature, lengths=lengths)
-
-
-def vjp(pair_scores: Tensor, *, wrt: VJPFields, cotangent: Tensor,
-        gap_score: float | Tensor = 0.0, temperature: float | Tensor = 1.0,
-        lengths: Tensor | None = None) -> dict[str, Tensor]:
-    """Return selected Needleman-Wunsch parameter VJPs in a dictionary."""
-    return _raw.vjp(pair_scores, wrt=wrt, cotangent=cotangent, gap_score=gap_score,
-                    temperature=temperature, lengths=lengths)
-
-
-__all__ = ["vjp_fields", "vjp_one", "vjp", *_kernel_all]
+__all__ = ["vjp_fields", "vjp_one", "vjp", *_bindings]

@@ -1,39 +1,11 @@
-"""Raw VJPs and kernel bindings for Smith-Waterman (orihime/raw/sw.py)."""
+"""Raw tier of Smith-Waterman (reference surface: orihime/raw/sw.py):
+the kernel bindings of ``ops/sw`` and parameter VJPs computed without an
+autograd graph."""
 
-from __future__ import annotations
-
-from torch import Tensor
-
+from ..families import raw_vjp
 from ..ops.sw import *  # noqa: F401,F403
-from ..ops.sw import __all__ as _kernel_all
-from ..sw import _sw_operator
-from ._base import VJPFields, _RawVJP
+from ..ops.sw import __all__ as _bindings
 
-vjp_fields: VJPFields = ("gap_score", "temperature")
-_INTERNAL = {"gap_score": "gap", "temperature": "temp"}
+vjp_fields, vjp_one, vjp = raw_vjp("sw")
 
-
-def _impl(pair_scores, *, wrt, cotangent, gap_score, temperature, lengths):
-    return _sw_operator._vjp_one(pair_scores, gap_score, temperature, grad_map=cotangent,
-                                 wrt=_INTERNAL[wrt], lengths=lengths)
-
-
-_raw = _RawVJP(vjp_fields=vjp_fields, vjp_one=_impl)
-
-
-def vjp_one(pair_scores: Tensor, *, wrt: str, cotangent: Tensor, gap_score: float | Tensor = 0.0,
-            temperature: float | Tensor = 1.0, lengths: Tensor | None = None) -> Tensor:
-    """Return one selected Smith-Waterman parameter VJP."""
-    return _raw.vjp_one(pair_scores, wrt=wrt, cotangent=cotangent, gap_score=gap_score,
-                        temperature=temperature, lengths=lengths)
-
-
-def vjp(pair_scores: Tensor, *, wrt: VJPFields, cotangent: Tensor,
-        gap_score: float | Tensor = 0.0, temperature: float | Tensor = 1.0,
-        lengths: Tensor | None = None) -> dict[str, Tensor]:
-    """Return selected Smith-Waterman parameter VJPs in a dictionary."""
-    return _raw.vjp(pair_scores, wrt=wrt, cotangent=cotangent, gap_score=gap_score,
-                    temperature=temperature, lengths=lengths)
-
-
-__all__ = ["vjp_fields", "vjp_one", "vjp", *_kernel_all]
+__all__ = ["vjp_fields", "vjp_one", "vjp", *_bindings]

@@ -287,7 +287,7 @@ def test_lengths_and_prefix_masks():
     assert abs(v[1] - v1[0]) < 1e-5 and v[2] == 0
     mask1 = torch.arange(9)[None, :] < ln[:, 0:1]
     mask2 = torch.arange(7)[None, :] < ln[:, 1:2]
-    from orihime_amd.sw import _sw_affine_operator
+    from orihime_amd.families import engine; _sw_affine_operator = engine("sw_affine")
 
     vm = _sw_affine_operator.value(x, -1.0, -0.3, 0.9, mask1=mask1, mask2=mask2)
     assert torch.allclose(vm, v)
@@ -479,3 +479,76 @@ def test_tensor_parameter_gradients_reuse_the_forward_sweeps(affine, monkeypatch
     ref = vgp(x, *[float(p.detach()) for p in params], ln)
     for p, r in zip(params, ref):
         assert torch.allclose(p.grad, (w * r).sum().reshape(1), rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("family", ["sw_affine", "sw", "nw_affine", "sv_affine"])
+def test_value_only_backward_runs_no_second_order_pass(family, monkeypatch):
+    """A backward through the value alone of ``<family>_forward`` runs no
+    tangent (second-order) pass; a backward that also goes through the map
+    runs one (reference: tests/test_soft_sw_affine.py:1088-1115)."""
+    name = {"sw_affine": "affine_tangent", "sw": "linear_tangent",
+            "nw_affine": "_three_state_tangent", "sv_affine": "_three_state_tangent"}[family]
+    calls = []
+    orig = getattr(_lib, name)
+
+    def counted(*a, **k):
+        calls.append(1)
+        return orig(*a, **k)
+
+    monkeypatch.setattr(_lib, name, counted)
+    kern = ohm.ops._kernels[family]
+    params = (-1.0, 0.7) if family == "sw" else (-1.0, -0.3, 0.7)
+    g = torch.Generator().manual_seed(776)
+    x = torch.randn(1, 8, 9, generator=g).requires_grad_(True)
+    value, _ = kern.forward(x, *params, None)
+    value.sum().backward()
+    assert calls == [] and x.grad is not None
+    x.grad = None
+    value, posteriors = kern.forward(x, *params, None)
+    (value.sum() + 0.0 * posteriors.sum()).backward()
+    assert len(calls) >= 1
+
+
+def test_public_surface_signatures_docs_and_selection():
+    """Public functions keep the reference's frozen signatures and documented
+    arguments (tests/test_v3_contract.py:280-310, tests/test_public_docs.py:283-293);
+    named operations validate ``output`` (tests/test_public_operations.py:106-120)."""
+    import inspect
+
+    kw = inspect.Parameter.KEYWORD_ONLY
+    for fam, gaps in (("sw", ("gap_score",)), ("sw_affine", ("gap_open_score", "gap_extend_score")),
+                      ("nw", ("gap_score",)), ("nw_affine", ("gap_open_score", "gap_extend_score")),
+                      ("sv", ("gap_score",)), ("sv_affine", ("gap_open_score", "gap_extend_score"))):
+        for suffix in ("", "_value", "_entropy"):
+            fn = getattr(ohm, fam + suffix)
+            sig = inspect.signature(fn)
+            assert tuple(sig.parameters) == ("pair_scores", *gaps, "temperature", "lengths", "mask",
+                                             "dtype")
+            assert sig.parameters["pair_scores"].default is inspect.Parameter.empty
+            for g in gaps:
+                assert sig.parameters[g].kind is kw and sig.parameters[g].default == 0.0
+            assert sig.parameters["temperature"].default == 1.0
+            assert all(sig.parameters[n].default is None for n in ("lengths", "mask", "dtype"))
+            doc = inspect.getdoc(fn)
+            for part in ("Args:", "Returns:", "Raises:", *(f"{n}:" for n in sig.parameters)):
+                assert part in doc, (fam + suffix, part)
+            assert fn.__name__ == fam + suffix
+        op = getattr(ohm.ops, fam)
+        assert op.backward_fields == ("pair_scores", *gaps, "temperature")
+        assert tuple(inspect.signature(op.backward).parameters)[-2:] == ("grad_map", "output")
+    assert tuple(ohm.ops.__all__) == ("sw", "sw_affine", "nw_affine", "nw", "sv", "sv_affine")
+    x = torch.randn(1, 3, 3)
+    with pytest.raises(ValueError, match="unknown output field"):
+        ohm.ops.sw.forward(x, output="scores")
+    with pytest.raises(ValueError, match="at least one"):
+        ohm.ops.sw.sensitivity(x, output=())
+    with pytest.raises(ValueError, match="duplicate"):
+        ohm.ops.sw.backward(x, grad_map=torch.ones_like(x), output=("temperature", "temperature"))
+    with pytest.raises(TypeError, match="output must be"):
+        ohm.ops.sw.forward(x, output=object())
+    with pytest.raises(TypeError, match="unexpected keyword"):
+        ohm.sw(x, gap_open_score=-1.0)
+    assert tuple(inspect.signature(ohm.nn.SmithWatermanAffine).parameters) == (
+        "gap_open_score", "gap_extend_score", "temperature", "learnable", "device", "dtype")
+    layer = ohm.nn.NW(gap_score=-0.5)
+    assert "gap_score" in dict(layer.named_buffers()) and "learnable=()" in repr(layer)

@@ -14,7 +14,9 @@ from __future__ import annotations
 import pytest
 import torch
 
-from orihime_amd.ops import sw_affine as sw_affine_ops
+import orihime_amd
+
+sw_affine_ops = orihime_amd.ops._kernels["sw_affine"]
 
 pytestmark = pytest.mark.gpu
 

@@ -231,7 +231,7 @@ def test_prefix_masks_equal_lengths():
     mask1 = torch.arange(L1)[None, :] < ln[:, 0:1]
     mask2 = torch.arange(L2)[None, :] < ln[:, 1:2]
     v = ohm.sv_affine_value(x, gap_open_score=go, gap_extend_score=ge, temperature=T, lengths=ln)
-    from orihime_amd.sv import _sv_affine_operator
+    from orihime_amd.families import engine; _sv_affine_operator = engine("sv_affine")
     vm = _sv_affine_operator.value(x, go, ge, T, mask1=mask1, mask2=mask2)
     assert torch.equal(v, vm)
     vg = ohm.ops._kernels["sv_affine"].value_grad_params(x, go, ge, T, ln)
