@@ -102,6 +102,24 @@ def fixed_batch(torch, seed=1, B=256, L=512):
     return torch.randn(B, L, L, generator=g) * 2 - 0.5
 
 
+def long_batch(torch, seed=3, B=16, L=4096):
+    """(scores, direction), both N(0,1), drawn in that order."""
+    g = torch.Generator().manual_seed(seed)
+    x = torch.randn(B, L, L, generator=g)
+    return x, torch.randn(B, L, L, generator=g)
+
+
+def embed_inputs(torch, B=512, seed=4, lo=128, hi=768, D=128):
+    """Q [B, hi, D], K [B, hi, D] BF16 N(0,1)/sqrt(D) (rows past each pair's
+    length are drawn too and ignored), lengths [B, 2] int32 ~ U[lo, hi];
+    lengths drawn first."""
+    g = torch.Generator().manual_seed(seed)
+    ln = torch.randint(lo, hi + 1, (B, 2), generator=g).to(torch.int32)
+    Q = (torch.randn(B, hi, D, generator=g) / D ** 0.5).to(torch.bfloat16)
+    K = (torch.randn(B, hi, D, generator=g) / D ** 0.5).to(torch.bfloat16)
+    return Q, K, ln
+
+
 # --------------------------------------------------------------------------
 # HIP events through the runtime torch already loaded
 # --------------------------------------------------------------------------
@@ -302,10 +320,28 @@ def kernel_durations(torch, lib, hip, launch, steps):
 # --------------------------------------------------------------------------
 
 
-def cpu_baseline(torch, pairs, params, seconds, family="sw"):
-    """The reference's CPU kernels (oracle/_ref; the restatement when they are
-    not built) on the first pairs of the batch, about `seconds` of work.
-    family: "sw" (Smith-Waterman affine), "nw" or "sv" (affine)."""
+def cpu_model() -> str:
+    """The host CPU's model name (lscpu), or the platform's processor string."""
+    import platform
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return platform.processor() or "unknown"
+
+
+def host_threads() -> int:
+    """Threads the CPU baseline may use: the host's cores, at most 16 (one
+    GPU's share of a shared host)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _cpu_run(torch, pairs, params, seconds, family, threads):
     import numpy as np
 
     from oracle import oracle as O
@@ -313,13 +349,12 @@ def cpu_baseline(torch, pairs, params, seconds, family="sw"):
     run = {"sw": O.sw_affine_forward, "nw": O.nw_affine_forward, "sv": O.sv_affine_forward}[family]
     built = {"sw": O.ref_available, "nw": O.nw_ref_available, "sv": O.sv_ref_available}[family]
     kind = "reference" if built() else "port"
-    threads = min(16, os.cpu_count() or 1) if kind == "reference" else 1
     old = torch.get_num_threads()
     torch.set_num_threads(threads)
     go, ge, T = params
     cells = done = 0
     elapsed = 0.0
-    chunk = 32
+    chunk = max(1, threads)
     try:
         while done < len(pairs) and elapsed < seconds:
             part = pairs[done:done + chunk]
@@ -331,38 +366,242 @@ def cpu_baseline(torch, pairs, params, seconds, family="sw"):
                 pad[i, :p.shape[0], :p.shape[1]] = p
                 ln[i] = p.shape
             t0 = time.perf_counter()
-            run(pad, go, ge, T, ln, impl="ref" if kind == "reference" else "oracle")
+            if kind == "reference":
+                run(pad, go, ge, T, ln, impl="ref")
+            else:  # the single-threaded restatement: threads over pairs
+                O.over_pairs(run, list(part), go, ge, T, threads=threads, chunk=1)
             elapsed += time.perf_counter() - t0
             cells += int((ln[:, 0].astype(np.int64) * ln[:, 1]).sum())
             done += len(part)
     finally:
         torch.set_num_threads(old)
+    return kind, cells, done, elapsed
+
+
+def cpu_baseline(torch, pairs, params, seconds, family="sw"):
+    """The reference's CPU kernels (oracle/_ref; the restatement when they are
+    not built) on the first pairs of the batch: about `seconds` of work on the
+    host's threads (at most 16), then a quarter of that on one thread.
+    family: "sw" (Smith-Waterman affine), "nw" or "sv" (affine)."""
+    threads = host_threads()
+    kind, cells, done, elapsed = _cpu_run(torch, pairs, params, seconds, family, threads)
+    _, cells1, done1, elapsed1 = _cpu_run(torch, pairs, params, seconds / 4, family, 1)
+    what = "forward+backward (value, map, gap and temperature gradients)"
     return {
         "value": round(cells / elapsed / 1e9, 6),
         "unit": "GCUPS",
         "cores": threads,
+        "threads": threads,
         "kind": kind,
-        "sample": (f"first {done} of {len(pairs)} pairs ({cells} cells), forward+backward "
-                   f"(value, map, gap and temperature gradients), {elapsed:.1f} s"),
+        "cpu_model": cpu_model(),
+        "host_cpus": os.cpu_count(),
+        "sample": f"first {done} of {len(pairs)} pairs ({cells} cells), {what}, {elapsed:.1f} s",
+        "single_thread": {
+            "value": round(cells1 / elapsed1 / 1e9, 6), "unit": "GCUPS", "threads": 1,
+            "sample": f"first {done1} pairs ({cells1} cells), {what}, {elapsed1:.1f} s"},
     }
 
 
-# --------------------------------------------------------------------------
+def hbm_copy_peak(torch, hip, nbytes=2 << 30, reps=10) -> float:
+    """HBM bandwidth of a device-to-device copy on this GPU (GB/s, bytes read
+    plus bytes written), best of `reps`."""
+    src = torch.ones(nbytes // 4, device="cuda")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream().cuda_stream
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    best = None
+    for _ in range(reps):
+        a, b = hip.event(), hip.event()
+        hip.record(a, stream)
+        dst.copy_(src)
+        hip.record(b, stream)
+        ms = hip.elapsed_ms(a, b)
+        hip.destroy(a)
+        hip.destroy(b)
+        best = ms if best is None else min(best, ms)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (best * 1e-3) / 1e9
 
 
-def embed_workload(torch, args, dev):
+def oracle_errors(pairs, outputs, params, n_values=256, n_maps=32):
+    """Errors of a packed launch's results against the FP64 oracle on pairs
+    spread over the size range (outside any timed region).  outputs: value
+    [B] and the per-pair maps (numpy)."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    value, maps = outputs
+    order = np.argsort([p.size for p in pairs], kind="stable")
+    pick = lambda n: sorted(int(order[k]) for k in
+                            np.linspace(0, len(order) - 1, min(n, len(order))).round().astype(int))
+    iv, im = pick(n_values), pick(n_maps)
+    go, ge, T = params
+    f64v = np.asarray(O.over_pairs(O.sw_affine_value, [pairs[i] for i in iv], go, ge, T,
+                                   prec="f64", threads=host_threads(), chunk=4)[0])
+    got = np.asarray(value, np.float64)[iv]
+    vrel = float(np.max(np.abs(got - f64v) / np.maximum(np.abs(f64v), 1e-3)))
+    _, f64m, *_ = O.over_pairs(O.sw_affine_forward, [pairs[i] for i in im], go, ge, T,
+                               prec="f64", threads=host_threads(), chunk=1)
+    mabs = max(float(np.max(np.abs(maps[i] - m), initial=0.0)) for i, m in zip(im, f64m))
+    return {"value_rel_max": vrel, "map_abs_max": mabs, "values_checked": len(iv),
+            "maps_checked": len(im), "against": "FP64 oracle",
+            "pairs": "spread over the size range (evenly spaced ranks of cell count)"}
+
+
+def _padded_oracle_errors(batch, outputs, params, idx):
+    """Value / map errors of a padded launch against the FP64 oracle."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    go, ge, T = params
+    sub = batch[idx].astype(np.float64)
+    f64 = O.over_pairs(O.sw_affine_forward, [x for x in sub], go, ge, T, prec="f64",
+                       threads=host_threads(), chunk=1)
+    v, m = outputs
+    vrel = float(np.max(np.abs(v[idx] - np.asarray(f64[0])) / np.maximum(np.abs(np.asarray(f64[0])), 1e-3)))
+    mabs = float(max(np.max(np.abs(m[k] - f64[1][k])) for k in range(len(idx))))
+    return {"value_rel_max": vrel, "map_abs_max": mabs, "values_checked": len(idx),
+            "maps_checked": len(idx), "against": "FP64 oracle", "pairs": list(map(int, idx))}
+
+
+def fixed_record(torch, lib, hip, args, dev):
+    """[256, 512, 512] (N(0,1)*2 - 0.5, seed 1; go=-11, ge=-1, T=1): fwd+bwd
+    step (graph replay), its kernel durations, sampled FP64 errors."""
+    params = (-11.0, -1.0, 1.0)
+    x = fixed_batch(torch)
+    launch = Launch(torch, lib, x.to(dev), None, None, 512, 512, params)
+    tuned = _tune(torch, launch, params, args)
+    for _ in range(max(1, args.warmup)):
+        launch.step()
+    torch.cuda.synchronize()
+    launch.capture()
+    ms, _ = timed(torch, launch.graph.replay, args.steps)
+    kd = kernel_durations(torch, lib, hip, launch, max(3, args.steps // 4))
+    cells = x.numel()
+    idx = [0, 97, 255]
+    err = _padded_oracle_errors(x.numpy(), (launch.value.cpu().numpy(),
+                                            launch.map[idx].cpu().numpy()), params, idx)
+    log(f"fixed 256x512^2 fwd+bwd: {ms:.3f} ms/step, {cells / ms / 1e6:.1f} GCUPS")
+    return {"workload": "fixed-shape affine soft-SW fwd+bwd, [256, 512, 512] N(0,1)*2-0.5 (seed 1), "
+                        "go=-11 ge=-1 T=1",
+            "cells": cells, "ms_per_step": round(ms, 4), "gcups": round(cells / ms / 1e6, 3),
+            "kernel_ms": {k: round(v, 4) for k, v in kd.items()}, "hip_graph": True,
+            "launch_config": tuned, "oracle_err": err}
+
+
+def long_record(torch, lib, hip, args, dev):
+    """[16, 4096, 4096] (N(0,1), seed 3; go=-11, ge=-1, T=1): fwd+bwd step
+    with the checkpointed reverse, the HVP along an N(0,1) direction, peak
+    HBM against a reverse that reads a full M/I/D table, FP64 errors of pair
+    0 (value, central 64 x 64 map crop)."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    params = (-11.0, -1.0, 1.0)
+    x, V = long_batch(torch)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    base = torch.cuda.memory_allocated()
+    torch.cuda.reset_peak_memory_stats()
+    launch = Launch(torch, lib, x.to(dev), None, None, 4096, 4096, params)
+    tuned = _tune(torch, launch, params, args)
+    for _ in range(max(1, args.warmup)):
+        launch.step()
+    torch.cuda.synchronize()
+    steps = max(3, args.steps // 4)
+    ms, _ = timed(torch, launch.step, steps)
+    kd = kernel_durations(torch, lib, hip, launch, steps)
+    peak = torch.cuda.max_memory_allocated() - base
+    strips = 4096 // 64
+    full_table = 16 * strips * (4096 + 63) * 64 * 3 * 4  # M, I, D per cell slot
+    bnd = 16 * strips * 4096 * 8 * 4
+    ckpt_ws = launch.ws.numel()
+    cells = x.numel()
+    Vd = V.to(dev)
+    mt = torch.empty_like(launch.scores)
+    tws = torch.empty(lib.workspace_bytes(True, lib.OP_TANGENT, 16, 4096, 4096, -1),
+                      dtype=torch.uint8, device=dev)
+    hvp = lambda: lib.affine_tangent(launch.scores, None, None, *params, direction=Vd,
+                                     map_tangent=mt, workspace=tws)
+    hvp()
+    torch.cuda.synchronize()
+    hms, _ = timed(torch, hvp, steps)
+    sl = slice(2016, 2080)
+    x0 = x[0].numpy().astype(np.float64)
+    f64 = O.sw_affine_forward(x0[None], *params, prec="f64")
+    v0 = float(launch.value[0])
+    crop = launch.map[0, sl, sl].cpu().numpy()
+    err = {"value_rel_max": abs(v0 - float(f64[0][0])) / abs(float(f64[0][0])),
+           "map_abs_max": float(np.max(np.abs(crop - f64[1][0, sl, sl]))),
+           "values_checked": 1, "maps_checked": "pair 0, central 64 x 64 crop",
+           "against": "FP64 oracle"}
+    del Vd, mt, tws, launch
+    torch.cuda.empty_cache()
+    log(f"long 16x4096^2 fwd+bwd: {ms:.3f} ms/step, HVP {hms:.3f} ms, peak HBM {peak / 1e9:.2f} GB")
+    return {"workload": "long pairs affine soft-SW, [16, 4096, 4096] N(0,1) (seed 3), go=-11 "
+                        "ge=-1 T=1: fwd+bwd with the checkpointed reverse, and the HVP",
+            "cells": cells, "ms_per_step": round(ms, 4), "gcups": round(cells / ms / 1e6, 3),
+            "kernel_ms": {k: round(v, 4) for k, v in kd.items()}, "launch_config": tuned,
+            "hvp_ms": round(hms, 4), "hvp_gcups": round(cells / hms / 1e6, 3),
+            "peak_hbm_bytes": int(peak), "workspace_bytes": int(ckpt_ws),
+            "full_table_workspace_bytes": int(full_table + bnd),
+            "full_table_peak_bytes": int(peak - ckpt_ws + full_table + bnd),
+            "oracle_err": err}
+
+
+def _embed_parts(torch, hip, Q, K, lnd, params, steps):
+    """Mean ms of the embedding-scored step's parts, each timed alone with
+    HIP events: scores (ragged MFMA), sweeps (value + map), dQ / dK (ragged
+    MFMA, transposed copies included)."""
+    from orihime_amd import _lib, embed
+
+    stream = torch.cuda.current_stream().cuda_stream
+    go, ge, T = params
+    M, N = Q.shape[1], K.shape[1]
+    out = {}
+    with torch.no_grad():
+        scores, offsets = embed.pair_scores(Q, K, lnd)
+        _, post = embed._sweep(scores, lnd, offsets, M, N, go, ge, T)
+        gv = torch.ones(Q.shape[0], device=Q.device)
+        dQ, dK = torch.empty_like(Q), torch.empty_like(K)
+        parts = {
+            "scores": lambda: _lib.embed_scores(Q, K, lnd, offsets, scores),
+            "sweeps": lambda: embed._sweep(scores, lnd, offsets, M, N, go, ge, T),
+            "dq_dk": lambda: _lib.embed_grads(Q, K, lnd, offsets, post, gv, dQ, dK),
+        }
+        for name, fn in parts.items():
+            fn()
+            torch.cuda.synchronize()
+            a, b = hip.event(), hip.event()
+            hip.record(a, stream)
+            for _ in range(steps):
+                fn()
+            hip.record(b, stream)
+            out[name] = round(hip.elapsed_ms(a, b) / steps, 4)
+            hip.destroy(a)
+            hip.destroy(b)
+    return out
+
+
+def embed_record(torch, hip, args, dev):
     """Embedding-scored alignment end to end: Q [512, M, 128], K [512, N, 128]
-    BF16 N(0,1)/sqrt(128), M, N ~ U[128, 768] (seed 4); scores Q K^T by a BF16
-    GEMM with FP32 accumulation, ragged soft-SW affine (go=-4, ge=-0.5,
-    T=0.25, per-pair lengths on padded storage), dQ and dK through the map.
-    A step is the forward (GEMM + sweeps) and the backward (dQ, dK GEMMs)."""
+    BF16 N(0,1)/sqrt(128), M, N ~ U[128, 768] (seed 4); scores Q K^T (BF16
+    products, FP32 accumulation), ragged soft-SW affine (go=-4, ge=-0.5,
+    T=0.25), dQ and dK through the map.  A step is the forward and the
+    backward."""
+    import numpy as np
+
+    from oracle import oracle as O
     from orihime_amd.embed import soft_align_embeddings
-    gen = torch.Generator().manual_seed(4)
-    B, Lmax, D = 512, 768, 128
-    ln = torch.randint(128, Lmax + 1, (B, 2), generator=gen).to(torch.int32)
-    Q = (torch.randn(B, Lmax, D, generator=gen) / D ** 0.5).to(torch.bfloat16).to(dev)
-    K = (torch.randn(B, Lmax, D, generator=gen) / D ** 0.5).to(torch.bfloat16).to(dev)
-    lnd = ln.to(dev)
+
+    params = (-4.0, -0.5, 0.25)
+    Qc, Kc, ln = embed_inputs(torch)
+    Q, K, lnd = Qc.to(dev), Kc.to(dev), ln.to(dev)
     cells = int((ln[:, 0].long() * ln[:, 1].long()).sum())
     Q.requires_grad_(True)
     K.requires_grad_(True)
@@ -370,39 +609,76 @@ def embed_workload(torch, args, dev):
     def step():
         Q.grad = None
         K.grad = None
-        soft_align_embeddings(Q, K, lnd, -4.0, -0.5, 0.25).sum().backward()
+        soft_align_embeddings(Q, K, lnd, *params).sum().backward()
 
+    from orihime_amd import embed, tune
     tuned = "automatic"
+    tune.reset()
     if not args.no_tune:
-        from orihime_amd import tune
-        from orihime_amd.embed import pair_scores
         with torch.no_grad():
-            cfg = tune.autotune(pair_scores(Q, K), -4.0, -0.5, 0.25, lengths=lnd)
+            scores, offsets = embed.pair_scores(Q, K, lnd)
+            cfg = tune.autotune(scores, *params, lengths=lnd, offsets=offsets)
+            del scores, offsets
         tuned = {"waves": cfg.waves, "groups_per_cu": cfg.groups_per_cu,
                  "teams": {0: "auto", 1: "on", 2: "off", 3: "grid", 4: "fused"}[cfg.teams]}
-        log(f"autotuned launch shape: {tuned}")
-
     for _ in range(max(1, args.warmup)):
         step()
     torch.cuda.synchronize()
-    ms, wall = timed(torch, step, args.steps)
-    gflop = 3 * 2.0 * B * Lmax * Lmax * D / 1e9  # scores, dQ, dK on padded storage
-    gcups = cells / (ms * 1e-3) / 1e9
-    log(f"embedding-scored fwd+bwd: {ms:.3f} ms/step, {gcups:.1f} GCUPS (sweep cells), "
-        f"{gflop / ms:.1f} TFLOP/s of GEMM work per step time")
+    ms, _ = timed(torch, step, args.steps)
+    parts = _embed_parts(torch, hip, Q.detach(), K.detach(), lnd, params, max(3, args.steps // 2))
+    # FP64 errors of 16 pairs spread over the size range
+    v = soft_align_embeddings(Q, K, lnd, *params)
+    v.sum().backward()
+    vh, dQ, dK = v.detach().cpu().numpy(), Q.grad.float().cpu().numpy(), K.grad.float().cpu().numpy()
+    lnh = ln.numpy()
+    order = np.argsort(lnh[:, 0].astype(np.int64) * lnh[:, 1], kind="stable")
+    idx = sorted(int(order[k]) for k in np.linspace(0, len(order) - 1, 16).round().astype(int))
+    Q64, K64 = Qc.double().numpy(), Kc.double().numpy()
+    S = [Q64[b, :lnh[b, 0]] @ K64[b, :lnh[b, 1]].T for b in idx]
+    val, mp, *_ = O.over_pairs(O.sw_affine_forward, S, *params, prec="f64",
+                               threads=host_threads(), chunk=1)
+    vrel = float(np.max(np.abs(vh[idx] - np.asarray(val)) / np.abs(np.asarray(val))))
+    gq = gk = scale = 0.0
+    for k, b in enumerate(idx):
+        m, n = lnh[b]
+        tq, tk = mp[k] @ K64[b, :n], mp[k].T @ Q64[b, :m]
+        scale = max(scale, float(np.abs(tq).max()), float(np.abs(tk).max()))
+        gq = max(gq, float(np.abs(dQ[b, :m] - tq).max()))
+        gk = max(gk, float(np.abs(dK[b, :n] - tk).max()))
+    gcups = cells / ms / 1e6
+    log(f"embedding-scored fwd+bwd: {ms:.3f} ms/step, {gcups:.1f} GCUPS; parts {parts}")
+    return {"workload": "embedding-scored ragged soft-SW affine, Q/K [512, <=768, 128] BF16, "
+                        "M,N ~ U[128,768] (seed 4), go=-4 ge=-0.5 T=0.25: scores, sweeps, dQ/dK",
+            "cells": cells, "ms_per_step": round(ms, 4), "gcups": round(gcups, 3),
+            "kernel_ms": parts, "launch_config": tuned,
+            "gemm_gflop_per_step_useful": round(3 * 2.0 * 128 * cells / 1e9, 2),
+            "oracle_err": {"value_rel_max": vrel, "dq_abs_max": gq, "dk_abs_max": gk,
+                           "grad_scale": scale, "pairs_checked": len(idx),
+                           "against": "FP64 oracle on the same BF16 values"}}
+
+
+def embed_workload(torch, args, dev):
+    """``--workload embed``: the embedding-scored record as its own line."""
+    rec = embed_record(torch, Hip(), args, dev)
     line = {
         "metric": "GCUPS embedding-scored soft-SW-affine fwd+bwd (Q K^T, sweeps, dQ/dK), 1 GPU",
-        "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "value": rec["gcups"], "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": rec["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "bf16 GEMM inputs, f32 sweeps",
-        "data": "synthetic",
-        "config": {"workload": "embedding-scored ragged soft-SW affine, Q/K [512, <=768, 128] "
-                               "BF16, M,N ~ U[128,768] (seed 4), go=-4 ge=-0.5 T=0.25",
-                   "cells": cells, "gemm_gflop_per_step": round(gflop, 1), "launch_config": tuned,
-                   "wall_ms_per_step": round(wall, 4)},
-        "roofline": None, "cpu_baseline": None,
+        "data": "synthetic", "config": rec, "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
+
+
+def _tune(torch, launch, params, args):
+    if args.no_tune:
+        return "automatic"
+    from orihime_amd import tune
+    t0 = time.perf_counter()
+    cfg = tune.autotune(launch.scores, *params, lengths=launch.lengths, offsets=launch.offsets)
+    return {"waves": cfg.waves, "groups_per_cu": cfg.groups_per_cu,
+            "teams": {0: "auto", 1: "on", 2: "off", 3: "grid", 4: "fused"}[cfg.teams],
+            "tuning_s": round(time.perf_counter() - t0, 2)}
 
 
 def nw_traffic(kernel: str, family: str = "nw"):
@@ -515,6 +791,9 @@ def main():
                     default="ragged")
     ap.add_argument("--no-padded", action="store_true", help="skip the padded comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the fixed, long and embedding-scored sub-records")
+    ap.add_argument("--no-oracle", action="store_true", help="skip the FP64 oracle errors")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-graph", action="store_true", help="time eager calls instead of a graph")
     ap.add_argument("--no-tune", action="store_true",
@@ -549,6 +828,12 @@ def main():
         del src, dst
         log("copy probe: 1 GiB read + 1 GiB written")
 
+    if args.workload == "embed":
+        return embed_workload(torch, args, dev)
+    if args.workload in ("nw", "sv"):
+        return nw_workload(torch, args, dev, Hip(), args.workload)
+
+    pairs = None
     if args.workload == "ragged":
         params = (-8.0, -0.5, 0.5)
         t0 = time.perf_counter()
@@ -569,17 +854,10 @@ def main():
         main_launch = Launch(torch, _lib, x.to(dev), None, None, 512, 512, params)
         workload = "fixed-shape affine soft-SW fwd+bwd, [256, 512, 512] (seed 1), go=-11 ge=-1 T=1"
         cpu_pairs = [x[i].numpy() for i in range(x.shape[0])]
-    elif args.workload == "embed":
-        return embed_workload(torch, args, dev)
-    elif args.workload in ("nw", "sv"):
-        return nw_workload(torch, args, dev, Hip(), args.workload)
     else:
         params = (-11.0, -1.0, 1.0)
-        gen = torch.Generator().manual_seed(3)
-        x = torch.randn(16, 4096, 4096, generator=gen)
-        V = torch.randn(16, 4096, 4096, generator=gen)
+        x, _ = long_batch(torch)
         cells = x.numel()
-        torch.cuda.reset_peak_memory_stats()
         main_launch = Launch(torch, _lib, x.to(dev), None, None, 4096, 4096, params)
         workload = ("long pairs affine soft-SW fwd+bwd, [16, 4096, 4096] N(0,1) (seed 3), "
                     "go=-11 ge=-1 T=1, checkpointed reverse sweep")
@@ -587,16 +865,8 @@ def main():
 
     # launch shape: autotuned on this batch (orihime_amd.tune), results do
     # not depend on it
-    tuned = None
-    if not args.no_tune:
-        from orihime_amd import tune
-        t0 = time.perf_counter()
-        cfg = tune.autotune(main_launch.scores, *params, lengths=main_launch.lengths,
-                            offsets=main_launch.offsets)
-        tuned = {"waves": cfg.waves, "groups_per_cu": cfg.groups_per_cu,
-                 "teams": {0: "auto", 1: "on", 2: "off", 3: "grid", 4: "fused"}[cfg.teams],
-                 "tuning_s": round(time.perf_counter() - t0, 2)}
-        log(f"autotuned launch shape: {tuned}")
+    tuned = _tune(torch, main_launch, params, args)
+    log(f"launch shape: {tuned}")
 
     # warm-up, sanity, graph
     for _ in range(max(1, args.warmup)):
@@ -628,10 +898,12 @@ def main():
     hip = Hip()
     rates = measured_rates(torch, hip)
     log("measured issue rates (T lane-ops/s): " + ", ".join(f"{k} {v}" for k, v in rates.items()))
+    hbm_peak = hbm_copy_peak(torch, hip)
+    log(f"measured HBM copy bandwidth: {hbm_peak:.0f} GB/s")
     kd = kernel_durations(torch, _lib, hip, main_launch, args.steps)
     log("kernel durations (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in kd.items()))
-    fused = (tuned is not None and tuned["teams"] == "fused") or (
-        tuned is None and os.environ.get("OHM_FUSED") == "1")
+    fused = isinstance(tuned, dict) and tuned["teams"] == "fused" or (
+        not isinstance(tuned, dict) and os.environ.get("OHM_FUSED") == "1")
     if fused:
         # the fused launch runs between the plan and forward marks
         kd["fused"] = kd["forward"]
@@ -639,42 +911,55 @@ def main():
     else:
         dom = max(("forward", "reverse"), key=lambda k: kd[k])
     t_dom = kd[dom] * 1e-3
-    ach_gbs = BYTES_PER_CELL[dom] * cells / t_dom / 1e9
-    ach_trans = TRANS_PER_CELL[dom] * cells / t_dom
-    # transcendental peak: measured on this device (v_exp_f32 issue rate),
-    # the quarter-rate assumption only when the probe is unavailable
     trans_peak = rates["exp"] * 1e12 if rates else TRANS_PEAK
     sweeps = ("forward", "reverse")  # one step = one forward and one reverse sweep per cell
-    step_roof = min(HBM_PEAK_GBS * 1e9 / sum(BYTES_PER_CELL[k] for k in sweeps),
-                    trans_peak / sum(TRANS_PER_CELL[k] for k in sweeps)) / 1e9
+    step_trans = sum(TRANS_PER_CELL[k] for k in sweeps)
+    step_bytes = sum(BYTES_PER_CELL[k] for k in sweeps)
+    step_roof = min(hbm_peak * 1e9 / step_bytes, trans_peak / step_trans) / 1e9
     wl_name = "ragged2048" if args.workload == "ragged" else args.workload
     traffic, traffic_src = profiled_traffic(wl_name, dom)
     valu, valu_src = profiled_valu(wl_name, dom)
-    # full-rate VALU issue measured on this device (wave-level instructions/s)
     issue_peak = max(rates["fma"], rates["dpp"]) * 1e12 / 64 if rates else VALU_LANE_OPS / 64
+    step_achieved = step_trans * cells / (ms * 1e-3)
+    dom_hbm = BYTES_PER_CELL[dom] * cells / t_dom / 1e9
     roofline = {
-        "bound": "hbm",
-        "achieved": round(ach_gbs, 2),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(ach_gbs / HBM_PEAK_GBS, 4),
+        # binding roof of the step: the reference's transcendentals per cell
+        # (22 for a forward and a reverse sweep) against the measured exp rate
+        "bound": "transcendental",
+        "achieved": round(step_achieved / 1e12, 4),
+        "peak": round(trans_peak / 1e12, 4),
+        "unit": "Top/s",
+        "frac": round(step_achieved / trans_peak, 4),
         "traffic": traffic,
-        "traffic_source": traffic_src,
-        "algorithmic_bytes": BYTES_PER_CELL[dom] * cells,
-        "kernel": KERNEL_NAMES[dom],
-        "kernel_ms": round(kd[dom], 4),
-        "binding": {
-            "bound": "transcendental",
-            "per_cell": TRANS_PER_CELL[dom],
-            "achieved": round(ach_trans / 1e12, 4),
-            "peak": round(trans_peak / 1e12, 4),
-            "peak_source": "measured v_exp_f32 issue rate" if rates else "quarter-rate assumption",
-            "unit": "Top/s",
-            "frac": round(ach_trans / trans_peak, 4),
+        "per_cell": step_trans,
+        "peak_source": "measured v_exp_f32 issue rate" if rates else "quarter-rate assumption",
+        "scope": "whole step (graph replay), " + ("one fused forward+reverse launch" if fused
+                                                 else "forward and reverse launches"),
+        "step_roof_gcups": round(step_roof, 1),
+        "step_frac": round(gcups / step_roof, 4),
+        "hbm": {
+            "bound": "hbm",
+            "kernel": KERNEL_NAMES[dom],
+            "kernel_ms": round(kd[dom], 4),
+            "algorithmic_bytes": BYTES_PER_CELL[dom] * cells,
+            "achieved": round(dom_hbm, 2),
+            "peak": round(hbm_peak, 1),
+            "peak_source": "measured device-to-device copy (bytes read + written)",
+            "peak_spec": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(dom_hbm / hbm_peak, 4),
+            "traffic": traffic,
+            "traffic_source": traffic_src,
+            "traffic_frac": None if traffic is None else round(traffic / t_dom / (hbm_peak * 1e9), 4),
+        },
+        "dominant_kernel": {
+            "kernel": KERNEL_NAMES[dom],
+            "kernel_ms": round(kd[dom], 4),
+            "transcendental_per_cell": TRANS_PER_CELL[dom],
+            "transcendental_frac": round(TRANS_PER_CELL[dom] * cells / t_dom / trans_peak, 4),
         },
         # what binds these kernels: VALU issue (profiled instruction count of
-        # the dominant kernel against the measured full-rate issue) and the
-        # measured HBM traffic against 8 TB/s
+        # the dominant kernel against the measured full-rate issue)
         "issue": None if valu is None else {
             "bound": "valu",
             "instructions": valu,
@@ -684,13 +969,16 @@ def main():
             "unit": "G wave-instructions/s",
             "frac": round(valu / t_dom / issue_peak, 4),
         },
-        "traffic_frac": None if traffic is None else round(traffic / t_dom / (HBM_PEAK_GBS * 1e9), 4),
         "measured_issue_rates_tops": rates,
-        "step_roof_gcups": round(step_roof, 1),
-        "step_frac": round(gcups / step_roof, 4),
     }
 
     extra = {}
+    if pairs is not None and not args.no_oracle:
+        t0 = time.perf_counter()
+        maps = [m.cpu().numpy() for m in ragged.unpack(main_launch.map, lengths, offsets)]
+        extra["oracle_err"] = oracle_errors(cpu_pairs, (main_launch.value.cpu().numpy(), maps), params)
+        log(f"FP64 oracle errors ({time.perf_counter() - t0:.1f} s): {extra['oracle_err']}")
+        del maps
     if args.workload == "ragged" and not args.no_padded:
         # the same batch padded to [B, 1024, 1024], cells beyond the lengths masked
         from orihime_amd.operator import _normalize_masked
@@ -701,10 +989,7 @@ def main():
             pad[i, :p.shape[0], :p.shape[1]] = p
         pad = _normalize_masked(pad, params[2]).contiguous()
         padded = Launch(torch, _lib, pad, None, None, L1max, L2max, params)
-        if not args.no_tune:
-            pcfg = tune.autotune(pad, *params)
-            extra["padded_launch_config"] = {"waves": pcfg.waves, "groups_per_cu": pcfg.groups_per_cu,
-                                             "teams": pcfg.teams}
+        extra["padded_launch_config"] = _tune(torch, padded, params, args)
         for _ in range(max(1, args.warmup)):
             padded.step()
         torch.cuda.synchronize()
@@ -719,39 +1004,34 @@ def main():
         extra.update({"padded_ms_per_step": round(pms, 4), "padded_gcups": round(pg, 2),
                       "ragged_over_padded": round(gcups / pg, 3),
                       "padded_vs_packed_value_rel_diff": vdiff})
+        if not args.no_oracle:
+            idx = [0, 1000, B - 1]
+            extra["padded_oracle_err"] = _padded_oracle_errors(
+                pad[idx].cpu().numpy(), (padded.value.cpu().numpy()[idx], padded.map[idx].cpu().numpy()),
+                params, list(range(3)))
+            extra["padded_oracle_err"]["pairs"] = idx
         del padded, pad
         torch.cuda.empty_cache()
 
     if args.workload == "long":
-        # peak HBM of the step against a reverse sweep that reads a full table
-        peak = torch.cuda.max_memory_allocated()
-        strips = 4096 // 64
-        full_table = 16 * strips * (4096 + 63) * 64 * 3 * 4  # M, I, D per cell slot
-        ckpt_ws = main_launch.ws.numel()
-        bnd = 16 * strips * 4096 * 8 * 4
-        extra.update({"peak_hbm_bytes": int(peak), "workspace_bytes": int(ckpt_ws),
-                      "full_table_workspace_bytes": int(full_table + bnd),
-                      "full_table_peak_bytes": int(peak - ckpt_ws + full_table + bnd)})
-        # Hessian-vector product along V
-        Vd = V.to(dev)
-        mt = torch.empty_like(main_launch.scores)
-        tws = torch.empty(_lib.workspace_bytes(True, _lib.OP_TANGENT, 16, 4096, 4096, -1),
-                          dtype=torch.uint8, device=dev)
-        hvp = lambda: _lib.affine_tangent(main_launch.scores, None, None, *params, direction=Vd,
-                                          map_tangent=mt, workspace=tws)
-        hvp()
-        torch.cuda.synchronize()
-        hms, _ = timed(torch, hvp, max(2, args.steps // 4))
-        log(f"HVP: {hms:.3f} ms, {cells / (hms * 1e-3) / 1e9:.1f} GCUPS")
-        extra.update({"hvp_ms": round(hms, 4), "hvp_gcups": round(cells / (hms * 1e-3) / 1e9, 3)})
-        del Vd, mt, tws
-        torch.cuda.empty_cache()
+        extra["long"] = long_record(torch, _lib, hip, args, dev)
+    if args.workload == "ragged" and not args.no_configs:
+        del main_launch.graph
+        extra["configs"] = {}
+        for name, fn in (("fixed_256x512", lambda: fixed_record(torch, _lib, hip, args, dev)),
+                         ("long_16x4096", lambda: long_record(torch, _lib, hip, args, dev)),
+                         ("embed_512", lambda: embed_record(torch, hip, args, dev))):
+            t0 = time.perf_counter()
+            extra["configs"][name] = fn()
+            log(f"{name} done in {time.perf_counter() - t0:.1f} s")
+            torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
-        log(f"CPU baseline (about {args.cpu_seconds:.0f} s)")
+        log(f"CPU baseline (about {args.cpu_seconds:.0f} s on {host_threads()} threads, then 1)")
         cpu = cpu_baseline(torch, cpu_pairs, params, args.cpu_seconds)
-        log(f"CPU {cpu['kind']}: {cpu['value']} GCUPS on {cpu['cores']} threads")
+        log(f"CPU {cpu['kind']}: {cpu['value']} GCUPS on {cpu['threads']} threads, "
+            f"{cpu['single_thread']['value']} on 1 ({cpu['cpu_model']})")
 
     line = {
         "metric": METRIC,
@@ -772,7 +1052,7 @@ def main():
             "forward_gcups": round(cells / (fwd_ms * 1e-3) / 1e9, 3),
             "kernel_ms": {k: round(v, 4) for k, v in kd.items()},
             "hip_graph": use_graph,
-            "launch_config": tuned or "automatic",
+            "launch_config": tuned,
             "wall_ms_per_step": round(wall_ms, 4),
             **extra,
         },

@@ -169,6 +169,54 @@ def sw_affine_forward(scores, gap_open, gap_ext, temp, lengths=None, impl="oracl
     return part, post, g[0], g[1], g[2]
 
 
+def sw_affine_value(scores, gap_open, gap_ext, temp, lengths=None, impl="oracle", prec="f32"):
+    """value[B] alone: the forward pass (src/sw_affine/kernels_cpu.cpp:94-195)."""
+    k, s, ln, B, L1, L2 = _setup(scores, lengths, impl, prec)
+    C = k.ctype
+    alpha = np.zeros((B, 3 * (L1 + 1) * (L2 + 1)), k.dtype)
+    part = np.zeros(B, k.dtype)
+    k.aff_forward(_ptr(s), _ptr(alpha), _ptr(part), _ptr(ln), B, L1, L2, C(gap_open),
+                  C(gap_ext), C(temp))
+    return part
+
+
+def over_pairs(fn, pairs, *args, threads=None, chunk=8, impl="oracle", **kw):
+    """Run ``fn(scores, *args, lengths=..., impl=..., **kw)`` (one of the
+    functions above) over a list of 2-D per-pair score arrays: chunks of
+    ``chunk`` pairs, each padded to its own largest pair, on a thread pool
+    (the restatement is single-threaded; ctypes releases the GIL).  The
+    reference build parallelises over its batch itself, so it runs one chunk
+    at a time.  Returns, per output of ``fn``, a list with one entry per
+    pair (maps cropped to the pair)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def run(lo):
+        part = pairs[lo:lo + chunk]
+        L1 = max(p.shape[0] for p in part)
+        L2 = max(p.shape[1] for p in part)
+        dt = np.float64 if kw.get("prec") == "f64" else np.float32
+        pad = np.zeros((len(part), L1, L2), dt)
+        ln = np.zeros((len(part), 2), np.int32)
+        for i, p in enumerate(part):
+            pad[i, :p.shape[0], :p.shape[1]] = p
+            ln[i] = p.shape
+        out = fn(pad, *args, lengths=ln, impl=impl, **kw)
+        out = out if isinstance(out, tuple) else (out,)
+        res = []
+        for o in out:
+            if o.ndim == 3:
+                res.append([o[i, :p.shape[0], :p.shape[1]].copy() for i, p in enumerate(part)])
+            else:
+                res.append(list(o))
+        return res
+
+    starts = range(0, len(pairs), chunk)
+    workers = 1 if impl == "ref" else (threads or min(16, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        parts = list(ex.map(run, starts))
+    return tuple([x for p in parts for x in p[k]] for k in range(len(parts[0])))
+
+
 def _aff_hvp_from(k, s, ln, B, L1, L2, alpha, part, V, go, ge, T):
     C = k.ctype
     asz = 3 * (L1 + 1) * (L2 + 1)

@@ -63,6 +63,11 @@ EXPORTS = (
     "ohm_nw_affine_tangent_cpu",
     "ohm_sv_affine_tangent",
     "ohm_sv_affine_tangent_cpu",
+    "ohm_embed_scores",
+    "ohm_embed_scores_cpu",
+    "ohm_embed_grads_workspace",
+    "ohm_embed_grads",
+    "ohm_embed_grads_cpu",
 )
 
 
@@ -368,3 +373,49 @@ def sv_affine_tangent(*args, **kw):
     ``affine_tangent``)."""
     return _three_state_tangent(_sva_tan, _sva_tan_cpu, _ws_sv_affine, "ohm_sv_affine_tangent",
                                 *args, **kw)
+
+
+# --------------------------------------------------------------------------
+# Embedding-scored alignment (include/ohm_embed.h)
+# --------------------------------------------------------------------------
+
+_emb_scores = _sig("ohm_embed_scores", ctypes.c_int, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _vp,
+                   _vp, _vp)
+_emb_scores_cpu = _sig("ohm_embed_scores_cpu", ctypes.c_int, _vp, _vp, _vp, _i32, _i32, _i32, _i32,
+                       _vp, _vp)
+_emb_ws = _sig("ohm_embed_grads_workspace", _sz, _i32, _i32, _i32, _i32)
+_emb_grads = _sig("ohm_embed_grads", ctypes.c_int, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _vp,
+                  _vp, _vp, _vp, _vp, _sz, _vp)
+_emb_grads_cpu = _sig("ohm_embed_grads_cpu", ctypes.c_int, _vp, _vp, _vp, _vp, _i32, _i32, _i32,
+                      _i32, _vp, _vp, _vp, _vp)
+
+
+def embed_scores(Q: torch.Tensor, K: torch.Tensor, lengths: torch.Tensor, offsets: torch.Tensor,
+                 scores: torch.Tensor) -> None:
+    """Packed ``Q_b K_b^T`` into ``scores`` (capacity B * Lq * Lk floats) and
+    the packed offsets.  Q [B, Lq, D], K [B, Lk, D] BF16, contiguous."""
+    B, Lq, D = Q.shape
+    Lk = K.shape[1]
+    args = (_ptr(Q), _ptr(K), _ptr(lengths), B, Lq, Lk, D, _ptr(offsets), _ptr(scores))
+    if Q.is_cuda:
+        stream = torch.cuda.current_stream(Q.device).cuda_stream
+        _check(_emb_scores(*args, stream), "ohm_embed_scores")
+    else:
+        _check(_emb_scores_cpu(*args), "ohm_embed_scores_cpu")
+
+
+def embed_grads(Q: torch.Tensor, K: torch.Tensor, lengths: torch.Tensor, offsets: torch.Tensor,
+                map: torch.Tensor, grad_value: torch.Tensor, dQ: torch.Tensor, dK: torch.Tensor,
+                workspace: torch.Tensor | None = None) -> None:
+    """dQ, dK (BF16, like Q and K) of sum_b g_b S_b through the packed map."""
+    B, Lq, D = Q.shape
+    Lk = K.shape[1]
+    args = (_ptr(Q), _ptr(K), _ptr(lengths), _ptr(offsets), B, Lq, Lk, D, _ptr(map),
+            _ptr(grad_value), _ptr(dQ), _ptr(dK))
+    if Q.is_cuda:
+        stream = torch.cuda.current_stream(Q.device).cuda_stream
+        need = int(_emb_ws(B, Lq, Lk, D))
+        ws = workspace if workspace is not None else _workspace(Q.device, need, stream)
+        _check(_emb_grads(*args, _ptr(ws), ws.numel(), stream), "ohm_embed_grads")
+    else:
+        _check(_emb_grads_cpu(*args), "ohm_embed_grads_cpu")

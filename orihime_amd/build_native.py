@@ -58,17 +58,35 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"command failed ({res.returncode}): {' '.join(cmd)}\n{out}")
 
 
+def _start(cmd: list[str]):
+    return cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+
+
+def _finish(job) -> None:
+    cmd, proc = job
+    out, _ = proc.communicate()
+    out = "\n".join(line for line in out.splitlines()
+                    if "packed-fp32-ops" not in line and "is not a recognized feature" not in line)
+    if proc.returncode != 0:
+        raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
+
+
 def build(verbose: bool = False) -> str:
     """Compile every native source and link libohm_sw.so; returns its path."""
     os.makedirs(BUILD, exist_ok=True)
     hipcc = _hipcc()
-    objs = []
-    kobj = os.path.join(BUILD, "sw_kernels.o")
-    _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "sw_kernels.hip"), "-o", kobj])
-    objs.append(kobj)
-    cobj = os.path.join(BUILD, "sw_cpu.o")
-    _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "sw_cpu.cpp"), "-o", cobj])
-    objs.append(cobj)
+    # the device sources compile in parallel (sw_kernels.hip dominates)
+    jobs, objs = [], []
+    for src in ("sw_kernels.hip", "embed_mfma.hip"):
+        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        jobs.append(_start([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]))
+        objs.append(obj)
+    for src in ("sw_cpu.cpp", "embed_cpu.cpp"):
+        obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
+        _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    for job in jobs:
+        _finish(job)
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -1,0 +1,436 @@
+// Embedding-scored alignment on the gfx950 matrix cores (include/ohm_embed.h).
+//
+// Three ragged BF16 MFMA kernels, each touching only a pair's own
+// M_b x N_b box:
+//   scores  s_b = Q_b K_b^T                 written packed (the sweeps' layout)
+//   dQ      dQ_b^T = K_b^T (g_b P_b)^T       reduction over the pair's columns
+//   dK      dK_b^T = Q_b^T (g_b P_b)         reduction over the pair's rows
+// All three use v_mfma_f32_16x16x32_bf16.  Operand layout (lane l, r = l%16,
+// q = l/16): A holds row r, reduction slots 8q..8q+7; B holds column r, the
+// same slots; the 16x16 FP32 result holds rows 4q..4q+3 of column r.  The
+// reduction slots of A and B are paired by the instruction, so any fixed
+// assignment of reduction indices to (q, slot) is a valid product.
+//
+// Each wave loads its operands straight from global memory: a reduction
+// slot run is 16 contiguous bytes (one dwordx4) when the reduction index is
+// the fast axis of the array.  For dQ and dK that axis is the embedding
+// dimension of Q and K, so the kernels read transposed copies (Q^T, K^T,
+// zero past each pair's length) written by embed_transpose_kernel.  The
+// map P is FP32; it enters the BF16 products as two terms, P = hi + lo with
+// hi = bf16(P) and lo = bf16(P - hi), which keeps ~16 bits of it.
+//
+// Work is dealt to the 8 XCDs in contiguous runs (work item w runs on XCD
+// w / (grid / 8)), so the tiles of one pair share an L2 and its Q and K rows
+// are fetched from HBM about once.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "ohm_embed.h"
+
+namespace ohm {
+void set_error(const std::string& msg);
+}
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+constexpr int kTile = 64;    // rows (or columns) of output per workgroup
+constexpr int kThreads = 256;  // four waves
+constexpr int kXcds = 8;
+
+#define DEV static __device__ __forceinline__
+
+DEV int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+DEV f32x4 mma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// FP32 -> BF16 bits, round to nearest even (finite inputs)
+DEV uint32_t bf16_bits(float x) {
+    uint32_t u = __builtin_bit_cast(uint32_t, x);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+DEV float bf16_value(uint32_t h) { return __builtin_bit_cast(float, h << 16); }
+
+// p = hi + lo as two BF16 vectors
+DEV void split8(const float (&p)[8], bf16x8& hi, bf16x8& lo) {
+    uint32_t h[8], l[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        h[v] = bf16_bits(p[v]);
+        l[v] = bf16_bits(p[v] - bf16_value(h[v]));
+    }
+    u32x4 hv, lv;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        hv[v] = h[2 * v] | (h[2 * v + 1] << 16);
+        lv[v] = l[2 * v] | (l[2 * v + 1] << 16);
+    }
+    hi = __builtin_bit_cast(bf16x8, hv);
+    lo = __builtin_bit_cast(bf16x8, lv);
+}
+
+// four FP32 values as four BF16 (8 bytes)
+DEV u32x2 pack4(float a, float b, float c, float d) {
+    u32x2 r;
+    r[0] = bf16_bits(a) | (bf16_bits(b) << 16);
+    r[1] = bf16_bits(c) | (bf16_bits(d) << 16);
+    return r;
+}
+
+// work item of this workgroup: contiguous runs of items per XCD (the
+// hardware deals workgroup ids to the XCDs round robin); grid % 8 == 0
+DEV int work_item() {
+    const int per = gridDim.x / kXcds;
+    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
+int round_grid(int64_t items) { return (int)((items + kXcds - 1) / kXcds * kXcds); }
+
+// --------------------------------------------------------------------------
+// offsets[b] = sum_{b' < b} M_b' N_b' (one workgroup; exact in int64)
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void embed_offsets_kernel(const int* __restrict__ lengths, int B,
+                                                             int Lq, int Lk, int64_t* __restrict__ offsets) {
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x, n = blockDim.x;
+    const int per = (B + n - 1) / n;
+    const int lo = min(B, t * per), hi = min(B, lo + per);
+    int64_t s = 0;
+    for (int b = lo; b < hi; ++b)
+        s += (int64_t)clampi(lengths[2 * b], 0, Lq) * clampi(lengths[2 * b + 1], 0, Lk);
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < n; d <<= 1) {  // inclusive scan
+        const int64_t v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = t > 0 ? part[t - 1] : 0;
+    for (int b = lo; b < hi; ++b) {
+        offsets[b] = run;
+        run += (int64_t)clampi(lengths[2 * b], 0, Lq) * clampi(lengths[2 * b + 1], 0, Lk);
+    }
+}
+
+// --------------------------------------------------------------------------
+// scores: one workgroup per 64 x 64 tile of a pair, a wave per 32 x 32 quarter
+// --------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kThreads) void embed_scores_kernel(
+    const __bf16* __restrict__ Q, const __bf16* __restrict__ K, const int* __restrict__ lengths,
+    const int64_t* __restrict__ offsets, int Lq, int Lk, int tiles_j, int tiles_pair, int items,
+    float* __restrict__ S) {
+    const int w = work_item();
+    if (w >= items) return;
+    const int b = w / tiles_pair, tile = w - b * tiles_pair;
+    const int M = clampi(lengths[2 * b], 0, Lq), N = clampi(lengths[2 * b + 1], 0, Lk);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+    const int i0 = (tile / tiles_j) * kTile + 32 * (wave >> 1);
+    const int j0 = (tile % tiles_j) * kTile + 32 * (wave & 1);
+    if (i0 >= M || j0 >= N) return;  // no barrier in this kernel
+    const __bf16* qa[2];
+    const __bf16* kb[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        qa[x] = Q + ((size_t)b * Lq + min(i0 + 16 * x + r, M - 1)) * D + 8 * q;
+        kb[x] = K + ((size_t)b * Lk + min(j0 + 16 * x + r, N - 1)) * D + 8 * q;
+    }
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < D; k += 32) {
+        const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(qa[0] + k);
+        const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(qa[1] + k);
+        const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(kb[0] + k);
+        const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(kb[1] + k);
+        acc[0][0] = mma(a0, b0, acc[0][0]);
+        acc[0][1] = mma(a0, b1, acc[0][1]);
+        acc[1][0] = mma(a1, b0, acc[1][0]);
+        acc[1][1] = mma(a1, b1, acc[1][1]);
+    }
+    float* Sb = S + offsets[b];
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+        const int col = j0 + 16 * y + r;
+        if (col >= N) continue;
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = i0 + 16 * x + 4 * q + v;
+                if (row < M) Sb[(size_t)row * N + col] = acc[x][y][v];
+            }
+    }
+}
+
+// --------------------------------------------------------------------------
+// X [B, L, D] -> XT [B, D, L32] (L32 = L rounded up to 32, so a wave's
+// 32-wide reduction run never leaves a row), zero for rows past each pair's
+// length (lengths[2 b + side]) and in the padding
+// --------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kThreads) void embed_transpose_kernel(const __bf16* __restrict__ X,
+                                                                   const int* __restrict__ lengths,
+                                                                   int side, int L, int L32,
+                                                                   __bf16* __restrict__ XT) {
+    const int b = blockIdx.y;
+    const int len = clampi(lengths[2 * b + side], 0, L);
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    if (j >= L32) return;
+    const __bf16* src = X + ((size_t)b * L + min(j, L - 1)) * D;
+    __bf16* dst = XT + (size_t)b * D * L32 + j;
+    for (int c = threadIdx.x >> 6; c < D / 8; c += kThreads / 64) {
+        u32x4 v = u32x4{0u, 0u, 0u, 0u};
+        if (j < len) v = *reinterpret_cast<const u32x4*>(src + 8 * c);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const uint16_t h = (uint16_t)(v[e >> 1] >> (16 * (e & 1)));
+            dst[(size_t)(8 * c + e) * L32] = __builtin_bit_cast(__bf16, h);
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// dQ: one workgroup per 64 rows of a pair's Q, a wave per 16 rows x all D
+// --------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kThreads) void embed_dq_kernel(
+    const __bf16* __restrict__ KT, const float* __restrict__ P, const int* __restrict__ lengths,
+    const int64_t* __restrict__ offsets, const float* __restrict__ gv, int Lq, int Lk, int Lk32,
+    int tiles_i, int items, __bf16* __restrict__ dQ) {
+    const int w = work_item();
+    if (w >= items) return;
+    const int b = w / tiles_i, t = w - b * tiles_i;
+    const int M = clampi(lengths[2 * b], 0, Lq), N = clampi(lengths[2 * b + 1], 0, Lk);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+    const int i0 = t * kTile + 16 * wave;
+    if (i0 >= Lq) return;
+    const int row = i0 + r;
+    __bf16* out = dQ + ((size_t)b * Lq + row) * D + 4 * q;
+    constexpr int NB = D / 16;
+    f32x4 acc[NB];
+#pragma unroll
+    for (int d = 0; d < NB; ++d) acc[d] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    if (i0 < M) {
+        const bool live = row < M;
+        const float* prow = P + offsets[b] + (size_t)min(row, M - 1) * N;
+        const __bf16* kt = KT + ((size_t)b * D + r) * Lk32 + 8 * q;
+        for (int j0 = 0; j0 < N; j0 += 32) {
+            const int jj = j0 + 8 * q;
+            float p[8];
+            if (live && jj + 8 <= N) {
+                const f4u u0 = *reinterpret_cast<const f4u*>(prow + jj);
+                const f4u u1 = *reinterpret_cast<const f4u*>(prow + jj + 4);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) { p[v] = u0[v]; p[4 + v] = u1[v]; }
+            } else {
+#pragma unroll
+                for (int v = 0; v < 8; ++v) p[v] = (live && jj + v < N) ? prow[jj + v] : 0.0f;
+            }
+            bf16x8 hi, lo;
+            split8(p, hi, lo);
+#pragma unroll
+            for (int d = 0; d < NB; ++d) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(kt + (size_t)16 * d * Lk32 + j0);
+                acc[d] = mma(a, hi, acc[d]);
+                acc[d] = mma(a, lo, acc[d]);
+            }
+        }
+    }
+    if (row >= Lq) return;
+    const float g = row < M ? gv[b] : 0.0f;
+#pragma unroll
+    for (int d = 0; d < NB; ++d)
+        *reinterpret_cast<u32x2*>(out + 16 * d) =
+            pack4(g * acc[d][0], g * acc[d][1], g * acc[d][2], g * acc[d][3]);
+}
+
+// --------------------------------------------------------------------------
+// dK: one workgroup per 64 rows of a pair's K, a wave per 16 rows x all D
+// --------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kThreads) void embed_dk_kernel(
+    const __bf16* __restrict__ QT, const float* __restrict__ P, const int* __restrict__ lengths,
+    const int64_t* __restrict__ offsets, const float* __restrict__ gv, int Lq, int Lk, int Lq32,
+    int tiles_j, int items, __bf16* __restrict__ dK) {
+    const int w = work_item();
+    if (w >= items) return;
+    const int b = w / tiles_j, t = w - b * tiles_j;
+    const int M = clampi(lengths[2 * b], 0, Lq), N = clampi(lengths[2 * b + 1], 0, Lk);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+    const int j0 = t * kTile + 16 * wave;
+    if (j0 >= Lk) return;
+    const int col = j0 + r;
+    __bf16* out = dK + ((size_t)b * Lk + col) * D + 4 * q;
+    constexpr int NB = D / 16;
+    f32x4 acc[NB];
+#pragma unroll
+    for (int d = 0; d < NB; ++d) acc[d] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    if (j0 < N) {
+        const bool live = col < N;
+        const float* pcol = P + offsets[b] + min(col, N - 1);
+        const __bf16* qt = QT + ((size_t)b * D + r) * Lq32 + 8 * q;
+        for (int i0 = 0; i0 < M; i0 += 32) {
+            const int ii = i0 + 8 * q;
+            float p[8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) p[v] = (live && ii + v < M) ? pcol[(size_t)(ii + v) * N] : 0.0f;
+            bf16x8 hi, lo;
+            split8(p, hi, lo);
+#pragma unroll
+            for (int d = 0; d < NB; ++d) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(qt + (size_t)16 * d * Lq32 + i0);
+                acc[d] = mma(a, hi, acc[d]);
+                acc[d] = mma(a, lo, acc[d]);
+            }
+        }
+    }
+    if (col >= Lk) return;
+    const float g = col < N ? gv[b] : 0.0f;
+#pragma unroll
+    for (int d = 0; d < NB; ++d)
+        *reinterpret_cast<u32x2*>(out + 16 * d) =
+            pack4(g * acc[d][0], g * acc[d][1], g * acc[d][2], g * acc[d][3]);
+}
+
+// --------------------------------------------------------------------------
+// Host side
+// --------------------------------------------------------------------------
+
+ohm_status fail(ohm_status st, const std::string& msg) {
+    ohm::set_error(msg);
+    return st;
+}
+
+ohm_status launched(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(OHM_ELAUNCH, std::string(what) + ": " + hipGetErrorString(e));
+    return OHM_OK;
+}
+
+bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+ohm_status check_dims(const void* Q, const void* K, const int32_t* lengths, int B, int Lq, int Lk,
+                      int D) {
+    if (B < 0 || Lq < 0 || Lk < 0) return fail(OHM_EINVAL, "B, Lq and Lk must be >= 0");
+    if (D != 32 && D != 64 && D != 128 && D != 256)
+        return fail(OHM_EINVAL, "embedding dimension D must be 32, 64, 128 or 256");
+    if (B > 0 && !lengths) return fail(OHM_EINVAL, "lengths is null");
+    if ((int64_t)B * Lq * D > ((int64_t)1 << 40) || (int64_t)B * Lk * D > ((int64_t)1 << 40))
+        return fail(OHM_EINVAL, "embedding tensors are too large");
+    if (B > 0 && Lq > 0 && (!Q || !aligned16(Q))) return fail(OHM_EINVAL, "Q must be a 16-byte aligned pointer");
+    if (B > 0 && Lk > 0 && (!K || !aligned16(K))) return fail(OHM_EINVAL, "K must be a 16-byte aligned pointer");
+    return OHM_OK;
+}
+
+template <typename F>
+ohm_status by_dim(int D, F&& f) {
+    switch (D) {
+        case 32: return f(std::integral_constant<int, 32>{});
+        case 64: return f(std::integral_constant<int, 64>{});
+        case 128: return f(std::integral_constant<int, 128>{});
+        default: return f(std::integral_constant<int, 256>{});
+    }
+}
+
+size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+extern "C" ohm_status ohm_embed_scores(const void* Q, const void* K, const int32_t* lengths,
+                                       int32_t B, int32_t Lq, int32_t Lk, int32_t D,
+                                       int64_t* offsets, float* scores, void* stream) {
+    ohm_status st = check_dims(Q, K, lengths, B, Lq, Lk, D);
+    if (st != OHM_OK) return st;
+    if (B == 0) return OHM_OK;
+    if (!offsets) return fail(OHM_EINVAL, "offsets is null");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(embed_offsets_kernel, dim3(1), dim3(1024), 0, s, lengths, B, Lq, Lk, offsets);
+    if ((st = launched("embed offsets kernel")) != OHM_OK) return st;
+    if (Lq == 0 || Lk == 0) return OHM_OK;
+    if (!scores) return fail(OHM_EINVAL, "scores is null");
+    const int tiles_j = (Lk + kTile - 1) / kTile;
+    const int tiles_pair = ((Lq + kTile - 1) / kTile) * tiles_j;
+    const int64_t items = (int64_t)B * tiles_pair;
+    if (items > (1 << 30)) return fail(OHM_EINVAL, "too many score tiles");
+    return by_dim(D, [&](auto dd) {
+        constexpr int DD = decltype(dd)::value;
+        hipLaunchKernelGGL(embed_scores_kernel<DD>, dim3(round_grid(items)), dim3(kThreads), 0, s,
+                           static_cast<const __bf16*>(Q), static_cast<const __bf16*>(K), lengths,
+                           offsets, Lq, Lk, tiles_j, tiles_pair, (int)items, scores);
+        return launched("embed scores kernel");
+    });
+}
+
+extern "C" size_t ohm_embed_grads_workspace(int32_t B, int32_t Lq, int32_t Lk, int32_t D) {
+    if (B <= 0 || D <= 0) return 0;
+    const size_t lq = round_up((size_t)(Lq > 0 ? Lq : 0), 32), lk = round_up((size_t)(Lk > 0 ? Lk : 0), 32);
+    return round_up((size_t)B * D * lq * 2, 256) + round_up((size_t)B * D * lk * 2, 256);
+}
+
+extern "C" ohm_status ohm_embed_grads(const void* Q, const void* K, const int32_t* lengths,
+                                      const int64_t* offsets, int32_t B, int32_t Lq, int32_t Lk,
+                                      int32_t D, const float* map, const float* grad_value,
+                                      void* dQ, void* dK, void* workspace, size_t workspace_bytes,
+                                      void* stream) {
+    ohm_status st = check_dims(Q, K, lengths, B, Lq, Lk, D);
+    if (st != OHM_OK) return st;
+    if (B == 0) return OHM_OK;
+    if ((Lq > 0 && (!dQ || !aligned16(dQ))) || (Lk > 0 && (!dK || !aligned16(dK))))
+        return fail(OHM_EINVAL, "dQ and dK must be 16-byte aligned pointers");
+    if (!offsets || !grad_value) return fail(OHM_EINVAL, "offsets and grad_value must be non-null");
+    if (Lq > 0 && Lk > 0 && !map) return fail(OHM_EINVAL, "map is null");
+    const size_t need = ohm_embed_grads_workspace(B, Lq, Lk, D);
+    if (need > 0 && (!workspace || workspace_bytes < need || !aligned16(workspace)))
+        return fail(OHM_EWORKSPACE, "workspace missing, misaligned or smaller than ohm_embed_grads_workspace()");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int lq32 = (int)round_up((size_t)Lq, 32), lk32 = (int)round_up((size_t)Lk, 32);
+    if ((int64_t)B * ((Lq + kTile - 1) / kTile + (Lk + kTile - 1) / kTile) > (1 << 30))
+        return fail(OHM_EINVAL, "too many gradient tiles");
+    __bf16* qt = static_cast<__bf16*>(workspace);
+    __bf16* kt = reinterpret_cast<__bf16*>(static_cast<char*>(workspace) +
+                                          round_up((size_t)B * D * lq32 * 2, 256));
+    return by_dim(D, [&](auto dd) {
+        constexpr int DD = decltype(dd)::value;
+        ohm_status r = OHM_OK;
+        if (Lq > 0 && Lk > 0) {
+            hipLaunchKernelGGL(embed_transpose_kernel<DD>, dim3((lq32 + 63) / 64, B), dim3(kThreads), 0, s,
+                               static_cast<const __bf16*>(Q), lengths, 0, Lq, lq32, qt);
+            if ((r = launched("embed transpose kernel (Q)")) != OHM_OK) return r;
+            hipLaunchKernelGGL(embed_transpose_kernel<DD>, dim3((lk32 + 63) / 64, B), dim3(kThreads), 0, s,
+                               static_cast<const __bf16*>(K), lengths, 1, Lk, lk32, kt);
+            if ((r = launched("embed transpose kernel (K)")) != OHM_OK) return r;
+        }
+        if (Lq > 0) {
+            const int tiles = (Lq + kTile - 1) / kTile;
+            const int64_t items = (int64_t)B * tiles;
+            hipLaunchKernelGGL(embed_dq_kernel<DD>, dim3(round_grid(items)), dim3(kThreads), 0, s, kt, map,
+                               lengths, offsets, grad_value, Lq, Lk, lk32, tiles, (int)items,
+                               static_cast<__bf16*>(dQ));
+            if ((r = launched("embed dQ kernel")) != OHM_OK) return r;
+        }
+        if (Lk > 0) {
+            const int tiles = (Lk + kTile - 1) / kTile;
+            const int64_t items = (int64_t)B * tiles;
+            hipLaunchKernelGGL(embed_dk_kernel<DD>, dim3(round_grid(items)), dim3(kThreads), 0, s, qt, map,
+                               lengths, offsets, grad_value, Lq, Lk, lq32, tiles, (int)items,
+                               static_cast<__bf16*>(dK));
+            if ((r = launched("embed dK kernel")) != OHM_OK) return r;
+        }
+        return r;
+    });
+}

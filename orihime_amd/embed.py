@@ -1,61 +1,108 @@
 """Embedding-scored soft alignment, end to end.
 
 Pair scores are inner products of per-position embeddings,
-``s_b = Q_b K_b^T`` (BF16 inputs, FP32 accumulation and output: one batched
-MFMA GEMM through hipBLASLt), fed into the ragged soft Smith-Waterman affine
-sweep (padded storage with per-pair lengths: cells past a pair's lengths are
-never computed), and differentiated back to the embeddings through the map:
-with ``P_b = dS_b / ds_b`` the marginal map,
+``s_b = Q_b K_b^T`` (BF16 inputs, FP32 accumulation), fed into the ragged
+soft Smith-Waterman affine sweep, and differentiated back to the embeddings
+through the map: with ``P_b = dS_b / ds_b`` the marginal map,
 
-    dQ_b = g_b (P_b K_b),    dK_b = g_b (Q_b^T P_b)^T
+    dQ_b = g_b P_b K_b,    dK_b = g_b P_b^T Q_b
 
-(FP32 GEMMs on the FP32 map).  Rows and columns past a pair's lengths have a
-zero map and so receive zero gradient.
+Everything runs through the C ABI (include/ohm_embed.h, include/ohm_sw.h):
+on the GPU, ragged BF16 MFMA kernels form each pair's M_b x N_b score block
+straight into the sweeps' packed layout (no padding, no GEMM over the padded
+box), the packed persistent sweeps give the value and the packed map, and two
+ragged MFMA kernels contract the map with K and Q.  CPU tensors run the
+library's host implementation of the same entry points.
 
     value = soft_align_embeddings(Q, K, lengths, gap_open=-4.0, gap_ext=-0.5,
                                   temperature=0.25)
     value.sum().backward()      # Q.grad, K.grad
 
-Q: [B, M, D], K: [B, N, D] (bfloat16 or float32, same device), lengths:
-[B, 2] int32 (M_b, N_b).
+Q: [B, M, D], K: [B, N, D] bfloat16 (D = 32, 64, 128 or 256), same device;
+lengths: [B, 2] int32 (M_b, N_b).  Rows past a pair's lengths are ignored and
+receive zero gradient.
 """
 
 from __future__ import annotations
 
 import torch
 
-from . import _ops  # noqa: F401  (registers torch.ops.orihime)
+from . import _lib, _ops  # noqa: F401  (registers torch.ops.orihime)
+from . import _checks
 
 
-def pair_scores(Q: torch.Tensor, K: torch.Tensor) -> torch.Tensor:
-    """``Q K^T`` per pair in FP32: BF16 products with FP32 accumulation."""
-    Kt = K.transpose(1, 2).contiguous()
-    if Q.is_cuda and Q.dtype == torch.bfloat16:
-        return torch.bmm(Q, Kt, out_dtype=torch.float32)
-    return torch.bmm(Q.float(), Kt.float())
+def _check(Q: torch.Tensor, K: torch.Tensor, lengths: torch.Tensor) -> None:
+    if Q.dim() != 3 or K.dim() != 3 or Q.shape[0] != K.shape[0] or Q.shape[2] != K.shape[2]:
+        raise ValueError("Q must be [B, M, D] and K [B, N, D] with matching B and D")
+    if Q.dtype != torch.bfloat16 or K.dtype != torch.bfloat16:
+        raise TypeError(f"Q and K must be torch.bfloat16, got {Q.dtype} and {K.dtype}")
+    if Q.device != K.device:
+        raise ValueError("Q and K must be on the same device")
+    if Q.shape[2] not in (32, 64, 128, 256):
+        raise ValueError(f"embedding dimension must be 32, 64, 128 or 256, got {Q.shape[2]}")
+    if tuple(lengths.shape) != (Q.shape[0], 2):
+        raise ValueError("lengths must be [B, 2] (M_b, N_b)")
+
+
+def _lengths(lengths: torch.Tensor, Q: torch.Tensor, K: torch.Tensor) -> torch.Tensor:
+    ln = lengths.to(device=Q.device, dtype=torch.int32).contiguous()
+    if not (Q.is_cuda and torch.cuda.is_current_stream_capturing()):
+        _checks.require((ln[:, 0] >= 0) & (ln[:, 0] <= Q.shape[1]) & (ln[:, 1] >= 0) & (ln[:, 1] <= K.shape[1]),
+                "lengths entries must lie within the padded input shape")
+    return ln
+
+
+def pair_scores(Q: torch.Tensor, K: torch.Tensor, lengths: torch.Tensor
+                ) -> tuple[torch.Tensor, torch.Tensor]:
+    """Packed pair scores (capacity B * M * N floats; pair b's block at
+    ``offsets[b]``, row stride N_b) and the int64 offsets."""
+    _check(Q, K, lengths)
+    Q, K = Q.contiguous(), K.contiguous()
+    ln = _lengths(lengths, Q, K)
+    B, M, _ = Q.shape
+    N = K.shape[1]
+    offsets = torch.empty(B, dtype=torch.int64, device=Q.device)
+    scores = torch.empty(B * M * N, dtype=torch.float32, device=Q.device)
+    _lib.embed_scores(Q, K, ln, offsets, scores)
+    return scores, offsets
+
+
+def embedding_grads(Q: torch.Tensor, K: torch.Tensor, lengths: torch.Tensor, offsets: torch.Tensor,
+                    post: torch.Tensor, grad_value: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """``dQ_b = g_b P_b K_b`` and ``dK_b = g_b P_b^T Q_b`` (BF16) from the
+    packed map ``post``."""
+    Q, K = Q.contiguous(), K.contiguous()
+    dQ, dK = torch.empty_like(Q), torch.empty_like(K)
+    g = grad_value.to(device=Q.device, dtype=torch.float32).contiguous()
+    _lib.embed_grads(Q, K, lengths, offsets, post, g, dQ, dK)
+    return dQ, dK
+
+
+def _sweep(scores, lengths, offsets, M, N, gap_open, gap_ext, temp):
+    B = lengths.shape[0]
+    value = torch.empty(B, dtype=torch.float32, device=scores.device)
+    post = torch.empty_like(scores)
+    _lib.affine_forward(scores, lengths, offsets, gap_open, gap_ext, temp, value=value, map=post,
+                        L1max=M, L2max=N)
+    return value, post
 
 
 class _SoftAlignEmbeddings(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, Q, K, lengths, gap_open, gap_ext, temperature):
-        scores = pair_scores(Q, K)
-        value, post = torch.ops.orihime.sw_affine_forward(scores, float(gap_open), float(gap_ext),
-                                                          float(temperature), lengths)
-        ctx.save_for_backward(Q, K, post)
+    def forward(ctx, Q, K, lengths, gap_open, gap_ext, temp):
+        Q, K = Q.contiguous(), K.contiguous()
+        scores, offsets = pair_scores(Q, K, lengths)
+        ln = lengths.to(device=Q.device, dtype=torch.int32).contiguous()
+        value, post = _sweep(scores, ln, offsets, Q.shape[1], K.shape[1], float(gap_open),
+                             float(gap_ext), float(temp))
+        ctx.save_for_backward(Q, K, ln, offsets, post)
         return value
 
     @staticmethod
     def backward(ctx, grad_value):
-        # FP32 GEMMs on the FP32 map; the per-pair factor g is applied to the
-        # small [B, *, D] products, and dK is formed as (Q^T P)^T, so the
-        # [B, M, N] map is only read by the two GEMMs (no scaled or transposed
-        # copy of it)
-        Q, K, post = ctx.saved_tensors
-        gv = grad_value.to(torch.float32)[:, None, None]
-        dQ = torch.bmm(post, K.float()) * gv
-        dKt = torch.bmm(Q.float().transpose(1, 2).contiguous(), post)
-        dK = dKt.transpose(1, 2) * gv
-        return dQ.to(Q.dtype), dK.to(K.dtype), None, None, None, None
+        Q, K, ln, offsets, post = ctx.saved_tensors
+        dQ, dK = embedding_grads(Q, K, ln, offsets, post, grad_value)
+        return dQ, dK, None, None, None, None
 
 
 def soft_align_embeddings(Q: torch.Tensor, K: torch.Tensor, lengths: torch.Tensor,
@@ -63,9 +110,7 @@ def soft_align_embeddings(Q: torch.Tensor, K: torch.Tensor, lengths: torch.Tenso
                           temperature: float = 0.25) -> torch.Tensor:
     """Soft Smith-Waterman (affine) value of every pair, scored by ``Q K^T``;
     differentiable in Q and K."""
-    if Q.dim() != 3 or K.dim() != 3 or Q.shape[0] != K.shape[0] or Q.shape[2] != K.shape[2]:
-        raise ValueError("Q must be [B, M, D] and K [B, N, D] with matching B and D")
-    if lengths.shape != (Q.shape[0], 2):
-        raise ValueError("lengths must be [B, 2] (M_b, N_b)")
-    return _SoftAlignEmbeddings.apply(Q, K, lengths.to(device=Q.device, dtype=torch.int32),
-                                      gap_open, gap_ext, temperature)
+    _check(Q, K, lengths)
+    _checks.temperature(temperature)
+    return _SoftAlignEmbeddings.apply(Q, K, lengths, float(gap_open), float(gap_ext),
+                                      float(temperature))

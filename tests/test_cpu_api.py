@@ -25,8 +25,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_library_exports_every_declared_symbol():
-    header = open(os.path.join(ROOT, "include", "ohm_sw.h")).read()
-    declared = set(re.findall(r"^\S.*?\b(ohm_\w+)\(", header, re.M))
+    declared = set()
+    for h in ("ohm_sw.h", "ohm_embed.h"):
+        header = open(os.path.join(ROOT, "include", h)).read()
+        declared |= set(re.findall(r"^\S.*?\b(ohm_\w+)\(", header, re.M))
     assert len(declared) >= 12
     lib = ctypes.CDLL(_lib.LIB_PATH)
     for name in declared:
@@ -434,17 +436,22 @@ def _embed_truth(Q, K, ln, go, ge, T):
 def test_embedding_scored_alignment_cpu_matches_fp64_oracle():
     from orihime_amd.embed import soft_align_embeddings
     g = torch.Generator().manual_seed(4)
-    B, M, N, D = 3, 40, 50, 16
+    B, M, N, D = 3, 40, 50, 32
     go, ge, T = -4.0, -0.5, 0.25
-    Q = (torch.randn(B, M, D, generator=g) / D ** 0.5).requires_grad_()
-    K = (torch.randn(B, N, D, generator=g) / D ** 0.5).requires_grad_()
+    Q = (torch.randn(B, M, D, generator=g) / D ** 0.5).bfloat16().requires_grad_()
+    K = (torch.randn(B, N, D, generator=g) / D ** 0.5).bfloat16().requires_grad_()
     ln = torch.tensor([[40, 50], [31, 22], [7, 50]], dtype=torch.int32)
     v = soft_align_embeddings(Q, K, ln, go, ge, T)
     v.sum().backward()
     val, dQ, dK = _embed_truth(Q.detach(), K.detach(), ln, go, ge, T)
     assert rel_err(v.detach().numpy(), val, 1e-3) < 1e-5
-    assert abs_err(Q.grad.numpy(), dQ) < 1e-4 and abs_err(K.grad.numpy(), dK) < 1e-4
+    # gradients are BF16 like the inputs: half an ulp is 2^-9 of the entry
+    gq, gk = Q.grad.float().numpy(), K.grad.float().numpy()
+    assert np.all(np.abs(gq - dQ) <= 2.0 ** -8 * np.abs(dQ) + 1e-6)
+    assert np.all(np.abs(gk - dK) <= 2.0 ** -8 * np.abs(dK) + 1e-6)
     assert float(Q.grad[1, 31:].abs().max()) == 0.0 and float(K.grad[1, 22:].abs().max()) == 0.0
+    with pytest.raises(TypeError, match="bfloat16"):
+        soft_align_embeddings(Q.detach().float(), K.detach().float(), ln, go, ge, T)
 
 
 @pytest.mark.parametrize("affine", [True, False], ids=["affine", "linear"])
