@@ -627,6 +627,8 @@ def embed_record(torch, hip, args, dev):
     ms, _ = timed(torch, step, args.steps)
     parts = _embed_parts(torch, hip, Q.detach(), K.detach(), lnd, params, max(3, args.steps // 2))
     # FP64 errors of 16 pairs spread over the size range
+    Q.grad = None
+    K.grad = None
     v = soft_align_embeddings(Q, K, lnd, *params)
     v.sum().backward()
     vh, dQ, dK = v.detach().cpu().numpy(), Q.grad.float().cpu().numpy(), K.grad.float().cpu().numpy()

@@ -42,7 +42,7 @@ typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
-constexpr int kTile = 64;    // rows (or columns) of output per workgroup
+constexpr int kRows = 128;     // output rows per workgroup (32 per wave)
 constexpr int kThreads = 256;  // four waves
 constexpr int kXcds = 8;
 
@@ -124,56 +124,71 @@ __global__ __launch_bounds__(1024) void embed_offsets_kernel(const int* __restri
 }
 
 // --------------------------------------------------------------------------
-// scores: one workgroup per 64 x 64 tile of a pair, a wave per 32 x 32 quarter
+// scores: one workgroup per 128 rows of a pair, a wave per 32 rows; the wave
+// keeps its rows of Q in registers and walks the pair's columns 32 at a
+// time, requesting the next 32 rows of K while it multiplies the current
 // --------------------------------------------------------------------------
 template <int D>
 __global__ __launch_bounds__(kThreads) void embed_scores_kernel(
     const __bf16* __restrict__ Q, const __bf16* __restrict__ K, const int* __restrict__ lengths,
-    const int64_t* __restrict__ offsets, int Lq, int Lk, int tiles_j, int tiles_pair, int items,
+    const int64_t* __restrict__ offsets, int Lq, int Lk, int tiles_i, int items,
     float* __restrict__ S) {
+    constexpr int KS = D / 32;  // reduction steps
     const int w = work_item();
     if (w >= items) return;
-    const int b = w / tiles_pair, tile = w - b * tiles_pair;
+    const int b = w / tiles_i, t = w - b * tiles_i;
     const int M = clampi(lengths[2 * b], 0, Lq), N = clampi(lengths[2 * b + 1], 0, Lk);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-    const int i0 = (tile / tiles_j) * kTile + 32 * (wave >> 1);
-    const int j0 = (tile % tiles_j) * kTile + 32 * (wave & 1);
-    if (i0 >= M || j0 >= N) return;  // no barrier in this kernel
-    const __bf16* qa[2];
-    const __bf16* kb[2];
+    const int i0 = t * (4 * 32) + 32 * wave;
+    if (i0 >= M || N == 0) return;  // no barrier in this kernel
+    bf16x8 a[2][KS];
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
-        qa[x] = Q + ((size_t)b * Lq + min(i0 + 16 * x + r, M - 1)) * D + 8 * q;
-        kb[x] = K + ((size_t)b * Lk + min(j0 + 16 * x + r, N - 1)) * D + 8 * q;
+        const __bf16* qr = Q + ((size_t)b * Lq + min(i0 + 16 * x + r, M - 1)) * D + 8 * q;
+#pragma unroll
+        for (int k = 0; k < KS; ++k) a[x][k] = *reinterpret_cast<const bf16x8*>(qr + 32 * k);
     }
-    f32x4 acc[2][2];
+    const __bf16* Kb = K + (size_t)b * Lk * D + 8 * q;
+    auto load_k = [&](bf16x8 (&dst)[2][KS], int j0) {
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y) {
+            const __bf16* kr = Kb + (size_t)min(j0 + 16 * y + r, N - 1) * D;
 #pragma unroll
-        for (int y = 0; y < 2; ++y) acc[x][y] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int k = 0; k < D; k += 32) {
-        const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(qa[0] + k);
-        const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(qa[1] + k);
-        const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(kb[0] + k);
-        const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(kb[1] + k);
-        acc[0][0] = mma(a0, b0, acc[0][0]);
-        acc[0][1] = mma(a0, b1, acc[0][1]);
-        acc[1][0] = mma(a1, b0, acc[1][0]);
-        acc[1][1] = mma(a1, b1, acc[1][1]);
-    }
+            for (int k = 0; k < KS; ++k) dst[y][k] = *reinterpret_cast<const bf16x8*>(kr + 32 * k);
+        }
+    };
     float* Sb = S + offsets[b];
-#pragma unroll
-    for (int y = 0; y < 2; ++y) {
-        const int col = j0 + 16 * y + r;
-        if (col >= N) continue;
+    bf16x8 bc[2][KS], bn[2][KS];
+    load_k(bc, 0);
+    for (int j0 = 0; j0 < N; j0 += 32) {
+        if (j0 + 32 < N) load_k(bn, j0 + 32);
+        f32x4 acc[2][2];
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = i0 + 16 * x + 4 * q + v;
-                if (row < M) Sb[(size_t)row * N + col] = acc[x][y][v];
-            }
+            for (int y = 0; y < 2; ++y) acc[x][y] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int k = 0; k < KS; ++k)
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) acc[x][y] = mma(a[x][k], bc[y][k], acc[x][y]);
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int col = j0 + 16 * y + r;
+            if (col >= N) continue;
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = i0 + 16 * x + 4 * q + v;
+                    if (row < M) Sb[(size_t)row * N + col] = acc[x][y][v];
+                }
+        }
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int k = 0; k < KS; ++k) bc[y][k] = bn[y][k];
     }
 }
 
@@ -205,106 +220,106 @@ __global__ __launch_bounds__(kThreads) void embed_transpose_kernel(const __bf16*
 }
 
 // --------------------------------------------------------------------------
-// dQ: one workgroup per 64 rows of a pair's Q, a wave per 16 rows x all D
+// dQ and dK: one workgroup per 128 output rows of a pair, a wave per 32 rows
+// (two 16-row blocks) x all D.  Each reduction step of 32 takes the D/16
+// A-operand runs of the transposed copy (shared by the wave's two blocks)
+// and the map values of the two blocks; the next step's operands are
+// requested before the current step's products.
+//   dQ^T[d, i] = sum_j K^T[d, j] P[i, j]   (map rows: 8 contiguous floats)
+//   dK^T[d, j] = sum_i Q^T[d, i] P[i, j]   (map columns: 8 rows, 64-byte
+//                                           segments across 16 lanes)
 // --------------------------------------------------------------------------
-template <int D>
-__global__ __launch_bounds__(kThreads) void embed_dq_kernel(
-    const __bf16* __restrict__ KT, const float* __restrict__ P, const int* __restrict__ lengths,
-    const int64_t* __restrict__ offsets, const float* __restrict__ gv, int Lq, int Lk, int Lk32,
-    int tiles_i, int items, __bf16* __restrict__ dQ) {
+template <int D, bool ROWS>
+__global__ __launch_bounds__(kThreads) void embed_grad_kernel(
+    const __bf16* __restrict__ XT, const float* __restrict__ P, const int* __restrict__ lengths,
+    const int64_t* __restrict__ offsets, const float* __restrict__ gv, int Lq, int Lk, int L32,
+    int tiles, int items, __bf16* __restrict__ out) {
+    constexpr int NB = D / 16;
     const int w = work_item();
     if (w >= items) return;
-    const int b = w / tiles_i, t = w - b * tiles_i;
+    const int b = w / tiles, t = w - b * tiles;
     const int M = clampi(lengths[2 * b], 0, Lq), N = clampi(lengths[2 * b + 1], 0, Lk);
+    // ROWS: the output rows are the map's rows (dQ), else its columns (dK)
+    const int Lout = ROWS ? Lq : Lk, nout = ROWS ? M : N, nred = ROWS ? N : M;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-    const int i0 = t * kTile + 16 * wave;
-    if (i0 >= Lq) return;
-    const int row = i0 + r;
-    __bf16* out = dQ + ((size_t)b * Lq + row) * D + 4 * q;
-    constexpr int NB = D / 16;
-    f32x4 acc[NB];
+    const int o0 = t * (4 * 32) + 32 * wave;
+    if (o0 >= Lout) return;
+    f32x4 acc[2][NB];
 #pragma unroll
-    for (int d = 0; d < NB; ++d) acc[d] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    if (i0 < M) {
-        const bool live = row < M;
-        const float* prow = P + offsets[b] + (size_t)min(row, M - 1) * N;
-        const __bf16* kt = KT + ((size_t)b * D + r) * Lk32 + 8 * q;
-        for (int j0 = 0; j0 < N; j0 += 32) {
-            const int jj = j0 + 8 * q;
-            float p[8];
-            if (live && jj + 8 <= N) {
-                const f4u u0 = *reinterpret_cast<const f4u*>(prow + jj);
-                const f4u u1 = *reinterpret_cast<const f4u*>(prow + jj + 4);
+    for (int x = 0; x < 2; ++x)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) { p[v] = u0[v]; p[4 + v] = u1[v]; }
-            } else {
+        for (int d = 0; d < NB; ++d) acc[x][d] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    if (o0 < nout && nred > 0) {
+        const float* Pb = P + offsets[b];
+        const __bf16* xt = XT + ((size_t)b * D + r) * L32 + 8 * q;
+        bool live[2];
+        const float* pp[2];
 #pragma unroll
-                for (int v = 0; v < 8; ++v) p[v] = (live && jj + v < N) ? prow[jj + v] : 0.0f;
+        for (int x = 0; x < 2; ++x) {
+            const int o = o0 + 16 * x + r;
+            live[x] = o < nout;
+            // dQ: row o of the map; dK: column o
+            pp[x] = ROWS ? Pb + (size_t)min(o, nout - 1) * N : Pb + min(o, nout - 1);
+        }
+        auto load = [&](int s0, bf16x8 (&av)[NB], float (&pv)[2][8]) {
+#pragma unroll
+            for (int d = 0; d < NB; ++d)
+                av[d] = *reinterpret_cast<const bf16x8*>(xt + (size_t)16 * d * L32 + s0);
+            const int ss = s0 + 8 * q;
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                if constexpr (ROWS) {
+                    if (live[x] && ss + 8 <= nred) {
+                        const f4u u0 = *reinterpret_cast<const f4u*>(pp[x] + ss);
+                        const f4u u1 = *reinterpret_cast<const f4u*>(pp[x] + ss + 4);
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) { pv[x][v] = u0[v]; pv[x][4 + v] = u1[v]; }
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 8; ++v) pv[x][v] = (live[x] && ss + v < nred) ? pp[x][ss + v] : 0.0f;
+                    }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 8; ++v)
+                        pv[x][v] = (live[x] && ss + v < nred) ? pp[x][(size_t)(ss + v) * N] : 0.0f;
+                }
             }
-            bf16x8 hi, lo;
-            split8(p, hi, lo);
+        };
+        bf16x8 ac[NB], an[NB];
+        float pc[2][8], pn[2][8];
+        load(0, ac, pc);
+        for (int s0 = 0; s0 < nred; s0 += 32) {
+            if (s0 + 32 < nred) load(s0 + 32, an, pn);
 #pragma unroll
-            for (int d = 0; d < NB; ++d) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8*>(kt + (size_t)16 * d * Lk32 + j0);
-                acc[d] = mma(a, hi, acc[d]);
-                acc[d] = mma(a, lo, acc[d]);
+            for (int x = 0; x < 2; ++x) {
+                bf16x8 hi, lo;
+                split8(pc[x], hi, lo);
+#pragma unroll
+                for (int d = 0; d < NB; ++d) {
+                    acc[x][d] = mma(ac[d], hi, acc[x][d]);
+                    acc[x][d] = mma(ac[d], lo, acc[x][d]);
+                }
             }
+#pragma unroll
+            for (int d = 0; d < NB; ++d) ac[d] = an[d];
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int v = 0; v < 8; ++v) pc[x][v] = pn[x][v];
         }
     }
-    if (row >= Lq) return;
-    const float g = row < M ? gv[b] : 0.0f;
+    // acc[x][d][v] = grad[row o0 + 16 x + r][16 d + 4 q + v]; zero past the length
 #pragma unroll
-    for (int d = 0; d < NB; ++d)
-        *reinterpret_cast<u32x2*>(out + 16 * d) =
-            pack4(g * acc[d][0], g * acc[d][1], g * acc[d][2], g * acc[d][3]);
-}
-
-// --------------------------------------------------------------------------
-// dK: one workgroup per 64 rows of a pair's K, a wave per 16 rows x all D
-// --------------------------------------------------------------------------
-template <int D>
-__global__ __launch_bounds__(kThreads) void embed_dk_kernel(
-    const __bf16* __restrict__ QT, const float* __restrict__ P, const int* __restrict__ lengths,
-    const int64_t* __restrict__ offsets, const float* __restrict__ gv, int Lq, int Lk, int Lq32,
-    int tiles_j, int items, __bf16* __restrict__ dK) {
-    const int w = work_item();
-    if (w >= items) return;
-    const int b = w / tiles_j, t = w - b * tiles_j;
-    const int M = clampi(lengths[2 * b], 0, Lq), N = clampi(lengths[2 * b + 1], 0, Lk);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-    const int j0 = t * kTile + 16 * wave;
-    if (j0 >= Lk) return;
-    const int col = j0 + r;
-    __bf16* out = dK + ((size_t)b * Lk + col) * D + 4 * q;
-    constexpr int NB = D / 16;
-    f32x4 acc[NB];
+    for (int x = 0; x < 2; ++x) {
+        const int o = o0 + 16 * x + r;
+        if (o >= Lout) continue;
+        const float g = o < nout ? gv[b] : 0.0f;
+        __bf16* dst = out + ((size_t)b * Lout + o) * D + 4 * q;
 #pragma unroll
-    for (int d = 0; d < NB; ++d) acc[d] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    if (j0 < N) {
-        const bool live = col < N;
-        const float* pcol = P + offsets[b] + min(col, N - 1);
-        const __bf16* qt = QT + ((size_t)b * D + r) * Lq32 + 8 * q;
-        for (int i0 = 0; i0 < M; i0 += 32) {
-            const int ii = i0 + 8 * q;
-            float p[8];
-#pragma unroll
-            for (int v = 0; v < 8; ++v) p[v] = (live && ii + v < M) ? pcol[(size_t)(ii + v) * N] : 0.0f;
-            bf16x8 hi, lo;
-            split8(p, hi, lo);
-#pragma unroll
-            for (int d = 0; d < NB; ++d) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8*>(qt + (size_t)16 * d * Lq32 + i0);
-                acc[d] = mma(a, hi, acc[d]);
-                acc[d] = mma(a, lo, acc[d]);
-            }
-        }
+        for (int d = 0; d < NB; ++d)
+            *reinterpret_cast<u32x2*>(dst + 16 * d) =
+                pack4(g * acc[x][d][0], g * acc[x][d][1], g * acc[x][d][2], g * acc[x][d][3]);
     }
-    if (col >= Lk) return;
-    const float g = col < N ? gv[b] : 0.0f;
-#pragma unroll
-    for (int d = 0; d < NB; ++d)
-        *reinterpret_cast<u32x2*>(out + 16 * d) =
-            pack4(g * acc[d][0], g * acc[d][1], g * acc[d][2], g * acc[d][3]);
 }
 
 // --------------------------------------------------------------------------
@@ -363,15 +378,14 @@ extern "C" ohm_status ohm_embed_scores(const void* Q, const void* K, const int32
     if ((st = launched("embed offsets kernel")) != OHM_OK) return st;
     if (Lq == 0 || Lk == 0) return OHM_OK;
     if (!scores) return fail(OHM_EINVAL, "scores is null");
-    const int tiles_j = (Lk + kTile - 1) / kTile;
-    const int tiles_pair = ((Lq + kTile - 1) / kTile) * tiles_j;
-    const int64_t items = (int64_t)B * tiles_pair;
+    const int tiles_i = (Lq + kRows - 1) / kRows;
+    const int64_t items = (int64_t)B * tiles_i;
     if (items > (1 << 30)) return fail(OHM_EINVAL, "too many score tiles");
     return by_dim(D, [&](auto dd) {
         constexpr int DD = decltype(dd)::value;
         hipLaunchKernelGGL(embed_scores_kernel<DD>, dim3(round_grid(items)), dim3(kThreads), 0, s,
                            static_cast<const __bf16*>(Q), static_cast<const __bf16*>(K), lengths,
-                           offsets, Lq, Lk, tiles_j, tiles_pair, (int)items, scores);
+                           offsets, Lq, Lk, tiles_i, (int)items, scores);
         return launched("embed scores kernel");
     });
 }
@@ -399,7 +413,7 @@ extern "C" ohm_status ohm_embed_grads(const void* Q, const void* K, const int32_
         return fail(OHM_EWORKSPACE, "workspace missing, misaligned or smaller than ohm_embed_grads_workspace()");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int lq32 = (int)round_up((size_t)Lq, 32), lk32 = (int)round_up((size_t)Lk, 32);
-    if ((int64_t)B * ((Lq + kTile - 1) / kTile + (Lk + kTile - 1) / kTile) > (1 << 30))
+    if ((int64_t)B * ((Lq + kRows - 1) / kRows + (Lk + kRows - 1) / kRows) > (1 << 30))
         return fail(OHM_EINVAL, "too many gradient tiles");
     __bf16* qt = static_cast<__bf16*>(workspace);
     __bf16* kt = reinterpret_cast<__bf16*>(static_cast<char*>(workspace) +
@@ -416,18 +430,18 @@ extern "C" ohm_status ohm_embed_grads(const void* Q, const void* K, const int32_
             if ((r = launched("embed transpose kernel (K)")) != OHM_OK) return r;
         }
         if (Lq > 0) {
-            const int tiles = (Lq + kTile - 1) / kTile;
+            const int tiles = (Lq + kRows - 1) / kRows;
             const int64_t items = (int64_t)B * tiles;
-            hipLaunchKernelGGL(embed_dq_kernel<DD>, dim3(round_grid(items)), dim3(kThreads), 0, s, kt, map,
-                               lengths, offsets, grad_value, Lq, Lk, lk32, tiles, (int)items,
+            hipLaunchKernelGGL((embed_grad_kernel<DD, true>), dim3(round_grid(items)), dim3(kThreads), 0, s,
+                               kt, map, lengths, offsets, grad_value, Lq, Lk, lk32, tiles, (int)items,
                                static_cast<__bf16*>(dQ));
             if ((r = launched("embed dQ kernel")) != OHM_OK) return r;
         }
         if (Lk > 0) {
-            const int tiles = (Lk + kTile - 1) / kTile;
+            const int tiles = (Lk + kRows - 1) / kRows;
             const int64_t items = (int64_t)B * tiles;
-            hipLaunchKernelGGL(embed_dk_kernel<DD>, dim3(round_grid(items)), dim3(kThreads), 0, s, qt, map,
-                               lengths, offsets, grad_value, Lq, Lk, lq32, tiles, (int)items,
+            hipLaunchKernelGGL((embed_grad_kernel<DD, false>), dim3(round_grid(items)), dim3(kThreads), 0, s,
+                               qt, map, lengths, offsets, grad_value, Lq, Lk, lq32, tiles, (int)items,
                                static_cast<__bf16*>(dK));
             if ((r = launched("embed dK kernel")) != OHM_OK) return r;
         }

@@ -71,20 +71,29 @@ def test_c3_packed_every_value_and_spread_maps(c3_batch):
           f"vs FP64 oracle {rel_err(vh, f64_v, 1e-3):.3e}", flush=True)
     assert err < 1e-5
     assert rel_err(vh, f64_v, 1e-3) < 1e-5
+    # maps and parameter gradients of 64 pairs spread over the size range,
+    # against the FP64 oracle (at a million cells the FP32 reference's own map
+    # carries ~1e-4 of rounding; its distance is reported beside)
     idx = _spread([p.size for p in pairs], 64)
     sub = [pairs[i] for i in idx]
+    tv, tm, ta, tb, tc = O.over_pairs(O.sw_affine_forward, sub, go, ge, T, prec="f64", chunk=1)
     rv, rm, ra, rb, rc = O.over_pairs(O.sw_affine_forward, sub, go, ge, T, impl=IMPL, chunk=8)
     maps = ragged.unpack(m, ln.to(DEV), off.to(DEV))
-    worst = 0.0
+    worst = worst_ref = 0.0
     for k, i in enumerate(idx):
-        e = abs_err(h(maps[i]), rm[k])
-        worst = max(worst, e)
-        assert e < 1e-4, (i, pairs[i].shape, e)
-    assert rel_err(h(a)[idx], np.asarray(ra), 1.0) < 1e-4
-    assert rel_err(h(b)[idx], np.asarray(rb), 1.0) < 1e-4
-    assert rel_err(h(c)[idx], np.asarray(rc), 1.0) < 1e-4
+        e = abs_err(h(maps[i]), tm[k])
+        worst, worst_ref = max(worst, e), max(worst_ref, abs_err(rm[k], tm[k]))
+        assert e < 1e-4, (i, pairs[i].shape, e, abs_err(rm[k], tm[k]))
+    for got, want in ((a, ta), (b, tb)):
+        assert rel_err(h(got)[idx], np.asarray(want), 1.0) < 1e-4
+    # the temperature gradient (S - E[score]) / T cancels: its FP32 noise grows
+    # with |S| / T, so it passes within 1e-4 or twice the FP32 reference's error
+    gt, t64, r32 = h(c)[idx].astype(np.float64), np.asarray(tc), np.asarray(rc, np.float64)
+    lim = np.maximum(1e-4 * np.maximum(1.0, np.abs(t64)), 2.0 * np.abs(r32 - t64))
+    assert np.all(np.abs(gt - t64) <= lim), float(np.max(np.abs(gt - t64) - lim))
     print(f"  64 spread pairs ({min(pairs[i].size for i in idx)}..{max(pairs[i].size for i in idx)} "
-          f"cells): map max abs err {worst:.3e}", flush=True)
+          f"cells): map max abs err vs FP64 {worst:.3e} (FP32 reference vs FP64: {worst_ref:.3e})",
+          flush=True)
 
 
 def test_c3_padded_mask_every_value_on_its_own_input(c3_batch):
@@ -107,11 +116,12 @@ def test_c3_padded_mask_every_value_on_its_own_input(c3_batch):
     print(f"  all {B} padded values: rel err vs FP32 reference on the sentinel input {err:.3e}",
           flush=True)
     assert err < 1e-5
-    # whole maps (masked cells included) of a few pairs
-    for i in (0, 1000, B - 1):
-        p = pairs[i]
-        r = O.sw_affine_forward(h(sent[i:i + 1]), go, ge, T, impl=IMPL)
-        assert abs_err(h(mp[i]), r[1][0]) < 1e-4, (i, p.shape)
+    # whole maps (masked cells included) of a few pairs against the FP64 oracle
+    idx = [0, 1000, B - 1]
+    t64 = O.over_pairs(O.sw_affine_forward, [h(sent[i]) for i in idx], go, ge, T, prec="f64",
+                       chunk=1)[1]
+    for k, i in enumerate(idx):
+        assert abs_err(h(mp[i]), t64[k]) < 1e-4, (i, pairs[i].shape)
 
 
 def test_c2_hvp_and_temperature_sensitivity_at_size():
