@@ -348,6 +348,33 @@ ohm_status ohm_sw_tangent_cpu(const float* scores, const int32_t* lengths,
                               float d_temp, float* map_tangent, float* contract, float* value,
                               float* map, int32_t threads);
 
+/* ------------------------------------------------------------------------ */
+/* Allocation-free entry point                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Bytes of the library's fixed device workspace (64 MiB per GPU, held from
+ * the first allocation-free call on that GPU until the process ends). */
+size_t ohm_scratch_bytes(void);
+
+/* ohm_sw_affine_forward without a workspace argument: the sweeps use the
+ * library's fixed device array instead, so the call allocates nothing and
+ * takes no caller buffer beyond inputs and outputs.  No counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops or harnesses that forbid hipMalloc).
+ * total_cells: sum_b L1_b * L2_b for the packed layout (offsets != NULL; an
+ * upper bound is accepted), ignored for the padded layout.  A batch whose
+ * workspace (ohm_sw_affine_workspace) exceeds ohm_scratch_bytes() is
+ * rejected with OHM_EWORKSPACE.  All calls on one GPU share the array:
+ * calls that overlap on one GPU (two streams, two threads) race.  The first
+ * call on each GPU looks the array up (hipGetSymbolAddress on the current
+ * device); make it outside graph capture. */
+ohm_status ohm_sw_affine_forward_noalloc(const float* scores, const int32_t* lengths,
+                                         const int64_t* offsets, int64_t total_cells, int32_t B,
+                                         int32_t L1max, int32_t L2max, float gap_open,
+                                         float gap_ext, float temp, float* value, float* map,
+                                         float* grad_open, float* grad_ext, float* grad_temp,
+                                         void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,8 @@ EXPORTS = (
     "ohm_nw_affine_tangent_cpu",
     "ohm_sv_affine_tangent",
     "ohm_sv_affine_tangent_cpu",
+    "ohm_scratch_bytes",
+    "ohm_sw_affine_forward_noalloc",
     "ohm_embed_scores",
     "ohm_embed_scores_cpu",
     "ohm_embed_grads_workspace",

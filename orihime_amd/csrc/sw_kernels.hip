@@ -1954,3 +1954,68 @@ extern "C" ohm_status ohm_sw_tangent(const float* scores, const int32_t* lengths
     if (st != OHM_OK) return st;
     return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 2, map, map_tangent}, stream);
 }
+
+// --------------------------------------------------------------------------
+// Allocation-free variant of ohm_sw_affine_forward: the workspace is a
+// fixed device array of this library, so callers that cannot allocate (code
+// under HIP graph capture, serving loops that forbid hipMalloc) can run the
+// sweeps.  Each GPU the library runs on holds its own copy of the array for
+// the life of the process; calls on one GPU share it, so calls that overlap
+// on one GPU race.  Every byte the kernels read from it is written first by
+// the same call (the planning kernel lays out and clears the claims, then
+// the sweeps write each table and boundary block before reading it).
+// --------------------------------------------------------------------------
+
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kScratchDevices];  // device address of `scratch`, per device
+
+// The array's device address on the current device (looked up once per
+// device; the host-side symbol is only a handle, never a device address).
+ohm_status current_scratch(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(OHM_ELAUNCH, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kScratchDevices) return fail(OHM_ELAUNCH, "device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(OHM_ELAUNCH, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return OHM_OK;
+}
+
+}  // namespace
+
+extern "C" size_t ohm_scratch_bytes(void) { return kScratchBytes; }
+
+extern "C" ohm_status ohm_sw_affine_forward_noalloc(const float* scores, const int32_t* lengths,
+                                                    const int64_t* offsets, int64_t total_cells,
+                                                    int32_t B, int32_t L1max, int32_t L2max,
+                                                    float gap_open, float gap_ext, float temp,
+                                                    float* value, float* map, float* grad_open,
+                                                    float* grad_ext, float* grad_temp, void* stream) {
+    if (offsets && total_cells < 0)
+        return fail(OHM_EINVAL, "the packed layout needs total_cells (the sum of L1_b * L2_b)");
+    const int op = (map || grad_open || grad_ext || grad_temp) ? OHM_OP_FORWARD : OHM_OP_VALUE;
+    const size_t need = ohm_sw_affine_workspace(op, B, L1max, L2max, offsets ? total_cells : -1);
+    if (need > kScratchBytes)
+        return fail(OHM_EWORKSPACE, "batch needs " + std::to_string(need) + " bytes of workspace; the "
+                    "allocation-free entry point has " + std::to_string(kScratchBytes) +
+                    " (use ohm_sw_affine_forward with a caller workspace)");
+    void* ws = nullptr;
+    ohm_status st = current_scratch(&ws);
+    if (st != OHM_OK) return st;
+    return ohm_sw_affine_forward(scores, lengths, offsets, B, L1max, L2max, gap_open, gap_ext, temp,
+                                 value, map, grad_open, grad_ext, grad_temp, ws, kScratchBytes,
+                                 stream);
+}
