@@ -60,7 +60,10 @@ struct PairDesc {
 //   affine: the sweep state every kCkpt steps (checkpoints; the reverse sweep
 //           recomputes the steps in between),
 //   linear and the tangent passes: every step (full table).
-constexpr int kCkpt = 8;                // steps between affine checkpoints
+#ifndef OHM_CKPT
+#define OHM_CKPT 8
+#endif
+constexpr int kCkpt = OHM_CKPT;         // steps between affine checkpoints (divides kChunk)
 constexpr int kPlanesAffineCkpt = 6;    // six state rows (log-domain sweep: M, I, D, A of the
                                         // previous step and A of the diagonal; scaled sweep:
                                         // g, h, a, c of the previous step, a and c of the diagonal)

@@ -269,6 +269,23 @@ DEV void load8(float (&dst)[8], const float* row, int c0, int len, bool ok) {
         }
     }
 }
+
+// K consecutive floats of a row (K = 4, 8 or 16: the checkpoint block width)
+template <int K>
+DEV void load_run(float (&dst)[K], const float* row, int c0, int len, bool ok) {
+    if constexpr (K == 8) {
+        load8(dst, row, c0, len, ok);
+    } else if constexpr (K == 16) {
+        load16(dst, row, c0, len, ok);
+    } else {
+        const float* p = row + c0;
+#pragma unroll
+        for (int u = 0; u < K; ++u) {
+            const int c = c0 + u;
+            dst[u] = (ok && c >= 0 && c < len) ? p[u] : 0.0f;
+        }
+    }
+}
 DEV void store8(float* row, int c0, int len, bool ok, const float (&src)[8]) {
     if (ok && c0 >= 0 && c0 + 8 <= len) {
 #pragma unroll
@@ -821,7 +838,7 @@ DEV void affine_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
 #pragma unroll
             for (int p = 0; p < kPlanesAffineCkpt; ++p) in.c[p] = kNinf;
         }
-        load8(in.sc, srow, t0 - r, L2, row_ok);
+        load_run<kCkpt>(in.sc, srow, t0 - r, L2, row_ok);
         in.qM = kNinf; in.qI = kNinf; in.qA = kNinf;
         if (has_up && lane < kCkpt && t0 + lane < L2) {
             const float* e = bup + (int64_t)(t0 + lane) * kBndStride;
@@ -850,9 +867,9 @@ DEV void affine_bwd_strip(const SwArgs& a, const PairDesc& d, int s, int lane, i
         auto body = [&](auto edge) {
             constexpr bool EDGE = decltype(edge)::value;
 #pragma unroll
-            for (int h = 1; h >= 0; --h) {
+            for (int h = kChunk / kCkpt - 1; h >= 0; --h) {
                 BlockIn in;
-                const int b = 2 * k + h;
+                const int b = (kChunk / kCkpt) * k + h;
                 load_block(in, b);
                 // 1. recompute the block (forward order)
                 float rm[kCkpt], ri[kCkpt], rd[kCkpt];
