@@ -221,37 +221,45 @@ __global__ __launch_bounds__(kThreads) void embed_transpose_kernel(const __bf16*
 
 // --------------------------------------------------------------------------
 // dQ and dK: one workgroup per 128 output rows of a pair, a wave per 32 rows
-// (two 16-row blocks) x all D.  Each reduction step of 32 takes the D/16
-// A-operand runs of the transposed copy (shared by the wave's two blocks)
-// and the map values of the two blocks; the next step's operands are
-// requested before the current step's products.
+// (two 16-row blocks) x all D.
 //   dQ^T[d, i] = sum_j K^T[d, j] P[i, j]   (map rows: 8 contiguous floats)
 //   dK^T[d, j] = sum_i Q^T[d, i] P[i, j]   (map columns: 8 rows, 64-byte
 //                                           segments across 16 lanes)
+// Per reduction step of 32, the D x 32 tile of the transposed copy is the
+// same for the four waves: the workgroup loads it once (32 bytes a thread)
+// into one of two LDS buffers (rows padded to 80 bytes, so the 16 rows a
+// wave reads at once fall in different bank groups), one step ahead; each
+// wave keeps its own map values in registers, also one step ahead.  One
+// barrier per step separates the buffer's last readers from its next writer.
 // --------------------------------------------------------------------------
+constexpr int kTileRowBytes = 80;  // 64 bytes of a 32-wide BF16 run + padding
+
 template <int D, bool ROWS>
-__global__ __launch_bounds__(kThreads) void embed_grad_kernel(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D <= 128 ? 3 : 2))) void embed_grad_kernel(
     const __bf16* __restrict__ XT, const float* __restrict__ P, const int* __restrict__ lengths,
     const int64_t* __restrict__ offsets, const float* __restrict__ gv, int Lq, int Lk, int L32,
     int tiles, int items, __bf16* __restrict__ out) {
     constexpr int NB = D / 16;
+    constexpr int kChunks = D * 4;  // 16-byte pieces of one D x 32 tile
+    constexpr int kPer = (kChunks + kThreads - 1) / kThreads;
+    __shared__ __align__(16) unsigned char tile[2][D * kTileRowBytes];
     const int w = work_item();
-    if (w >= items) return;
+    if (w >= items) return;  // uniform over the workgroup
     const int b = w / tiles, t = w - b * tiles;
     const int M = clampi(lengths[2 * b], 0, Lq), N = clampi(lengths[2 * b + 1], 0, Lk);
     // ROWS: the output rows are the map's rows (dQ), else its columns (dK)
     const int Lout = ROWS ? Lq : Lk, nout = ROWS ? M : N, nred = ROWS ? N : M;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
     const int o0 = t * (4 * 32) + 32 * wave;
-    if (o0 >= Lout) return;
     f32x4 acc[2][NB];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int d = 0; d < NB; ++d) acc[x][d] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    if (o0 < nout && nred > 0) {
+    // the workgroup runs the reduction when any of its rows is inside the pair
+    if (t * (4 * 32) < nout && nred > 0) {
         const float* Pb = P + offsets[b];
-        const __bf16* xt = XT + ((size_t)b * D + r) * L32 + 8 * q;
+        const bool mine = o0 < nout;  // this wave has rows inside the pair
         bool live[2];
         const float* pp[2];
 #pragma unroll
@@ -261,10 +269,7 @@ __global__ __launch_bounds__(kThreads) void embed_grad_kernel(
             // dQ: row o of the map; dK: column o
             pp[x] = ROWS ? Pb + (size_t)min(o, nout - 1) * N : Pb + min(o, nout - 1);
         }
-        auto load = [&](int s0, bf16x8 (&av)[NB], float (&pv)[2][8]) {
-#pragma unroll
-            for (int d = 0; d < NB; ++d)
-                av[d] = *reinterpret_cast<const bf16x8*>(xt + (size_t)16 * d * L32 + s0);
+        auto load_p = [&](int s0, float (&pv)[2][8]) {
             const int ss = s0 + 8 * q;
 #pragma unroll
             for (int x = 0; x < 2; ++x) {
@@ -285,29 +290,66 @@ __global__ __launch_bounds__(kThreads) void embed_grad_kernel(
                 }
             }
         };
-        bf16x8 ac[NB], an[NB];
+        // the workgroup's share of one D x 32 tile: piece c = thread + 256 k is
+        // row c / 4, bytes 16 (c % 4) of the row's 64
+        const __bf16* xt = XT + (size_t)b * D * L32;
+        auto load_tile = [&](int s0, u32x4 (&st)[kPer]) {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int c = threadIdx.x + kThreads * k;
+                if (c < kChunks)
+                    st[k] = *reinterpret_cast<const u32x4*>(xt + (size_t)(c >> 2) * L32 + s0 + 8 * (c & 3));
+            }
+        };
+        auto store_tile = [&](int buf, const u32x4 (&st)[kPer]) {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int c = threadIdx.x + kThreads * k;
+                if (c < kChunks)
+                    *reinterpret_cast<u32x4*>(&tile[buf][(c >> 2) * kTileRowBytes + 16 * (c & 3)]) = st[k];
+            }
+        };
+        u32x4 stage[kPer];
         float pc[2][8], pn[2][8];
-        load(0, ac, pc);
-        for (int s0 = 0; s0 < nred; s0 += 32) {
-            if (s0 + 32 < nred) load(s0 + 32, an, pn);
+        load_tile(0, stage);
+        store_tile(0, stage);
+        if (mine) load_p(0, pc);
+        __syncthreads();
+        const int nsteps = (nred + 31) / 32;
+        for (int st = 0; st < nsteps; ++st) {
+            const int buf = st & 1;
+            const bool more = st + 1 < nsteps;
+            if (more) {
+                load_tile(32 * (st + 1), stage);
+                if (mine) load_p(32 * (st + 1), pn);
+            }
+            if (mine) {
+                const unsigned char* tb = &tile[buf][r * kTileRowBytes + 16 * q];
 #pragma unroll
-            for (int x = 0; x < 2; ++x) {
-                bf16x8 hi, lo;
-                split8(pc[x], hi, lo);
+                for (int x = 0; x < 2; ++x) {
+                    bf16x8 hi, lo;
+                    split8(pc[x], hi, lo);
 #pragma unroll
-                for (int d = 0; d < NB; ++d) {
-                    acc[x][d] = mma(ac[d], hi, acc[x][d]);
-                    acc[x][d] = mma(ac[d], lo, acc[x][d]);
+                    for (int d = 0; d < NB; ++d) {
+                        const bf16x8 a = *reinterpret_cast<const bf16x8*>(tb + 16 * d * kTileRowBytes);
+                        acc[x][d] = mma(a, hi, acc[x][d]);
+                        acc[x][d] = mma(a, lo, acc[x][d]);
+                    }
                 }
             }
+            if (more) {
+                store_tile(buf ^ 1, stage);
+                if (mine) {
 #pragma unroll
-            for (int d = 0; d < NB; ++d) ac[d] = an[d];
+                    for (int x = 0; x < 2; ++x)
 #pragma unroll
-            for (int x = 0; x < 2; ++x)
-#pragma unroll
-                for (int v = 0; v < 8; ++v) pc[x][v] = pn[x][v];
+                        for (int v = 0; v < 8; ++v) pc[x][v] = pn[x][v];
+                }
+            }
+            __syncthreads();
         }
     }
+    if (o0 >= Lout) return;
     // acc[x][d][v] = grad[row o0 + 16 x + r][16 d + 4 q + v]; zero past the length
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
