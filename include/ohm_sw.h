@@ -21,7 +21,16 @@
  *             offsets[b] (row stride lengths[b,1]).
  *   lengths   int32 [B, 2] active lengths (L1, L2) of each pair, 0 <= L <= max.
  *             NULL means every pair uses the full (L1max, L2max) (padded only).
- *   offsets   int64 [B] or NULL, see above.
+ *             The GPU kernels read lengths on the device and clamp them into
+ *             [0, max]; they do not report an out-of-range length (host
+ *             callers validate, as the reference bindings do).
+ *   offsets   int64 [B] or NULL, see above.  Precondition of the packed
+ *             layout (not checked on the device): for every pair,
+ *             0 <= offsets[b] and offsets[b] + L1_b * L2_b <= total cells of
+ *             the scores buffer (L clamped as above), and the sum of
+ *             L1_b * L2_b is the total_cells given to the *_workspace query.
+ *             A workspace sized for fewer cells than the lengths ask for
+ *             turns the pairs that do not fit into NaN outputs.
  *   Map-shaped outputs and directions use the same layout as scores.  In the
  *   padded layout, cells outside a pair's lengths are written as 0.
  *   All GPU pointers are device pointers; *_cpu entry points take host

@@ -8,6 +8,10 @@ orihime/operator.py:334-386.
 
 Checks on device tensors reduce on the device and read back one bool, so an
 invalid input raises an ordinary Python exception (never a device assert).
+While the tensor's stream is being captured into a HIP graph nothing may be
+read back, so those checks are skipped (``deferred``): the kernels clamp
+lengths into [0, max] and never address outside the inputs, and a value out
+of the numerical domain gives wrong numbers, not a fault.
 """
 
 from __future__ import annotations
@@ -38,11 +42,28 @@ def _plain(t: Tensor) -> Tensor:
     return t
 
 
+def deferred(t) -> bool:
+    """True while ``t`` is a device tensor whose current stream is being
+    captured into a HIP graph, where reading device data back is forbidden."""
+    return isinstance(t, Tensor) and t.is_cuda and torch.cuda.is_current_stream_capturing()
+
+
+def host_scalar(t: Tensor, name: str) -> float:
+    """A one-element tensor parameter as a Python float (the kernels take
+    scoring parameters by value)."""
+    if deferred(t):
+        raise RuntimeError(f"{name} is a device tensor, read on the host before the launch; "
+                           "pass it as a Python number while capturing a HIP graph")
+    return float(t.detach().reshape(-1)[0].item())
+
+
 def require(condition: Tensor, message: str) -> None:
     """RuntimeError(message) unless every element of the bool ``condition``
-    holds."""
+    holds; skipped while capturing (``deferred``)."""
     if not isinstance(condition, Tensor) or condition.dtype is not torch.bool:
         raise TypeError("internal validation condition must be a bool tensor")
+    if deferred(condition):
+        return
     if not bool(_plain(condition).all().cpu()):
         raise RuntimeError(message)
 
@@ -130,7 +151,7 @@ def numerical_domain(input_name: str, scores: Tensor, names: tuple[str, ...],
     rejected.  ``names`` are the internal parameter names, "temp" among them."""
     temp = params[names.index("temp")]
     temperature(temp)
-    if scores.is_floating_point():
+    if scores.is_floating_point() and not deferred(scores):
         require(~torch.isposinf(scores),
                 f"{input_name} contains an unsupported infinity; use -inf score masks")
         bound = DOMAIN_LIMIT * temperature_like(temp, scores)
@@ -155,5 +176,5 @@ def mask_to_sentinel(scores: Tensor, temp: Param) -> Tensor:
     return torch.where(torch.isneginf(scores), (top - span * t).to(scores.dtype), scores)
 
 
-__all__ = ["DOMAIN_LIMIT", "derivative_vector", "mask_to_sentinel", "numerical_domain", "require",
-           "scalar_shape", "temperature", "temperature_like"]
+__all__ = ["DOMAIN_LIMIT", "deferred", "derivative_vector", "host_scalar", "mask_to_sentinel",
+           "numerical_domain", "require", "scalar_shape", "temperature", "temperature_like"]

@@ -25,7 +25,7 @@ from typing import Optional
 import torch
 from torch import Tensor
 
-from . import _lib
+from . import _checks, _lib
 
 _LIB = torch.library.Library("orihime", "FRAGMENT")
 
@@ -68,7 +68,11 @@ def _check_lengths(lengths: Optional[Tensor], scores: Tensor) -> Optional[Tensor
         raise RuntimeError("lengths must have shape [B, 2]")
     if lengths.dtype != torch.int32:
         raise RuntimeError("lengths must be int32")
-    if lengths.numel():
+    # The reference bindings copy lengths to the host to range-check them
+    # (src/sw_affine/torch_hip.cpp:69-99); while a HIP graph is being
+    # captured nothing can be read back, and the kernels clamp each length
+    # into [0, max] instead.
+    if lengths.numel() and not _checks.deferred(lengths):
         lo = lengths.amin(dim=0)
         hi = lengths.amax(dim=0)
         bounds = torch.stack([lo, hi]).cpu()
@@ -168,7 +172,7 @@ def _guarded(schema_name: str, impl):
 def _scalar(t: Tensor, name: str) -> float:
     if t.numel() != 1:
         raise RuntimeError(f"{name} must be a scalar tensor")
-    return float(t.detach().reshape(()).item())
+    return _checks.host_scalar(t, name)
 
 
 def _vec(scores: Tensor, n: int) -> Tensor:

@@ -522,12 +522,13 @@ struct CallOut {
 // pairs and raised claims[kClaimWsShort]; this one-block launch overwrites
 // their value, gradients, contractions and every map / map-tangent cell with
 // NaN, so the failure cannot pass for an empty alignment.  It returns at once
-// when the word is clear.
+// when the word is clear.  Lengths are clamped to [0, max] as the plan kernel
+// clamps them, so the cells written are the ones the pair owns.
 __global__ __launch_bounds__(256) void lost_pairs_kernel(const int* __restrict__ claims,
                                                          const PairDesc* __restrict__ desc,
                                                          const int32_t* __restrict__ lengths,
                                                          const int64_t* __restrict__ offsets,
-                                                         int B, CallOut o) {
+                                                         int B, int L1max, int L2max, CallOut o) {
     if (__builtin_amdgcn_readfirstlane(claims[kClaimWsShort]) == 0) return;
     const float nan = __builtin_nanf("");
     const int tid = tid_x();
@@ -542,7 +543,8 @@ __global__ __launch_bounds__(256) void lost_pairs_kernel(const int* __restrict__
     if (!o.map && !o.map_t) return;
     for (int b = 0; b < B; ++b) {
         if (!desc[b].lost) continue;
-        const int64_t n = (int64_t)max(lengths[2 * b], 0) * max(lengths[2 * b + 1], 0);
+        const int64_t n = (int64_t)min(max(lengths[2 * b], 0), L1max) *
+                          min(max(lengths[2 * b + 1], 0), L2max);
         for (int64_t q = tid; q < n; q += 256) {
             if (o.map) o.map[offsets[b] + q] = nan;
             if (o.map_t) o.map_t[offsets[b] + q] = nan;
@@ -1232,7 +1234,7 @@ ohm_status check_launch(const char* what) {
 // descriptors that setup() laid out at the front of the workspace.  (Padded
 // calls need none: setup() requires the padded bound, so every pair fits.)
 ohm_status finish_lost(const int32_t* lengths, const int64_t* offsets, int B, int L1max,
-                       void* ws, const CallOut& o, void* stream) {
+                       int L2max, void* ws, const CallOut& o, void* stream) {
     if (!offsets || B == 0) return OHM_OK;
     size_t claims, desc, order, stil, tgt, pst, grid;
     ws_fixed(B, L1max > 0 ? (L1max + kStrip - 1) / kStrip : 1, &claims, &desc, &order, &stil, &tgt,
@@ -1240,7 +1242,8 @@ ohm_status finish_lost(const int32_t* lengths, const int64_t* offsets, int B, in
     char* base = static_cast<char*>(ws);
     hipLaunchKernelGGL(lost_pairs_kernel, dim3(1), dim3(256), 0, static_cast<hipStream_t>(stream),
                        reinterpret_cast<const int*>(base + claims),
-                       reinterpret_cast<const PairDesc*>(base + desc), lengths, offsets, B, o);
+                       reinterpret_cast<const PairDesc*>(base + desc), lengths, offsets, B, L1max,
+                       L2max, o);
     return check_launch("lost-pairs kernel");
 }
 
@@ -1824,7 +1827,7 @@ extern "C" ohm_status ohm_nw_affine_tangent(const float* scores, const int32_t* 
                            direction, d_gap_open, d_gap_ext, d_temp, map_tangent, contract,
                            value, map, workspace, workspace_bytes, stream);
     if (st != OHM_OK) return st;
-    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 3, map, map_tangent}, stream);
+    return finish_lost(lengths, offsets, B, L1max, L2max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 3, map, map_tangent}, stream);
 }
 
 extern "C" ohm_status ohm_nw_affine_forward(const float* scores, const int32_t* lengths,
@@ -1838,7 +1841,7 @@ extern "C" ohm_status ohm_nw_affine_forward(const float* scores, const int32_t* 
                            value, map, grad_open, grad_ext, grad_temp, workspace,
                            workspace_bytes, stream);
     if (st != OHM_OK) return st;
-    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {grad_open, grad_ext, grad_temp}, nullptr, 0, map, nullptr}, stream);
+    return finish_lost(lengths, offsets, B, L1max, L2max, workspace, CallOut{value, {grad_open, grad_ext, grad_temp}, nullptr, 0, map, nullptr}, stream);
 }
 
 extern "C" size_t ohm_sv_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
@@ -1860,7 +1863,7 @@ extern "C" ohm_status ohm_sv_affine_tangent(const float* scores, const int32_t* 
                            direction, d_gap_open, d_gap_ext, d_temp, map_tangent, contract,
                            value, map, workspace, workspace_bytes, stream);
     if (st != OHM_OK) return st;
-    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 3, map, map_tangent}, stream);
+    return finish_lost(lengths, offsets, B, L1max, L2max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 3, map, map_tangent}, stream);
 }
 
 extern "C" ohm_status ohm_sv_affine_forward(const float* scores, const int32_t* lengths,
@@ -1874,7 +1877,7 @@ extern "C" ohm_status ohm_sv_affine_forward(const float* scores, const int32_t* 
                            value, map, grad_open, grad_ext, grad_temp, workspace,
                            workspace_bytes, stream);
     if (st != OHM_OK) return st;
-    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {grad_open, grad_ext, grad_temp}, nullptr, 0, map, nullptr}, stream);
+    return finish_lost(lengths, offsets, B, L1max, L2max, workspace, CallOut{value, {grad_open, grad_ext, grad_temp}, nullptr, 0, map, nullptr}, stream);
 }
 
 // Linear-gap Needleman-Wunsch and Saigo-Vert: the affine sweeps with one gap
@@ -1898,7 +1901,7 @@ extern "C" ohm_status ohm_nw_forward(const float* scores, const int32_t* lengths
     ohm_status st = nw_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap, gap, temp, value, map,
                            grad_gap, nullptr, grad_temp, workspace, workspace_bytes, stream, true);
     if (st != OHM_OK) return st;
-    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {grad_gap, grad_temp, nullptr}, nullptr, 0, map, nullptr}, stream);
+    return finish_lost(lengths, offsets, B, L1max, L2max, workspace, CallOut{value, {grad_gap, grad_temp, nullptr}, nullptr, 0, map, nullptr}, stream);
 }
 
 extern "C" ohm_status ohm_sv_forward(const float* scores, const int32_t* lengths,
@@ -1909,7 +1912,7 @@ extern "C" ohm_status ohm_sv_forward(const float* scores, const int32_t* lengths
     ohm_status st = sv_forward_impl(scores, lengths, offsets, B, L1max, L2max, gap, gap, temp, value, map,
                            grad_gap, nullptr, grad_temp, workspace, workspace_bytes, stream, true);
     if (st != OHM_OK) return st;
-    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {grad_gap, grad_temp, nullptr}, nullptr, 0, map, nullptr}, stream);
+    return finish_lost(lengths, offsets, B, L1max, L2max, workspace, CallOut{value, {grad_gap, grad_temp, nullptr}, nullptr, 0, map, nullptr}, stream);
 }
 
 extern "C" size_t ohm_sw_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
@@ -1930,7 +1933,7 @@ extern "C" ohm_status ohm_sw_affine_forward(const float* scores, const int32_t* 
                         value, map, grad_open, grad_ext, grad_temp, workspace, workspace_bytes,
                         stream);
     if (st != OHM_OK) return st;
-    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {grad_open, grad_ext, grad_temp}, nullptr, 0, map, nullptr}, stream);
+    return finish_lost(lengths, offsets, B, L1max, L2max, workspace, CallOut{value, {grad_open, grad_ext, grad_temp}, nullptr, 0, map, nullptr}, stream);
 }
 
 extern "C" ohm_status ohm_sw_affine_tangent(const float* scores, const int32_t* lengths,
@@ -1945,7 +1948,7 @@ extern "C" ohm_status ohm_sw_affine_tangent(const float* scores, const int32_t* 
                         direction, d_gap_open, d_gap_ext, d_temp, map_tangent, contract, value,
                         map, workspace, workspace_bytes, stream);
     if (st != OHM_OK) return st;
-    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 3, map, map_tangent}, stream);
+    return finish_lost(lengths, offsets, B, L1max, L2max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 3, map, map_tangent}, stream);
 }
 
 extern "C" ohm_status ohm_sw_forward(const float* scores, const int32_t* lengths,
@@ -1956,7 +1959,7 @@ extern "C" ohm_status ohm_sw_forward(const float* scores, const int32_t* lengths
     ohm_status st = forward_impl(false, scores, lengths, offsets, B, L1max, L2max, gap, 0.0f, temp, value,
                         map, grad_gap, grad_temp, nullptr, workspace, workspace_bytes, stream);
     if (st != OHM_OK) return st;
-    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {grad_gap, grad_temp, nullptr}, nullptr, 0, map, nullptr}, stream);
+    return finish_lost(lengths, offsets, B, L1max, L2max, workspace, CallOut{value, {grad_gap, grad_temp, nullptr}, nullptr, 0, map, nullptr}, stream);
 }
 
 extern "C" ohm_status ohm_sw_tangent(const float* scores, const int32_t* lengths,
@@ -1969,7 +1972,7 @@ extern "C" ohm_status ohm_sw_tangent(const float* scores, const int32_t* lengths
                         direction, d_gap, 0.0f, d_temp, map_tangent, contract, value, map,
                         workspace, workspace_bytes, stream);
     if (st != OHM_OK) return st;
-    return finish_lost(lengths, offsets, B, L1max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 2, map, map_tangent}, stream);
+    return finish_lost(lengths, offsets, B, L1max, L2max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 2, map, map_tangent}, stream);
 }
 
 // --------------------------------------------------------------------------

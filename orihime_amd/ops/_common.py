@@ -8,6 +8,7 @@ import warnings
 import torch
 from torch import Tensor
 
+from .. import _checks
 from ..operator import KernelCall, _validate_temperature
 
 
@@ -23,14 +24,15 @@ def warn_positive_gap(value, name: str) -> None:
 
 def scalar(param) -> float:
     if isinstance(param, Tensor):
-        return float(param.detach().reshape(-1)[0].item())
+        return _checks.host_scalar(param, "scoring parameter")
     return float(param)
 
 
 def _prefix_mask_lengths(mask: Tensor, name: str) -> Tensor:
     m = mask.to(torch.int32)
     has_hole = (m[:, 1:] - m[:, :-1] > 0).any(dim=1)
-    if has_hole.any():
+    # the hole check reads back one bool; it is skipped under graph capture
+    if not _checks.deferred(has_hole) and has_hole.any():
         bad = has_hole.nonzero(as_tuple=True)[0].tolist()
         raise ValueError(
             f"{name} is not prefix-only (has holes) at batch indices: {bad}. Masks must have "

@@ -12,6 +12,13 @@ packed layout.
     value.sum().backward()                              # flat.grad: packed map
 
 Semantics per pair are those of ``orihime_amd.sw_affine`` on that pair alone.
+
+Every call checks, on the device, that each pair's block lies inside ``flat``
+(``0 <= offsets[b]`` and ``offsets[b] + L1_b * L2_b <= flat.numel()``) and
+that the lengths lie in [0, max]; that costs one read-back.  While a HIP graph
+is being captured nothing is read back: pass ``L1max``/``L2max`` and Python
+numbers for the parameters, and the checks are left to the caller (the
+kernels still clamp lengths into [0, max]).
 """
 
 from __future__ import annotations
@@ -21,7 +28,7 @@ from collections.abc import Sequence
 import torch
 from torch import Tensor
 
-from . import _lib
+from . import _checks, _lib
 from .operator import _validate_temperature
 
 
@@ -66,14 +73,28 @@ def _check(flat: Tensor, lengths: Tensor, offsets: Tensor, L1max, L2max):
     lengths = lengths.contiguous()
     offsets = offsets.contiguous()
     if L1max is None or L2max is None:
+        if _checks.deferred(lengths):
+            raise ValueError("pass L1max and L2max while capturing a HIP graph (otherwise they "
+                             "are read from lengths on the host)")
         mx = lengths.amax(dim=0).cpu() if lengths.shape[0] else torch.zeros(2, dtype=torch.int32)
         L1max = int(mx[0]) if L1max is None else int(L1max)
         L2max = int(mx[1]) if L2max is None else int(L2max)
-    return lengths, offsets, int(L1max), int(L2max)
+    L1max, L2max = int(L1max), int(L2max)
+    if L1max < 0 or L2max < 0:
+        raise ValueError("L1max and L2max must be non-negative")
+    if lengths.shape[0] and not _checks.deferred(lengths):
+        l1, l2 = lengths[:, 0], lengths[:, 1]
+        _checks.require((l1 >= 0) & (l1 <= L1max) & (l2 >= 0) & (l2 <= L2max),
+                        f"lengths must lie in [0, L1max={L1max}] x [0, L2max={L2max}]")
+        end = offsets + l1.long() * l2.long()
+        _checks.require((offsets >= 0) & (end <= flat.numel()),
+                        "every pair's block offsets[b] + L1_b * L2_b must lie within the "
+                        f"packed scores ({flat.numel()} elements)")
+    return lengths, offsets, L1max, L2max
 
 
 def _params(*vals):
-    return tuple(float(v.detach().reshape(-1)[0].item()) if isinstance(v, Tensor) else float(v)
+    return tuple(_checks.host_scalar(v, "scoring parameter") if isinstance(v, Tensor) else float(v)
                  for v in vals)
 
 

@@ -164,3 +164,25 @@ def test_undersized_packed_workspace_makes_lost_pairs_nan():
                 assert torch.equal(short[k][b], full[k][b]), k
             assert torch.equal(short["map"][blk], full["map"][blk])
     assert not torch.isnan(full["value"]).any()
+
+
+@pytest.mark.gpu
+def test_lost_pair_nan_stays_inside_its_clamped_block():
+    """A lost pair whose lengths exceed (L1max, L2max) is clamped like every
+    other pair, so its NaN fill covers only the L1max x L2max cells it owns."""
+    print("[case] lost pair with lengths above the bounds, 3 pairs 100x100", flush=True)
+    g = torch.Generator().manual_seed(12)
+    n, tail = 30_000, 2_000
+    scores = torch.randn(n + tail, generator=g).cuda()
+    lengths = torch.tensor([(100, 100), (100, 100), (120, 100)], dtype=torch.int32).cuda()
+    offsets = torch.tensor([0, 10_000, 20_000], dtype=torch.int64).cuda()
+    need = _lib.workspace_bytes(True, _lib.OP_FORWARD, 3, 100, 100, 50)
+    ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+    out = dict(value=torch.empty(3, device="cuda"), map=torch.full((n + tail,), 7.0, device="cuda"))
+    _lib.affine_forward(scores, lengths, offsets, -8.0, -0.5, 0.5, L1max=100, L2max=100,
+                        workspace=ws, **out)
+    torch.cuda.synchronize()
+    value, mp = out["value"].cpu(), out["map"].cpu()
+    assert torch.isnan(value[2])
+    assert torch.isnan(mp[20_000:n]).all()
+    assert (mp[n:] == 7.0).all(), "NaN fill ran past the pair's clamped block"
