@@ -206,13 +206,23 @@ class Launch:
         self.value = torch.empty(B, device=dev)
         self.map = torch.empty_like(scores)
         self.grads = [torch.empty(B, device=dev) for _ in range(3)]
+        self.ws = None
+        self.size_workspace()
         total = scores.numel() if offsets is not None else -1
-        nbytes = lib.workspace_bytes(True, lib.OP_FORWARD, B, L1max, L2max, total)
-        self.ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         self.ws_value = torch.empty(lib.workspace_bytes(True, lib.OP_VALUE, B, L1max, L2max, total),
                                     dtype=torch.uint8, device=dev)
         self.value_only = torch.empty(B, device=dev)
         self.graph = None
+
+    def size_workspace(self):
+        """(Re)allocate the forward + reverse workspace for the checkpoint
+        stride in effect (the tuner may change it)."""
+        total = self.scores.numel() if self.offsets is not None else -1
+        nbytes = self.lib.workspace_bytes(True, self.lib.OP_FORWARD, self.B, self.L1max,
+                                          self.L2max, total)
+        if self.ws is None or self.ws.numel() != nbytes:
+            self.ws = None
+            self.ws = self.torch.empty(nbytes, dtype=self.torch.uint8, device=self.scores.device)
 
     def step(self):
         go, ge, T = self.params
@@ -620,7 +630,8 @@ def embed_record(torch, hip, args, dev):
             cfg = tune.autotune(scores, *params, lengths=lnd, offsets=offsets)
             del scores, offsets
         tuned = {"waves": cfg.waves, "groups_per_cu": cfg.groups_per_cu,
-                 "teams": {0: "auto", 1: "on", 2: "off", 3: "grid", 4: "fused"}[cfg.teams]}
+                 "teams": {0: "auto", 1: "on", 2: "off", 3: "grid", 4: "fused"}[cfg.teams],
+                 "checkpoint_stride": cfg.ckpt or 8}
     for _ in range(max(1, args.warmup)):
         step()
     torch.cuda.synchronize()
@@ -678,8 +689,11 @@ def _tune(torch, launch, params, args):
     from orihime_amd import tune
     t0 = time.perf_counter()
     cfg = tune.autotune(launch.scores, *params, lengths=launch.lengths, offsets=launch.offsets)
+    launch.size_workspace()
+    from orihime_amd import _lib
     return {"waves": cfg.waves, "groups_per_cu": cfg.groups_per_cu,
             "teams": {0: "auto", 1: "on", 2: "off", 3: "grid", 4: "fused"}[cfg.teams],
+            "checkpoint_stride": _lib.checkpoint_stride(),
             "tuning_s": round(time.perf_counter() - t0, 2)}
 
 

@@ -95,6 +95,18 @@ ohm_status ohm_set_stage_events(void* const* events, int32_t count);
  * by timing the caller's batch. */
 ohm_status ohm_set_launch_config(int32_t waves, int32_t groups_per_cu, int32_t teams);
 
+/* Tile hook (tuning; no reference counterpart: the reference's adjoint keeps
+ * whole DP tables).  steps: anti-diagonal steps between the checkpoints that
+ * the affine forward sweep stores for the reverse sweep (4, 8 or 16; 0 =
+ * automatic, 8).  Shorter strides store more and recompute less.  The
+ * checkpoints are exact sweep states, so results are bitwise the same for
+ * every stride.  The workspace queries use the stride set when they are
+ * called: set it before sizing a workspace.  Process-wide and unsynchronised
+ * like ohm_set_launch_config. */
+ohm_status ohm_set_checkpoint_stride(int32_t steps);
+/* The stride in effect (after resolving 0). */
+int32_t ohm_checkpoint_stride(void);
+
 /* Operation kinds for the workspace queries. */
 #define OHM_OP_VALUE 0    /* value only */
 #define OHM_OP_FORWARD 1  /* value + map + parameter gradients */

@@ -37,6 +37,8 @@ EXPORTS = (
     "ohm_version",
     "ohm_set_stage_events",
     "ohm_set_launch_config",
+    "ohm_set_checkpoint_stride",
+    "ohm_checkpoint_stride",
     "ohm_sw_affine_workspace",
     "ohm_sw_workspace",
     "ohm_sw_affine_forward",
@@ -84,6 +86,8 @@ _last_error = _sig("ohm_last_error", ctypes.c_char_p)
 _version = _sig("ohm_version", ctypes.c_char_p)
 _stage_events = _sig("ohm_set_stage_events", ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), _i32)
 _launch_config = _sig("ohm_set_launch_config", ctypes.c_int, _i32, _i32, _i32)
+_set_ckpt = _sig("ohm_set_checkpoint_stride", ctypes.c_int, _i32)
+_get_ckpt = _sig("ohm_checkpoint_stride", _i32)
 _ws_affine = _sig("ohm_sw_affine_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 _ws_linear = _sig("ohm_sw_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 _ws_nw_affine = _sig("ohm_nw_affine_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
@@ -166,6 +170,16 @@ def set_stage_events(events) -> None:
 def set_launch_config(waves: int = 0, groups_per_cu: int = 0, teams: int = 0) -> None:
     """Launch shape of the GPU kernels (0 = automatic; see include/ohm_sw.h)."""
     _check(_launch_config(waves, groups_per_cu, teams), "ohm_set_launch_config")
+
+
+def set_checkpoint_stride(steps: int = 0) -> None:
+    """Steps between the affine forward sweep's checkpoints (4, 8 or 16; 0 =
+    automatic).  Results do not depend on it; workspace sizes do."""
+    _check(_set_ckpt(steps), "ohm_set_checkpoint_stride")
+
+
+def checkpoint_stride() -> int:
+    return int(_get_ckpt())
 
 
 def workspace_bytes(affine: bool, op: int, B: int, L1: int, L2: int, total_cells: int = -1) -> int:

@@ -1288,6 +1288,18 @@ struct Geometry {
 // Launch configuration set through ohm_set_launch_config (0 = automatic);
 // it takes precedence over the OHM_* environment overrides.
 std::atomic<int> g_cfg_waves{0}, g_cfg_per_cu{0}, g_cfg_teams{0};
+// Checkpoint stride of the scaled affine sweeps (ohm_set_checkpoint_stride;
+// 0 = automatic: kCkpt).
+std::atomic<int> g_cfg_ckpt{0};
+
+int ckpt_stride() {
+    const int v = g_cfg_ckpt.load();
+    return v ? v : kCkpt;
+}
+// Table stride the plan kernel lays out for the affine forward + reverse
+// sweeps: the scaled sweeps' stride, or the log-domain sweeps' kCkpt (which
+// redo flagged pairs in the same table) when that is finer.
+int affine_table_stride(int ck) { return ck < kCkpt ? ck : kCkpt; }
 
 Geometry choose_geometry(int B, int L1max) {
     const int cus = device_cus();
@@ -1310,6 +1322,7 @@ struct Launch {
     Geometry geo;
     size_t lds;
     hipStream_t stream;
+    int ck;  // checkpoint stride of the scaled affine sweeps, read once per call
 };
 
 // Operations for setup(): which table the plan kernel lays out.
@@ -1333,12 +1346,14 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     if (offsets && !lengths) return fail(OHM_EINVAL, "the packed layout needs lengths");
     if ((int64_t)((L1max + kStrip - 1) / kStrip) * 4 > 60 * 1024)
         return fail(OHM_EINVAL, "L1 too long for one launch (max 983040 rows)");
+    L.ck = ckpt_stride();
     // the Needleman-Wunsch and Saigo-Vert passes keep full tables
     const TableSpec tf = op == kOpNw ? TableSpec{kPlanesNw, 1}
                          : op == kOpSv ? TableSpec{kPlanesSv, 1}
                          : op == kOpNwTangent ? TableSpec{kPlanesNwTan, 1}
                          : op == kOpSvTangent ? TableSpec{kPlanesSvTan, 1}
-                         : affine ? TableSpec{kPlanesAffineCkpt, kCkpt} : TableSpec{kPlanesLinearFwd, 1};
+                         : affine ? TableSpec{kPlanesAffineCkpt, affine_table_stride(L.ck)}
+                                  : TableSpec{kPlanesLinearFwd, 1};
     const TableSpec tt = {affine ? kPlanesAffineTan : kPlanesLinearTan, 1};
     WsLayout w = ws_layout(op >= kOpNw ? kOpForward : op, B, L1max, L2max, -1, tf, tt);
     if (!ws) return fail(OHM_EWORKSPACE, "workspace is null");
@@ -1384,6 +1399,17 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
     stage_mark(0, L.stream);
     return check_launch("plan kernel");
 }
+
+// Runs BODY with `constexpr int CK` set to the checkpoint stride CKV (4, 8
+// or 16).
+#define OHM_WITH_CK(CKV, BODY)                                                            \
+    do {                                                                                  \
+        switch (CKV) {                                                                    \
+            case 4: { constexpr int CK = 4; BODY; break; }                                \
+            case 16: { constexpr int CK = 16; BODY; break; }                              \
+            default: { constexpr int CK = 8; BODY; break; }                               \
+        }                                                                                 \
+    } while (0)
 
 #define OHM_DISPATCH_W(L, KERNEL_EXPR)                                                    \
     do {                                                                                  \
@@ -1504,21 +1530,21 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
             // reverse entries in words 4..6 of each block)
             if ((st = launch_grid_fill(L)) != OHM_OK) return st;
             const size_t lds = grid_cum_bytes(L.a.max_strips);
-            if (need_bwd) st = launch_grid(sw_affine_scaled_forward_grid<true>, lds, L);
-            else st = launch_grid(sw_affine_scaled_forward_grid<false>, lds, L);
+            if (need_bwd) OHM_WITH_CK(L.ck, st = launch_grid(sw_affine_scaled_forward_grid<true, CK>, lds, L));
+            else st = launch_grid(sw_affine_scaled_forward_grid<false, kCkpt>, lds, L);
             if (st != OHM_OK) return st;
         } else if (fused) {
             Launch F = L;
             F.lds = 2 * (size_t)L.a.max_strips * sizeof(int);
-            OHM_DISPATCH_W(F, (sw_affine_scaled_fused_kernel<W>));
+            OHM_WITH_CK(L.ck, OHM_DISPATCH_W(F, (sw_affine_scaled_fused_kernel<W, CK>)));
         } else if (team) {
             const size_t lds = team_lds_bytes(L.a.max_strips);
-            if (need_bwd) st = launch_team(sw_affine_scaled_forward_team<true>, lds, L);
-            else st = launch_team(sw_affine_scaled_forward_team<false>, lds, L);
+            if (need_bwd) OHM_WITH_CK(L.ck, st = launch_team(sw_affine_scaled_forward_team<true, CK>, lds, L));
+            else st = launch_team(sw_affine_scaled_forward_team<false, kCkpt>, lds, L);
             if (st != OHM_OK) return st;
         } else {
-            if (need_bwd) OHM_DISPATCH_W(L, (sw_affine_scaled_forward_kernel<true, W>));
-            else OHM_DISPATCH_W(L, (sw_affine_scaled_forward_kernel<false, W>));
+            if (need_bwd) OHM_WITH_CK(L.ck, OHM_DISPATCH_W(L, (sw_affine_scaled_forward_kernel<true, W, CK>)));
+            else OHM_DISPATCH_W(L, (sw_affine_scaled_forward_kernel<false, W, kCkpt>));
         }
         stage_mark(1, L.stream);
         L.a.sel = 1;
@@ -1538,13 +1564,15 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     if (scaled) {
         L.a.sel = 0;
         if (grid) {
-            if ((st = launch_grid(sw_affine_scaled_backward_grid, grid_bwd_lds_bytes(L.a.max_strips), L)) != OHM_OK)
-                return st;
+            OHM_WITH_CK(L.ck, st = launch_grid(sw_affine_scaled_backward_grid<CK>,
+                                               grid_bwd_lds_bytes(L.a.max_strips), L));
+            if (st != OHM_OK) return st;
         } else if (team) {
-            if ((st = launch_team(sw_affine_scaled_backward_team, team_bwd_lds_bytes(L.a.max_strips), L)) != OHM_OK)
-                return st;
+            OHM_WITH_CK(L.ck, st = launch_team(sw_affine_scaled_backward_team<CK>,
+                                               team_bwd_lds_bytes(L.a.max_strips), L));
+            if (st != OHM_OK) return st;
         } else if (!fused) {
-            OHM_DISPATCH_W(L, (sw_affine_scaled_backward_kernel<W>));
+            OHM_WITH_CK(L.ck, OHM_DISPATCH_W(L, (sw_affine_scaled_backward_kernel<W, CK>)));
         }
         stage_mark(3, L.stream);
         L.a.sel = 1;
@@ -1801,10 +1829,20 @@ extern "C" ohm_status ohm_set_launch_config(int32_t waves, int32_t groups_per_cu
     return OHM_OK;
 }
 
+extern "C" ohm_status ohm_set_checkpoint_stride(int32_t steps) {
+    if (!(steps == 0 || steps == 4 || steps == 8 || steps == 16))
+        return fail(OHM_EINVAL, "checkpoint stride must be 0 (automatic), 4, 8 or 16");
+    g_cfg_ckpt.store(steps);
+    return OHM_OK;
+}
+
+extern "C" int32_t ohm_checkpoint_stride(void) { return ckpt_stride(); }
+
 extern "C" size_t ohm_sw_affine_workspace(int32_t op, int32_t B, int32_t L1max, int32_t L2max,
                                           int64_t total_cells) {
     if (B < 0 || L1max < 0 || L2max < 0 || op < 0 || op > 2) return 0;
-    return ws_layout(op, B, L1max, L2max, total_cells, TableSpec{kPlanesAffineCkpt, kCkpt},
+    return ws_layout(op, B, L1max, L2max, total_cells,
+                     TableSpec{kPlanesAffineCkpt, affine_table_stride(ckpt_stride())},
                      TableSpec{kPlanesAffineTan, 1}).total;
 }
 

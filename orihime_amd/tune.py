@@ -1,18 +1,24 @@
-"""Launch-shape autotuner for the GPU sweeps.
+"""Launch-shape and tile autotuner for the GPU sweeps.
 
 The kernels' results do not depend on the launch shape (waves per workgroup,
 persistent workgroups per CU, per-pair workgroups or wave-scheduled team
-kernels) — only their speed does, and the best shape depends on the batch:
-many mid-size pairs favour 4 waves per pair, few long pairs the team
-kernels.  ``autotune`` times a small set of candidate shapes on the caller's
-own batch (one forward+backward call each, after a warm-up), applies the
-fastest through ``ohm_set_launch_config`` and caches it per batch signature
-(pair count bucket, maximum lengths, layout), so a training loop pays the
-search once.
+kernels) nor on the tile of the affine sweeps' adjoint (the checkpoint
+stride: anti-diagonal steps per recomputed block, 4, 8 or 16) — only their
+speed does, and the best choice depends on the batch: many mid-size pairs
+favour 4 waves per pair, few long pairs the team or grid kernels; a shorter
+stride stores more checkpoints and holds fewer registers per block.
+``autotune`` times the candidate shapes on the caller's own batch (one
+forward+backward call each, after a warm-up), then the strides under the
+fastest shape (affine sweeps only: the linear sweeps keep full tables),
+applies the winner through ``ohm_set_launch_config`` and
+``ohm_set_checkpoint_stride`` and caches it per batch signature (pair count
+bucket, maximum lengths, layout), so a training loop pays the search once.
 
     cfg = orihime_amd.tune.autotune(scores, 2.0, -0.5, 1.0, lengths=lens)
 
-``reset()`` returns to the automatic choice.
+``reset()`` returns to the automatic choice.  Workspace sizes depend on the
+stride: size workspaces after tuning (the library's own callers query the
+size on every call).
 """
 
 from __future__ import annotations
@@ -40,6 +46,8 @@ CANDIDATES = (
     (8, 0, 4),
 )
 GRID_MAX_PAIRS = 64
+# checkpoint strides of the affine sweeps (steps per recomputed block)
+STRIDES = (8, 4, 16)
 
 
 @dataclass(frozen=True)
@@ -48,6 +56,7 @@ class LaunchConfig:
     groups_per_cu: int
     teams: int
     ms: float
+    ckpt: int = 0  # checkpoint stride (0: automatic)
 
 
 _cache: dict[tuple, LaunchConfig] = {}
@@ -77,7 +86,8 @@ def _time(fn, reps):
 
 
 def autotune(scores, gap_open, gap_ext=None, temperature=1.0, *, lengths=None, offsets=None,
-             reps: int = 3, candidates=CANDIDATES, apply: bool = True) -> LaunchConfig:
+             reps: int = 3, candidates=CANDIDATES, strides=STRIDES,
+             apply: bool = True) -> LaunchConfig:
     """Choose the fastest launch shape for this batch (affine when ``gap_ext``
     is given, linear otherwise) and apply it.  ``scores`` is a GPU tensor in
     the padded ([B, L1, L2]) or packed layout (``offsets``, see
@@ -92,8 +102,14 @@ def autotune(scores, gap_open, gap_ext=None, temperature=1.0, *, lengths=None, o
         B = int(lengths.shape[0]) if lengths is not None else int(scores.shape[0])
         L1max, L2max = key[3], key[4]
         total = -1 if offsets is None else int(scores.numel())
-        ws = torch.empty(_lib.workspace_bytes(affine, _lib.OP_FORWARD, B, L1max, L2max, total),
-                         dtype=torch.uint8, device=scores.device)
+        strides = strides if affine else ()
+        # the finest stride needs the largest workspace
+        _lib.set_checkpoint_stride(min(strides) if strides else 0)
+        try:
+            nbytes = _lib.workspace_bytes(affine, _lib.OP_FORWARD, B, L1max, L2max, total)
+        finally:
+            _lib.set_checkpoint_stride()
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=scores.device)
         value = torch.empty(B, device=scores.device)
         mp = torch.empty_like(scores)
         g0, g1, g2 = (torch.empty(B, device=scores.device) for _ in range(3))
@@ -115,15 +131,27 @@ def autotune(scores, gap_open, gap_ext=None, temperature=1.0, *, lengths=None, o
                     continue
                 _lib.set_launch_config(w, g, t)
                 results.append(LaunchConfig(w, g, t, _time(call, reps)))
+            best = min(results, key=lambda c: c.ms)
+            _lib.set_launch_config(best.waves, best.groups_per_cu, best.teams)
+            default = _lib.checkpoint_stride()  # what the shapes above ran with
+            for ck in strides:
+                if ck == default:
+                    continue
+                _lib.set_checkpoint_stride(ck)
+                results.append(LaunchConfig(best.waves, best.groups_per_cu, best.teams,
+                                            _time(call, reps), ck))
         finally:
             _lib.set_launch_config()
+            _lib.set_checkpoint_stride()
         cfg = min(results, key=lambda c: c.ms)
         _cache[key] = cfg
     if apply:
         _lib.set_launch_config(cfg.waves, cfg.groups_per_cu, cfg.teams)
+        _lib.set_checkpoint_stride(cfg.ckpt)
     return cfg
 
 
 def reset() -> None:
-    """Back to the automatic launch shape (the cache is kept)."""
+    """Back to the automatic launch shape and stride (the cache is kept)."""
     _lib.set_launch_config()
+    _lib.set_checkpoint_stride()

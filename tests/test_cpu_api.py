@@ -60,6 +60,30 @@ def test_launch_config_hook_validates_and_resets():
     _lib.set_launch_config()
 
 
+def test_checkpoint_stride_hook_and_workspace_sizes():
+    """The tile hook accepts the strides that divide the chunk width, and the
+    affine workspace query follows it: a shorter stride stores more
+    checkpoints, a longer one never less than the log-domain sweeps need."""
+    assert _lib.checkpoint_stride() == 8
+    for bad in (1, 2, 3, 12, 32, -4):
+        with pytest.raises(RuntimeError, match="checkpoint stride"):
+            _lib.set_checkpoint_stride(bad)
+    sizes = {}
+    try:
+        for ck in (4, 8, 16):
+            _lib.set_checkpoint_stride(ck)
+            assert _lib.checkpoint_stride() == ck
+            sizes[ck] = _lib.workspace_bytes(True, _lib.OP_FORWARD, 64, 700, 900)
+            assert _lib.workspace_bytes(False, _lib.OP_FORWARD, 64, 700, 900) == \
+                _lib.workspace_bytes(False, _lib.OP_FORWARD, 64, 700, 900)
+    finally:
+        _lib.set_checkpoint_stride()
+    assert _lib.checkpoint_stride() == 8
+    assert sizes[4] > sizes[8] == sizes[16]
+    # the table part (6 planes per stored step) doubles from stride 8 to 4
+    assert sizes[4] - sizes[8] > 0.4 * sizes[8]
+
+
 def test_gpu_entry_points_reject_bad_arguments_without_a_device():
     # argument checks run before any HIP call
     lib = _lib._lib

@@ -356,12 +356,54 @@ def test_autotuned_launch_shape_changes_speed_not_results():
     try:
         cfg = tune.autotune(st, go, ge, T, lengths=lt)
         assert cfg.ms > 0 and (cfg.waves, cfg.groups_per_cu, cfg.teams) in [c[:3] for c in tune.CANDIDATES]
+        assert cfg.ckpt in (0,) + tune.STRIDES and _lib.checkpoint_stride() == (cfg.ckpt or 8)
         out = torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)
     finally:
         tune.reset()
     assert torch.equal(out[1], base[1])
     for k in (0, 2, 3, 4):
         assert rel_err(h(out[k]), h(base[k]), 1.0) < 1e-5
+
+
+@pytest.mark.parametrize("shape", [(4, 0, 4), (4, 0, 2), (0, 0, 1), (0, 0, 3)],
+                         ids=["fused", "per_pair", "team", "grid"])
+def test_checkpoint_stride_changes_speed_not_results(shape):
+    """The checkpoint stride (steps per recomputed block of the adjoint: 4, 8
+    or 16) is a tile choice of the autotuner.  Checkpoints hold the exact
+    sweep state and each chunk's sums run in the same order for every
+    stride, so value, map and gradients are bitwise those of stride 8, under
+    every launch shape, in the padded and the packed layout.  Pair 0 leaves
+    the scaled sweep's range and is redone by the log-domain sweeps, whose
+    checkpoints share the table laid out for the chosen stride."""
+    go, ge, T = PARAMS[0]
+    s, _, _, ln = _inputs(4, 200, 333, seed=stable_seed("ckpt stride"))
+    s[0] = -80.0
+    s[0, 20, 25] = 80.0
+    s[0, 21, 26] = 79.0
+    ln[0] = (200, 333)
+    st, lt = g(s), g(ln)
+    pairs = [s[b, :ln[b, 0], :ln[b, 1]] for b in range(4)]
+    flat, pl, po = ragged.pack([g(p) for p in pairs])
+    case(f"checkpoint strides 4/8/16, launch shape {shape}, lengths={ln.tolist()}")
+    outs = {}
+    try:
+        _lib.set_launch_config(*shape)
+        for ck in (8, 4, 16):
+            _lib.set_checkpoint_stride(ck)
+            padded = [t.clone() for t in torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)]
+            v, m, a, b, c = ragged.sw_affine_forward(flat, pl, po, gap_open_score=go,
+                                                     gap_extend_score=ge, temperature=T)
+            outs[ck] = padded + [v, m, a, b, c]
+        torch.cuda.synchronize()
+    finally:
+        _lib.set_launch_config()
+        _lib.set_checkpoint_stride()
+    for ck in (4, 16):
+        for k, (x, y) in enumerate(zip(outs[ck], outs[8])):
+            assert torch.equal(x, y), (ck, k)
+    ref = O.sw_affine_forward(s, go, ge, T, ln)
+    assert rel_err(h(outs[8][0]), ref[0], 1e-3) < 1e-5
+    assert abs_err(h(outs[8][1]), ref[1]) < 1e-4
 
 
 def test_scaled_sweep_out_of_range_pair_is_redone_in_log_domain(monkeypatch):
