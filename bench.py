@@ -18,6 +18,9 @@ accumulation, the ragged sweep, dQ and dK through the map.
 Workload "nw" / "sv": the ragged batch of "ragged" through the
 Needleman-Wunsch (global) / Saigo-Vert affine sweeps: value, map and
 parameter gradients.
+Workload "grid": the reference's own Smith-Waterman benchmark grid (22 shapes
+from 64 x 64 to 2048 x 2048, batches 1 to 128) through the public calls, with
+that harness's inputs, parameters and timing protocol.
 
 A step is one forward (value, log-partition) plus backward (marginal map and
 the gradients of gap_open, gap_extend and temperature) over the whole batch:
@@ -74,12 +77,47 @@ TRANS_PER_CELL = {"forward": 11, "reverse": 11, "fused": 22}
 BYTES_PER_CELL = {"forward": 4, "reverse": 4, "fused": 8}
 KERNEL_NAMES = {
     "plan": "sw_plan_kernel",
-    "forward": "sw_affine_scaled_forward_kernel<true, W>",
-    "reverse": "sw_affine_scaled_backward_kernel<W>",
+    "forward": "sw_affine_scaled_forward_kernel<true, W, CK>",
+    "reverse": "sw_affine_scaled_backward_kernel<W, CK>",
     # one launch running each pair's forward and reverse sweep (the launch
-    # shape the autotuner picks on the ragged batch)
-    "fused": "sw_affine_scaled_fused_kernel<W>",
+    # shape the autotuner picks on the ragged batch); W waves per workgroup,
+    # CK steps between checkpoints
+    "fused": "sw_affine_scaled_fused_kernel<W, CK>",
 }
+
+# The reference's Smith-Waterman shape catalog ("rectangular-production-v1",
+# benchmarks/grid_cases.json:884-1306 of the reference): case id, batch, L1,
+# L2 and the explicit lengths of the one ragged case (else every pair is
+# full).  Kept as data so the benchmark needs no reference files.
+RECT_CASES = [
+    ("rect-b1-64x64", 1, 64, 64, None),
+    ("rect-b1-128x128", 1, 128, 128, None),
+    ("rect-b1-256x256", 1, 256, 256, None),
+    ("rect-b1-512x512", 1, 512, 512, None),
+    ("rect-b1-1024x1024", 1, 1024, 1024, None),
+    ("rect-b1-2048x2048", 1, 2048, 2048, None),
+    ("rect-b1-128x2048", 1, 128, 2048, None),
+    ("rect-b1-2048x128", 1, 2048, 128, None),
+    ("rect-b1-512x2048", 1, 512, 2048, None),
+    ("rect-b1-2048x512", 1, 2048, 512, None),
+    ("rect-b1-511x1023", 1, 511, 1023, None),
+    ("rect-b1-1023x511", 1, 1023, 511, None),
+    ("rect-b1-1023x2047", 1, 1023, 2047, None),
+    ("rect-b1-2047x1023", 1, 2047, 1023, None),
+    ("rect-b8-64x64", 8, 64, 64, None),
+    ("rect-b32-64x64", 32, 64, 64, None),
+    ("rect-b128-64x64", 128, 64, 64, None),
+    ("rect-b8-128x128", 8, 128, 128, None),
+    ("rect-b32-128x128", 32, 128, 128, None),
+    ("rect-b8-256x256", 8, 256, 256, None),
+    ("rect-b4-512x512", 4, 512, 512, None),
+    ("rect-b4-2048x2048-ragged", 4, 2048, 2048,
+     [[2048, 2048], [1537, 997], [511, 2047], [127, 257]]),
+]
+# Scoring parameters of that harness (benchmarks/grid.py:42-48).
+GRID_PARAMS = {"sw": {"gap_score": -0.75, "temperature": 0.73},
+               "sw_affine": {"gap_open_score": -1.25, "gap_extend_score": -0.25,
+                             "temperature": 0.73}}
 
 
 def log(msg: str) -> None:
@@ -798,13 +836,84 @@ def nw_workload(torch, args, dev, hip, family="nw"):
     print(json.dumps(line), flush=True)
 
 
+def _grid_time(torch, fn, repeats, target_ms):
+    """The reference harness's protocol (benchmarks/grid.py:233-266): five
+    warm-up calls, a one-call probe, then `repeats` runs of
+    max(3, min(200, ceil(target / probe))) calls each; GPU-event ms per call."""
+    import math
+    import statistics
+
+    out = None
+    for _ in range(5):
+        out = fn()
+    torch.cuda.synchronize()
+    if out is None or not bool(torch.isfinite(out).all()):
+        raise RuntimeError("non-finite output")
+
+    def run(n):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(n):
+            fn()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / n
+
+    probe = max(run(1), 0.001)
+    n = max(3, min(200, math.ceil(target_ms / probe)))
+    samples = [run(n) for _ in range(repeats)]
+    return {"iterations": n, "median_ms": statistics.median(samples), "min_ms": min(samples),
+            "max_ms": max(samples)}
+
+
+def grid_workload(torch, args, dev):
+    """The reference's Smith-Waterman benchmark grid: every
+    rectangular-production-v1 case through the public calls ``sw`` /
+    ``sw_affine`` (map surface: forward and reverse sweeps) and ``sw_value``
+    / ``sw_affine_value`` (value surface: forward only), without autograd,
+    inputs U(-0.5, 0.5) from a CPU generator seeded 0x5A17 + case index
+    (benchmarks/grid.py:90-96,174-231).  One JSON line with every result."""
+    import orihime_amd as ohm
+
+    rows = []
+    for index, (cid, B, L1, L2, explicit) in enumerate(RECT_CASES):
+        g = torch.Generator(device="cpu").manual_seed(0x5A17 + index)
+        x = (torch.rand((B, L1, L2), generator=g, dtype=torch.float32) - 0.5).to(dev)
+        lens = explicit if explicit is not None else [[L1, L2]] * B
+        lengths = torch.tensor(lens, dtype=torch.int32, device=dev)
+        cells = sum(a * b for a, b in lens)
+        for op in ("sw_affine", "sw"):
+            for surface in ("value", "map"):
+                fn_name = op if surface == "map" else f"{op}_value"
+                fn = getattr(ohm, fn_name)
+                kw = dict(GRID_PARAMS[op], lengths=lengths)
+                with torch.no_grad():
+                    t = _grid_time(torch, lambda: fn(x, **kw), args.grid_repeats,
+                                   args.grid_target_ms)
+                row = {"case": cid, "operator": op, "surface": surface, "batch": B,
+                       "cells": cells, **{k: round(v, 5) if isinstance(v, float) else v
+                                          for k, v in t.items()},
+                       "items_per_second": round(1000.0 * B / t["median_ms"], 1),
+                       "gcups": round(cells / t["median_ms"] / 1e6, 3)}
+                rows.append(row)
+                log(f"{cid} {fn_name}: {t['median_ms']:.4f} ms ({row['gcups']} GCUPS)")
+        del x
+    line = {"metric": "reference benchmark grid (rectangular-production-v1), median ms per call",
+            "unit": "ms", "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
+            "dtype": "f32", "results": rows}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("ragged", "fixed", "long", "embed", "nw", "sv"),
+    ap.add_argument("--workload",
+                    choices=("ragged", "fixed", "long", "embed", "nw", "sv", "grid"),
                     default="ragged")
+    ap.add_argument("--grid-repeats", type=int, default=5)
+    ap.add_argument("--grid-target-ms", type=float, default=100.0)
     ap.add_argument("--no-padded", action="store_true", help="skip the padded comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-configs", action="store_true",
@@ -848,6 +957,8 @@ def main():
         return embed_workload(torch, args, dev)
     if args.workload in ("nw", "sv"):
         return nw_workload(torch, args, dev, Hip(), args.workload)
+    if args.workload == "grid":
+        return grid_workload(torch, args, dev)
 
     pairs = None
     if args.workload == "ragged":
