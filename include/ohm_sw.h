@@ -370,6 +370,25 @@ ohm_status ohm_sw_tangent_cpu(const float* scores, const int32_t* lengths,
                               float* map, int32_t threads);
 
 /* ------------------------------------------------------------------------ */
+/* Input scan (drop-in validation)                                           */
+/* ------------------------------------------------------------------------ */
+
+/* One pass over padded scores [B, L1, L2] (every cell of every box) and the
+ * lengths [B, 2] (nullable) for the checks the reference's Python engine and
+ * bindings make before a call (orihime/operator.py:249-386,
+ * src/sw_affine/torch_hip.cpp:69-99), which it runs as several torch
+ * reductions with host copies.  Writes counts[4] (device int32): [0] +inf
+ * cells, [1] NaN cells and finite cells with |x| > bound other than
+ * `marker` (the legacy finite mask value), [2] -inf cells, [3] lengths
+ * entries outside [0, L1] x [0, L2]; and pair_max[B] (device float): each
+ * box's largest non-NaN cell, -inf for an empty box (the mask sentinel's
+ * reference point).  Enqueues three small launches on `stream`; nothing
+ * needs clearing first and nothing synchronises the host. */
+ohm_status ohm_scan_inputs(const float* scores, const int32_t* lengths, int32_t B, int32_t L1,
+                           int32_t L2, float bound, float marker, int32_t* counts,
+                           float* pair_max, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Allocation-free entry point                                               */
 /* ------------------------------------------------------------------------ */
 

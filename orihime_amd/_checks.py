@@ -162,6 +162,12 @@ def numerical_domain(input_name: str, scores: Tensor, names: tuple[str, ...],
             _param_in_domain(name, p, temp)
 
 
+def _sentinel_from(scores: Tensor, top: Tensor, t: float) -> Tensor:
+    span = sum(scores.shape[1:]) * DOMAIN_LIMIT + MASK_MARGIN
+    top = torch.where(torch.isfinite(top), top, torch.zeros_like(top))
+    return torch.where(torch.isneginf(scores), (top - span * t).to(scores.dtype), scores)
+
+
 def mask_to_sentinel(scores: Tensor, temp: Param) -> Tensor:
     """Replace -inf cells by a finite score so low that no alignment through
     it can compete: the largest retained score minus (L1 + L2) * 80 + 200
@@ -170,11 +176,71 @@ def mask_to_sentinel(scores: Tensor, temp: Param) -> Tensor:
     t = temperature_like(temp, scores)
     if isinstance(t, Tensor):
         t = t.detach()
-    span = sum(scores.shape[1:]) * DOMAIN_LIMIT + MASK_MARGIN
     top = scores.amax(dim=tuple(range(1, scores.ndim)), keepdim=True).detach()
-    top = torch.where(torch.isfinite(top), top, torch.zeros_like(top))
-    return torch.where(torch.isneginf(scores), (top - span * t).to(scores.dtype), scores)
+    return _sentinel_from(scores, top, t)
 
 
-__all__ = ["DOMAIN_LIMIT", "deferred", "derivative_vector", "host_scalar", "mask_to_sentinel",
-           "numerical_domain", "require", "scalar_shape", "temperature", "temperature_like"]
+# --------------------------------------------------------------------------
+# Device fast path: the checks above in one native scan
+# --------------------------------------------------------------------------
+
+_RANGE_OK = "_ohm_range_ok"
+
+
+def lengths_checked(lengths: Tensor, L1: int, L2: int) -> bool:
+    """True when ``scan`` already checked these lengths against (L1, L2) and
+    they have not been modified in place since."""
+    return getattr(lengths, _RANGE_OK, None) == (L1, L2, lengths._version)
+
+
+def scan_applies(scores: Tensor, temp: Param, lengths: Tensor | None) -> bool:
+    """Whether ``scan`` can stand in for the torch checks: a plain contiguous
+    FP32 device tensor [B, L1, L2], a Python-number temperature, int32
+    lengths on the same device, outside graph capture."""
+    fx = torch._C._functorch
+    return (isinstance(scores, Tensor) and scores.is_cuda and scores.dim() == 3
+            and scores.dtype == torch.float32 and scores.is_contiguous()
+            and not fx.is_batchedtensor(scores) and not fx.is_gradtrackingtensor(scores)
+            and not isinstance(temp, Tensor) and not deferred(scores)
+            and (lengths is None or (lengths.is_cuda and lengths.dtype == torch.int32
+                                     and lengths.is_contiguous())))
+
+
+def scan(input_name: str, scores: Tensor, names: tuple[str, ...], params: tuple[Param, ...],
+         lengths: Tensor | None, length_error: str) -> Tensor:
+    """``numerical_domain`` + ``mask_to_sentinel`` + the lengths range check
+    for a device tensor (``scan_applies``) in one native pass over the
+    scores and one read-back: the same errors in the same order (lengths,
+    temperature, +inf, scores domain, parameters), and the same normalised
+    scores.  Without -inf cells the scores are returned as they are."""
+    from . import _lib
+
+    temp = params[names.index("temp")]
+    temperature(temp)
+    t = float(temp)
+    B = scores.shape[0]
+    counts = torch.empty(4, dtype=torch.int32, device=scores.device)
+    pair_max = torch.empty(B, dtype=torch.float32, device=scores.device)
+    # the bound in FP32, as a tensor comparison against the Python float rounds it
+    bound = float(torch.tensor(DOMAIN_LIMIT * t, dtype=torch.float32))
+    _lib.scan_inputs(scores.detach(), lengths, bound, LEGACY_MASK_VALUE, counts, pair_max)
+    pinf, bad, ninf, bad_len = (int(v) for v in counts.cpu().tolist())
+    if bad_len:
+        raise RuntimeError(length_error)
+    if lengths is not None:
+        setattr(lengths, _RANGE_OK, (scores.shape[1], scores.shape[2], lengths._version))
+    if pinf:
+        raise RuntimeError(f"{input_name} contains an unsupported infinity; use -inf score masks")
+    if bad:
+        raise RuntimeError(f"{_DOMAIN_TEXT}; {input_name} is out of domain")
+    for name, p in zip(names, params):
+        if name != "temp":
+            _param_in_domain(name, p, temp)
+    if not ninf:
+        return scores
+    return _sentinel_from(scores, pair_max.view(B, 1, 1), t)
+
+
+__all__ = ["DOMAIN_LIMIT", "deferred", "derivative_vector", "host_scalar", "lengths_checked",
+           "mask_to_sentinel", "numerical_domain", "require", "scalar_shape", "scan",
+           "scan_applies", "temperature", "temperature_like"]

@@ -39,6 +39,7 @@ EXPORTS = (
     "ohm_set_launch_config",
     "ohm_set_checkpoint_stride",
     "ohm_checkpoint_stride",
+    "ohm_scan_inputs",
     "ohm_sw_affine_workspace",
     "ohm_sw_workspace",
     "ohm_sw_affine_forward",
@@ -87,6 +88,8 @@ _version = _sig("ohm_version", ctypes.c_char_p)
 _stage_events = _sig("ohm_set_stage_events", ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), _i32)
 _launch_config = _sig("ohm_set_launch_config", ctypes.c_int, _i32, _i32, _i32)
 _set_ckpt = _sig("ohm_set_checkpoint_stride", ctypes.c_int, _i32)
+_scan = _sig("ohm_scan_inputs", ctypes.c_int, _vp, _vp, _i32, _i32, _i32, ctypes.c_float,
+             ctypes.c_float, _vp, _vp, _vp)
 _get_ckpt = _sig("ohm_checkpoint_stride", _i32)
 _ws_affine = _sig("ohm_sw_affine_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
 _ws_linear = _sig("ohm_sw_workspace", _sz, _i32, _i32, _i32, _i32, _i64)
@@ -180,6 +183,16 @@ def set_checkpoint_stride(steps: int = 0) -> None:
 
 def checkpoint_stride() -> int:
     return int(_get_ckpt())
+
+
+def scan_inputs(scores: torch.Tensor, lengths, bound: float, marker: float,
+                counts: torch.Tensor, pair_max: torch.Tensor) -> None:
+    """Device scan of padded scores [B, L1, L2] and lengths (see
+    include/ohm_sw.h, ohm_scan_inputs): counts [4] int32, pair_max [B]."""
+    B, L1, L2 = scores.shape
+    stream = torch.cuda.current_stream(scores.device).cuda_stream
+    _check(_scan(_ptr(scores), _ptr(lengths), B, L1, L2, float(bound), float(marker),
+                 _ptr(counts), _ptr(pair_max), stream), "ohm_scan_inputs")
 
 
 def workspace_bytes(affine: bool, op: int, B: int, L1: int, L2: int, total_cells: int = -1) -> int:

@@ -69,10 +69,12 @@ def _check_lengths(lengths: Optional[Tensor], scores: Tensor) -> Optional[Tensor
     if lengths.dtype != torch.int32:
         raise RuntimeError("lengths must be int32")
     # The reference bindings copy lengths to the host to range-check them
-    # (src/sw_affine/torch_hip.cpp:69-99); while a HIP graph is being
-    # captured nothing can be read back, and the kernels clamp each length
-    # into [0, max] instead.
-    if lengths.numel() and not _checks.deferred(lengths):
+    # (src/sw_affine/torch_hip.cpp:69-99).  Skipped while a HIP graph is
+    # being captured (nothing can be read back; the kernels clamp each length
+    # into [0, max]) and when the engine's input scan has just checked these
+    # lengths against this shape.
+    if (lengths.numel() and not _checks.deferred(lengths)
+            and not _checks.lengths_checked(lengths, L1, L2)):
         lo = lengths.amin(dim=0)
         hi = lengths.amax(dim=0)
         bounds = torch.stack([lo, hi]).cpu()

@@ -77,7 +77,7 @@ def build(verbose: bool = False) -> str:
     hipcc = _hipcc()
     # the device sources compile in parallel (sw_kernels.hip dominates)
     jobs, objs = [], []
-    for src in ("sw_kernels.hip", "embed_mfma.hip"):
+    for src in ("sw_kernels.hip", "embed_mfma.hip", "scan_inputs.hip"):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         jobs.append(_start([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]))
         objs.append(obj)

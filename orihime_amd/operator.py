@@ -31,6 +31,7 @@ _validate_derivative_vector = _checks.derivative_vector
 _validate_numerical_domain = _checks.numerical_domain
 _normalize_masked = _checks.mask_to_sentinel
 _graph_safe_assert = _checks.require
+_LENGTHS_RANGE = "lengths entries must lie within the padded input shape"
 
 
 # --------------------------------------------------------------------------
@@ -327,7 +328,8 @@ class Operator:
         if not all(isinstance(p, (float, int, Tensor)) and not isinstance(p, bool) for p in params):
             raise TypeError("scoring parameters must be real numbers or tensors")
 
-    def _validate_contract(self, primary: Tensor, params: tuple[Param, ...], lengths) -> None:
+    def _validate_contract(self, primary: Tensor, params: tuple[Param, ...], lengths,
+                           check_range: bool = True) -> None:
         name = self._tensor_inputs[0]
         expected = "[" + ", ".join(self._shape) + "]"
         if primary.ndim != len(self._shape):
@@ -355,9 +357,10 @@ class Operator:
                 raise TypeError(
                     f"{pname} tensor must have dtype torch.float32 outside autocast; got {p.dtype}"
                 )
-        self._validate_lengths(primary, lengths)
+        self._validate_lengths(primary, lengths, check_range)
 
-    def _validate_lengths(self, primary: Tensor, lengths: Tensor | None) -> None:
+    def _validate_lengths(self, primary: Tensor, lengths: Tensor | None,
+                          check_range: bool = True) -> None:
         if lengths is None:
             return
         if self._length_axes is None:
@@ -376,6 +379,8 @@ class Operator:
             )
         if not lengths.is_contiguous():
             raise ValueError("lengths must be contiguous")
+        if not check_range:
+            return
         ok = torch.stack(
             tuple(
                 (lengths[:, k] >= 0) & (lengths[:, k] <= primary.shape[ax])
@@ -383,7 +388,7 @@ class Operator:
             ),
             dim=-1,
         )
-        _graph_safe_assert(ok, "lengths entries must lie within the padded input shape")
+        _graph_safe_assert(ok, _LENGTHS_RANGE)
 
     def _mask_primary(self, primary: Tensor, mask: Tensor | None) -> Tensor:
         if not isinstance(primary, Tensor):
@@ -431,9 +436,18 @@ class Operator:
                 p.to(dtype) if isinstance(p, Tensor) and p.is_floating_point() else p
                 for p in params
             )
-        self._validate_contract(primary, params, lengths)
-        _validate_numerical_domain(self._tensor_inputs[0], primary, self._params, params)
-        primary = _normalize_masked(primary, params[self._params.index("temp")])
+        temp = params[self._params.index("temp")]
+        # device tensors: one native scan does the range, domain and mask
+        # checks with a single read-back (the torch checks below do each
+        # with reductions of their own)
+        fast = self._length_axes in ((-2, -1), (1, 2)) and _checks.scan_applies(primary, temp, lengths)
+        self._validate_contract(primary, params, lengths, check_range=not fast)
+        if fast:
+            primary = _checks.scan(self._tensor_inputs[0], primary, self._params, params, lengths,
+                                   _LENGTHS_RANGE)
+        else:
+            _validate_numerical_domain(self._tensor_inputs[0], primary, self._params, params)
+            primary = _normalize_masked(primary, temp)
         return primary, params, self._config_items(config)
 
     @staticmethod
