@@ -83,13 +83,16 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(const float* __restr
     for (int q = 0; q < kScanPerThread; ++q) {
         const int64_t i = start + (int64_t)q * kScanThreads + t;  // coalesced per q
         if (i < cells) {
+            // classified on the bits: the library is built without NaN
+            // semantics for float compares (-fno-honor-nans)
             const float x = base[i];
-            const bool is_nan = x != x;
-            const bool is_pinf = x == __builtin_inff();
-            const bool is_ninf = x == -__builtin_inff();
+            const uint32_t u = __builtin_bit_cast(uint32_t, x), m = u & 0x7fffffffu;
+            const bool is_nan = m > 0x7f800000u;
+            const bool is_pinf = u == 0x7f800000u;
+            const bool is_ninf = u == 0xff800000u;
             pinf += is_pinf;
             ninf += is_ninf;
-            bad += is_nan || (!is_pinf && !is_ninf && x != marker && fabsf(x) > bound);
+            bad += is_nan || (m < 0x7f800000u && x != marker && fabsf(x) > bound);
             if (!is_nan) mx = fmaxf(mx, x);
         }
     }
