@@ -1010,6 +1010,7 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
 #include "sw_teams.hiph"
 #include "sw_grid.hiph"
 #include "sw_affine_scaled.hiph"
+#include "sw_linear_scaled.hiph"
 
 // ==========================================================================
 // Linear gap: forward.  Table planes: A (cell value), X = s + LSE(A', 0).
@@ -1190,8 +1191,10 @@ __global__ __launch_bounds__(W * 64) void sw_backward_kernel(SwArgs a) {
     __shared__ float s_red[2 * W];
     const int lane = tid_x() & 63, wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     for (;;) {
-        const int p = claim_pair<W>(a, &s_pair, s_prog, 1);
+        if (a.sel && __builtin_amdgcn_readfirstlane(a.claims[kClaimFlagged]) == 0) return;  // uniform
+        const int p = claim_pair<W>(a, &s_pair, s_prog, a.sel ? 5 : 1);
         if (p < 0) break;
+        if (a.sel && __builtin_amdgcn_readfirstlane(a.pst[4 * p + kPstFlag]) == 0) continue;
         const PairDesc d = a.desc[p];
         const float St = a.stil[p];
         float acc[2] = {0.0f, 0.0f}, comp[2] = {0.0f, 0.0f};
@@ -1518,6 +1521,10 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     L.a.g1 = g1;
     L.a.g2 = g2;
     const bool scaled = affine && use_scaled(L.a);
+    // linear gap: exponent-tracked sweeps (per-pair workgroups), the
+    // log-domain sweeps for the pairs they flag; OHM_LINEAR_SCALED=0 keeps the
+    // log-domain sweeps alone
+    const bool lin_scaled = !affine && use_scaled(L.a) && env_int("OHM_LINEAR_SCALED", 1) != 0;
     const bool grid = scaled && use_grid(L.a);
     const bool fused = scaled && !grid && need_bwd && use_fused();
     const bool team = scaled && !grid && !fused && use_teams(B, L.a.max_strips);
@@ -1553,11 +1560,18 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     } else if (affine) {
         if (need_bwd) OHM_DISPATCH_W(L, (sw_affine_forward_kernel<true, W>));
         else OHM_DISPATCH_W(L, (sw_affine_forward_kernel<false, W>));
+    } else if (lin_scaled) {
+        if (need_bwd) OHM_DISPATCH_W(L, (sw_linear_scaled_forward_kernel<true, W>));
+        else OHM_DISPATCH_W(L, (sw_linear_scaled_forward_kernel<false, W>));
+        stage_mark(1, L.stream);
+        L.a.sel = 1;
+        if (need_bwd) OHM_DISPATCH_W(L, (sw_forward_kernel<true, W>));
+        else OHM_DISPATCH_W(L, (sw_forward_kernel<false, W>));
     } else {
         if (need_bwd) OHM_DISPATCH_W(L, (sw_forward_kernel<true, W>));
         else OHM_DISPATCH_W(L, (sw_forward_kernel<false, W>));
     }
-    if (!scaled) stage_mark(1, L.stream);
+    if (!scaled && !lin_scaled) stage_mark(1, L.stream);
     stage_mark(2, L.stream);
     if ((st = check_launch("forward kernel")) != OHM_OK) return st;
     if (!need_bwd) return OHM_OK;
@@ -1579,10 +1593,16 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
         OHM_DISPATCH_W(L, (sw_affine_backward_kernel<W>));
     } else if (affine) {
         OHM_DISPATCH_W(L, (sw_affine_backward_kernel<W>));
+    } else if (lin_scaled) {
+        L.a.sel = 0;
+        OHM_DISPATCH_W(L, (sw_linear_scaled_backward_kernel<W>));
+        stage_mark(3, L.stream);
+        L.a.sel = 1;
+        OHM_DISPATCH_W(L, (sw_backward_kernel<W>));
     } else {
         OHM_DISPATCH_W(L, (sw_backward_kernel<W>));
     }
-    if (!scaled) stage_mark(3, L.stream);
+    if (!scaled && !lin_scaled) stage_mark(3, L.stream);
     stage_mark(4, L.stream);
     return check_launch("backward kernel");
 }

@@ -96,6 +96,41 @@ def test_c3_packed_every_value_and_spread_maps(c3_batch):
           flush=True)
 
 
+def test_c3_batch_linear_gap_values_and_spread_maps(c3_batch, monkeypatch):
+    """The ragged batch through the linear-gap sweeps (the reference
+    harness's linear parameters, gap -0.75, T 0.73): every value and the maps
+    and gradients of 16 pairs spread over the size range against the FP64
+    oracle, for the exponent-tracked sweeps and (reported beside) the
+    log-domain sweeps."""
+    pairs = c3_batch
+    gp, T = -0.75, 0.73
+    case(f"C3 batch linear gap fwd+bwd, {len(pairs)} pairs")
+    flat, ln, off = ragged.pack([torch.from_numpy(p) for p in pairs])
+    flat, ln, off = flat.to(DEV), ln.to(DEV), off.to(DEV)
+    runs = {}
+    for env in ("1", "0"):
+        monkeypatch.setenv("OHM_LINEAR_SCALED", env)
+        runs[env] = ragged.sw_forward(flat, ln, off, gap_score=gp, temperature=T)
+        torch.cuda.synchronize()
+    v, m, a, c = runs["1"]
+    f64_v = np.asarray(O.over_pairs(O.sw_forward, pairs, gp, T, prec="f64", chunk=16)[0])
+    err, err_log = rel_err(h(v), f64_v, 1e-3), rel_err(h(runs["0"][0]), f64_v, 1e-3)
+    print(f"  all values: rel err vs FP64 {err:.3e} (log-domain sweeps {err_log:.3e})", flush=True)
+    assert err < 1e-5
+    idx = _spread([p.size for p in pairs], 16)
+    sub = [pairs[i] for i in idx]
+    tv, tm, ta, tc = O.over_pairs(O.sw_forward, sub, gp, T, prec="f64", chunk=1)
+    maps, maps_log = ragged.unpack(m, ln, off), ragged.unpack(runs["0"][1], ln, off)
+    worst = worst_log = 0.0
+    for k, i in enumerate(idx):
+        e, el = abs_err(h(maps[i]), tm[k]), abs_err(h(maps_log[i]), tm[k])
+        worst, worst_log = max(worst, e), max(worst_log, el)
+    print(f"  16 spread pairs: map max abs err vs FP64 {worst:.3e} (log-domain sweeps "
+          f"{worst_log:.3e})", flush=True)
+    assert worst < 1e-4
+    assert rel_err(h(a)[idx], np.asarray(ta), 1.0) < 1e-4
+
+
 def test_c3_padded_mask_every_value_on_its_own_input(c3_batch):
     pairs = c3_batch
     go, ge, T = -8.0, -0.5, 0.5

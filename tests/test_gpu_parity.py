@@ -431,6 +431,55 @@ def test_scaled_sweep_out_of_range_pair_is_redone_in_log_domain(monkeypatch):
     assert rel_err(h(out[2]), ref[2], 1.0) < 1e-4 and rel_err(h(out[3]), ref[3], 1.0) < 1e-4
 
 
+def test_linear_scaled_sweep_out_of_range_pair_is_redone_in_log_domain(monkeypatch):
+    """Linear gap: the exponent-tracked sweeps flag the pair whose adjacent
+    scores jump from -80 T to +80 T; the log-domain linear sweeps redo it, so
+    its outputs are the log-domain ones bit for bit, while the other pair
+    keeps the scaled result.  Both match the oracle."""
+    gp, T = -11.0, 1.0
+    rng = np.random.default_rng(7)
+    s = rng.normal(size=(2, 40, 50)).astype(np.float32)
+    s[0] = -80.0
+    s[0, 20, 25] = 80.0
+    s[0, 21, 26] = 79.0
+    ln = np.array([[40, 50], [40, 50]], np.int32)
+    case("linear scaled sweep range fallback")
+    out = [t.clone() for t in torch.ops.orihime.soft_sw_with_grads(g(s), gp, T, g(ln))]
+    monkeypatch.setenv("OHM_LINEAR_SCALED", "0")
+    logd = torch.ops.orihime.soft_sw_with_grads(g(s), gp, T, g(ln))
+    for x, y in zip(out, logd):
+        assert torch.equal(x[0], y[0])
+    ref = O.sw_forward(s, gp, T, ln)
+    assert rel_err(h(out[0]), ref[0], 1e-3) < 1e-5
+    assert abs_err(h(out[1]), ref[1]) < 1e-4
+    assert rel_err(h(out[2]), ref[2], 1.0) < 1e-4
+
+
+@pytest.mark.parametrize("params", [(-1.0, 1.0), (-0.4, 0.5), (-3.0, 0.73)], ids=lambda p: "g%g_T%g" % p)
+def test_linear_scaled_sweeps_agree_with_log_domain(params, monkeypatch):
+    """The exponent-tracked linear sweeps against the log-domain linear sweeps
+    on a ragged batch with a full 64-row strip chain (FP32 rounding apart)
+    and against the oracle on a sample of pairs."""
+    gp, T = params
+    s, _, _, ln = _inputs(6, 300, 333, seed=stable_seed(params, "linear scaled"))
+    case(f"linear scaled vs log-domain params={params} lengths={ln.tolist()}")
+    st, lt = g(s), g(ln)
+    fast = [t.clone() for t in torch.ops.orihime.soft_sw_with_grads(st, gp, T, lt)]
+    monkeypatch.setenv("OHM_LINEAR_SCALED", "0")
+    slow = torch.ops.orihime.soft_sw_with_grads(st, gp, T, lt)
+    assert rel_err(h(fast[0]), h(slow[0]), 1e-3) < 1e-5
+    assert abs_err(h(fast[1]), h(slow[1])) < 1e-4
+    assert rel_err(h(fast[2]), h(slow[2]), 1.0) < 1e-4
+    # grad_T = (S - sum P s - g dS/dg) / T cancels: compared within the FP32
+    # noise that cancellation leaves
+    assert entropy_ok(h(fast[3]), h(slow[3]).astype(np.float64), h(fast[0]), T)
+    f64 = O.sw_forward(s[:2].astype(np.float64), gp, T, ln[:2], prec="f64")
+    assert rel_err(h(fast[0])[:2], f64[0], 1e-3) < 1e-5
+    assert abs_err(h(fast[1])[:2], f64[1]) < 1e-4
+    assert rel_err(h(fast[2])[:2], f64[2], 1.0) < 1e-4
+    assert entropy_ok(h(fast[3])[:2], f64[3], f64[0], T)
+
+
 def test_scaled_hvp_out_of_range_pair_is_redone_in_log_domain(monkeypatch):
     """The exponent-tracked HVP sweeps flag the pair whose adjacent scores
     jump from -80 T to +80 T; the log-domain tangent sweeps redo it, so its
