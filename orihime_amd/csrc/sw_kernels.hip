@@ -1525,6 +1525,7 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     // log-domain sweeps for the pairs they flag; OHM_LINEAR_SCALED=0 keeps the
     // log-domain sweeps alone
     const bool lin_scaled = !affine && use_scaled(L.a) && env_int("OHM_LINEAR_SCALED", 1) != 0;
+    const bool lin_grid = lin_scaled && use_grid(L.a);
     const bool grid = scaled && use_grid(L.a);
     const bool fused = scaled && !grid && need_bwd && use_fused();
     const bool team = scaled && !grid && !fused && use_teams(B, L.a.max_strips);
@@ -1561,8 +1562,19 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
         if (need_bwd) OHM_DISPATCH_W(L, (sw_affine_forward_kernel<true, W>));
         else OHM_DISPATCH_W(L, (sw_affine_forward_kernel<false, W>));
     } else if (lin_scaled) {
-        if (need_bwd) OHM_DISPATCH_W(L, (sw_linear_scaled_forward_kernel<true, W>));
-        else OHM_DISPATCH_W(L, (sw_linear_scaled_forward_kernel<false, W>));
+        if (lin_grid) {
+            // one fill serves both sweeps (forward entries in words 0..1,
+            // reverse entries in words 4..6 of each block)
+            if ((st = launch_grid_fill(L)) != OHM_OK) return st;
+            const size_t lds = grid_cum_bytes(L.a.max_strips);
+            if (need_bwd) st = launch_grid(sw_linear_scaled_forward_grid<true>, lds, L);
+            else st = launch_grid(sw_linear_scaled_forward_grid<false>, lds, L);
+            if (st != OHM_OK) return st;
+        } else if (need_bwd) {
+            OHM_DISPATCH_W(L, (sw_linear_scaled_forward_kernel<true, W>));
+        } else {
+            OHM_DISPATCH_W(L, (sw_linear_scaled_forward_kernel<false, W>));
+        }
         stage_mark(1, L.stream);
         L.a.sel = 1;
         if (need_bwd) OHM_DISPATCH_W(L, (sw_forward_kernel<true, W>));
@@ -1595,7 +1607,12 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
         OHM_DISPATCH_W(L, (sw_affine_backward_kernel<W>));
     } else if (lin_scaled) {
         L.a.sel = 0;
-        OHM_DISPATCH_W(L, (sw_linear_scaled_backward_kernel<W>));
+        if (lin_grid) {
+            st = launch_grid(sw_linear_scaled_backward_grid, grid_bwd_lds_bytes(L.a.max_strips), L);
+            if (st != OHM_OK) return st;
+        } else {
+            OHM_DISPATCH_W(L, (sw_linear_scaled_backward_kernel<W>));
+        }
         stage_mark(3, L.stream);
         L.a.sel = 1;
         OHM_DISPATCH_W(L, (sw_backward_kernel<W>));

@@ -314,6 +314,33 @@ def test_affine_grid_kernels_match_team_kernels_and_oracle(shape, params, monkey
 
 
 @pytest.mark.parametrize("shape", [(3, 300, 517), (1, 700, 400), (4, 129, 64)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("params", [(-1.0, 1.0), (-0.4, 0.5)], ids=lambda p: "g%g_T%g" % p)
+def test_linear_grid_kernels_vs_oracle(shape, params, monkeypatch):
+    """The grid-scheduled linear sweeps (strips over every workgroup,
+    agent-scope hand-off) against the per-pair workgroup sweeps and the
+    oracle.  Per-pair sums are combined per strip in one and per wave in the
+    other, so values and gradients agree to FP32 rounding."""
+    gp, T = params
+    s, _, _, ln = _inputs(*shape, seed=stable_seed(shape, params, "linear grid"))
+    st, lt = g(s), g(ln)
+    case(f"linear grid kernels shape={shape} params={params} lengths={ln.tolist()}")
+    monkeypatch.setenv("OHM_GRID", "1")
+    out = [t.clone() for t in torch.ops.orihime.soft_sw_with_grads(st, gp, T, lt)]
+    torch.cuda.synchronize()
+    monkeypatch.setenv("OHM_GRID", "0")
+    wg = torch.ops.orihime.soft_sw_with_grads(st, gp, T, lt)
+    assert rel_err(h(out[0]), h(wg[0]), 1e-3) < 1e-6
+    assert abs_err(h(out[1]), h(wg[1])) < 1e-6
+    # against FP64: at 700 x 400 the FP32 log-domain reference's own map is
+    # ~1e-4 from it (the exponent-tracked sweeps keep ~1e-6)
+    f64 = O.sw_forward(s.astype(np.float64), gp, T, ln, prec="f64")
+    assert rel_err(h(out[0]), f64[0], 1e-3) < 1e-5
+    assert abs_err(h(out[1]), f64[1]) < 1e-4
+    assert rel_err(h(out[2]), f64[2], 1.0) < 1e-4
+    assert entropy_ok(h(out[3]), f64[3], f64[0], T)
+
+
+@pytest.mark.parametrize("shape", [(3, 300, 517), (1, 700, 400), (4, 129, 64)], ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("params", PARAMS[:2], ids=lambda p: "go%g_ge%g_T%g" % p)
 def test_affine_grid_tangent_sweeps_vs_oracle(shape, params, monkeypatch):
     """HVP, parameter sensitivities and marginals backward through the
