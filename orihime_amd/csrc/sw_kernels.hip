@@ -492,19 +492,32 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
     for (int q = tid; q < B; q += kPlanThreads) order[q] = s_idx[q];
 }
 
-// Zero the padded-layout cells of pair b outside its lengths.
+// Zero the padded-layout cells of pair b outside its lengths, with the
+// `nthr` threads of the caller (a wave or a workgroup): the tail of each of
+// the rows [0, L1) row by row, threads across its columns, then the
+// contiguous rows [L1, L1max) in 16-byte stores.  (No per-element index
+// division: a padded pair can have millions of such cells.)
 DEV void zero_padding(float* out, const PairDesc& d, int L1max, int L2max, int tid, int nthr) {
     if (!out) return;
-    const int64_t base = d.s_off;
+    float* base = out + d.s_off;
     const int L1 = d.L1, L2 = d.L2;
-    // rows [0, L1): columns [L2, L2max)
     const int w = L2max - L2;
     if (w > 0)
-        for (int64_t q = tid; q < (int64_t)L1 * w; q += nthr)
-            out[base + (q / w) * L2max + L2 + q % w] = 0.0f;
-    // rows [L1, L1max): everything
-    for (int64_t q = tid; q < (int64_t)(L1max - L1) * L2max; q += nthr)
-        out[base + (int64_t)L1 * L2max + q] = 0.0f;
+        for (int rr = 0; rr < L1; ++rr) {
+            float* row = base + (int64_t)rr * L2max + L2;
+            for (int c = tid; c < w; c += nthr) row[c] = 0.0f;
+        }
+    float* p = base + (int64_t)L1 * L2max;
+    int64_t n = (int64_t)(L1max - L1) * L2max;
+    // head up to a 16-byte boundary, then float4 stores, then the tail
+    const int head = (int)min<int64_t>(n, ((16 - ((uintptr_t)p & 15)) & 15) / 4);
+    if (tid < head) p[tid] = 0.0f;
+    p += head;
+    n -= head;
+    const int64_t n4 = n / 4;
+    float4* p4 = reinterpret_cast<float4*>(p);
+    for (int64_t q = tid; q < n4; q += nthr) p4[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (tid < (int)(n - 4 * n4)) p[4 * n4 + tid] = 0.0f;
 }
 
 // Outputs of one call, for lost_pairs_kernel.

@@ -529,6 +529,36 @@ def test_scaled_hvp_out_of_range_pair_is_redone_in_log_domain(monkeypatch):
     assert close_or_nearer_truth(hv, r, truth, tol2(r))
 
 
+def test_linear_masked_padding_stays_in_the_scaled_sweep():
+    """The linear-gap twin of the affine case below: -inf masks replaced by
+    the sentinel, gap paths running hundreds of cells into the padding; the
+    exponent-tracked linear sweep keeps every pair (no range flag) and
+    matches the FP64 oracle on the same sentinel-filled scores."""
+    from orihime_amd.operator import _normalize_masked
+    gp, T = -1.0, 0.73
+    rng = np.random.default_rng(stable_seed("linear masked padding"))
+    B, L1, L2 = 3, 300, 420
+    s = rng.normal(size=(B, L1, L2)).astype(np.float32)
+    for b, (m, n) in enumerate([(300, 420), (90, 400), (250, 60)]):
+        s[b, m:, :] = -np.inf
+        s[b, :, n:] = -np.inf
+    case("linear masked padding, scaled sweep range")
+    st = _normalize_masked(g(s), T).contiguous()
+    ws = torch.zeros(_lib.workspace_bytes(False, _lib.OP_FORWARD, B, L1, L2), dtype=torch.uint8, device=DEV)
+    out = [torch.empty(B, device=DEV), torch.empty_like(st), torch.empty(B, device=DEV),
+           torch.empty(B, device=DEV)]
+    _lib.linear_forward(st, None, None, gp, T, value=out[0], map=out[1], grad_gap=out[2],
+                        grad_temp=out[3], workspace=ws)
+    torch.cuda.synchronize()
+    flagged = int(ws[:256].view(torch.int32)[8].item())
+    assert flagged == 0, f"{flagged} masked pairs fell back to the log-domain sweeps"
+    f64 = O.sw_forward(h(st).astype(np.float64), gp, T, prec="f64")
+    assert rel_err(h(out[0]), f64[0], 1e-3) < 1e-5
+    assert abs_err(h(out[1]), f64[1]) < 1e-4
+    assert rel_err(h(out[2]), f64[2], 1.0) < 1e-4
+    assert entropy_ok(h(out[3]), f64[3], f64[0], T)
+
+
 def test_masked_padding_stays_in_the_scaled_sweep():
     """Padded storage masked with -inf (replaced by the sentinel far below the
     scores, orihime/operator.py:334-386): masked cells have E = 0 and, hundreds
