@@ -604,8 +604,10 @@ def long_record(torch, lib, hip, args, dev):
 
 def _embed_parts(torch, hip, Q, K, lnd, params, steps):
     """Mean ms of the embedding-scored step's parts, each timed alone with
-    HIP events: scores (ragged MFMA), sweeps (value + map), dQ / dK (ragged
-    MFMA, transposed copies included)."""
+    HIP events: the score kernel (ragged MFMA) and the sweeps on its output
+    (value + map); for comparison the sweeps that generate the scores
+    themselves (``fused=True``); dQ / dK (ragged MFMA, transposed copies
+    included)."""
     from orihime_amd import _lib, embed
 
     stream = torch.cuda.current_stream().cuda_stream
@@ -618,6 +620,7 @@ def _embed_parts(torch, hip, Q, K, lnd, params, steps):
         gv = torch.ones(Q.shape[0], device=Q.device)
         dQ, dK = torch.empty_like(Q), torch.empty_like(K)
         parts = {
+            "fused_sweeps": lambda: embed._fused_sweep(Q, K, lnd, go, ge, T),
             "scores": lambda: _lib.embed_scores(Q, K, lnd, offsets, scores),
             "sweeps": lambda: embed._sweep(scores, lnd, offsets, M, N, go, ge, T),
             "dq_dk": lambda: _lib.embed_grads(Q, K, lnd, offsets, post, gv, dQ, dK),
@@ -654,10 +657,10 @@ def embed_record(torch, hip, args, dev):
     Q.requires_grad_(True)
     K.requires_grad_(True)
 
-    def step():
+    def step(fused=False):
         Q.grad = None
         K.grad = None
-        soft_align_embeddings(Q, K, lnd, *params).sum().backward()
+        soft_align_embeddings(Q, K, lnd, *params, fused=fused).sum().backward()
 
     from orihime_amd import embed, tune
     tuned = "automatic"
@@ -672,8 +675,10 @@ def embed_record(torch, hip, args, dev):
                  "checkpoint_stride": cfg.ckpt or 8}
     for _ in range(max(1, args.warmup)):
         step()
+        step(True)
     torch.cuda.synchronize()
     ms, _ = timed(torch, step, args.steps)
+    ms_fused, _ = timed(torch, lambda: step(True), args.steps)
     parts = _embed_parts(torch, hip, Q.detach(), K.detach(), lnd, params, max(3, args.steps // 2))
     # FP64 errors of 16 pairs spread over the size range
     Q.grad = None
@@ -697,10 +702,13 @@ def embed_record(torch, hip, args, dev):
         gq = max(gq, float(np.abs(dQ[b, :m] - tq).max()))
         gk = max(gk, float(np.abs(dK[b, :n] - tk).max()))
     gcups = cells / ms / 1e6
-    log(f"embedding-scored fwd+bwd: {ms:.3f} ms/step, {gcups:.1f} GCUPS; parts {parts}")
+    log(f"embedding-scored fwd+bwd: {ms:.3f} ms/step ({ms_fused:.3f} with the scores generated "
+        f"inside the sweeps), {gcups:.1f} GCUPS; parts {parts}")
     return {"workload": "embedding-scored ragged soft-SW affine, Q/K [512, <=768, 128] BF16, "
-                        "M,N ~ U[128,768] (seed 4), go=-4 ge=-0.5 T=0.25: scores, sweeps, dQ/dK",
+                        "M,N ~ U[128,768] (seed 4), go=-4 ge=-0.5 T=0.25: ragged MFMA scores, "
+                        "packed sweeps, dQ/dK",
             "cells": cells, "ms_per_step": round(ms, 4), "gcups": round(gcups, 3),
+            "ms_per_step_scores_inside_sweeps": round(ms_fused, 4),
             "kernel_ms": parts, "launch_config": tuned,
             "gemm_gflop_per_step_useful": round(3 * 2.0 * 128 * cells / 1e9, 2),
             "oracle_err": {"value_rel_max": vrel, "dq_abs_max": gq, "dk_abs_max": gk,

@@ -47,6 +47,45 @@ ohm_status ohm_embed_scores_cpu(const void* Q, const void* K, const int32_t* len
                                 int32_t Lq, int32_t Lk, int32_t D, int64_t* offsets,
                                 float* scores);
 
+/* The packed offsets alone (the exclusive prefix sum of M_b * N_b), as
+ * ohm_embed_scores writes them. */
+ohm_status ohm_embed_offsets(const int32_t* lengths, int32_t B, int32_t Lq, int32_t Lk,
+                             int64_t* offsets, void* stream);
+
+/* Soft Smith-Waterman (affine) of every pair scored by Q_b K_b^T, with the
+ * scores generated inside the sweeps: each wave forms its strip's 16 x 16
+ * score blocks on the matrix cores (the BF16 products and FP32 reduction
+ * order of ohm_embed_scores, so each score is bitwise the one it writes)
+ * into LDS, so the scores never go through memory.  Outputs as
+ * ohm_sw_affine_forward on the packed layout (value, map, the three
+ * parameter gradients; map and gradients nullable) and the packed offsets.
+ * `scores` (capacity B * Lq * Lk floats) is written only for the pairs that
+ * leave the exponent-tracked range, which the log-domain sweeps then redo
+ * from it.  D is 32, 64 or 128; the gap factors must be in the
+ * exponent-tracked range (|gap| log2(e) / T <= 120; otherwise: OHM_EINVAL,
+ * use ohm_embed_scores and ohm_sw_affine_forward).  Workspace:
+ * ohm_sw_affine_workspace(OHM_OP_FORWARD, B, Lq, Lk, B * Lq * Lk).
+ * Bit for bit the result of ohm_embed_scores + ohm_sw_affine_forward; on
+ * short pairs (512 pairs of U[128, 768]^2, D = 128) those two are the
+ * faster route (1.53 against 2.27 ms): the sweeps there are latency-bound
+ * and the score blocks sit on each wave's critical path.
+ * Replaces: the caller-side Q K^T GEMM and the sweep over its result
+ * (orihime/sw.py:116-141 with orihime/ops/sw_affine.py). */
+ohm_status ohm_embed_sw_affine_forward(const void* Q, const void* K, const int32_t* lengths,
+                                       int32_t B, int32_t Lq, int32_t Lk, int32_t D,
+                                       float gap_open, float gap_ext, float temp,
+                                       int64_t* offsets, float* scores, float* value, float* map,
+                                       float* grad_open, float* grad_ext, float* grad_temp,
+                                       void* workspace, size_t workspace_bytes, void* stream);
+/* Host twin: ohm_embed_scores_cpu, then ohm_sw_affine_forward_cpu (scores
+ * written for every pair). */
+ohm_status ohm_embed_sw_affine_forward_cpu(const void* Q, const void* K, const int32_t* lengths,
+                                           int32_t B, int32_t Lq, int32_t Lk, int32_t D,
+                                           float gap_open, float gap_ext, float temp,
+                                           int64_t* offsets, float* scores, float* value,
+                                           float* map, float* grad_open, float* grad_ext,
+                                           float* grad_temp, int32_t threads);
+
 /* Workspace bytes of ohm_embed_grads (transposed copies of Q and K). */
 size_t ohm_embed_grads_workspace(int32_t B, int32_t Lq, int32_t Lk, int32_t D);
 

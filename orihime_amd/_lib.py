@@ -40,6 +40,9 @@ EXPORTS = (
     "ohm_set_checkpoint_stride",
     "ohm_checkpoint_stride",
     "ohm_scan_inputs",
+    "ohm_embed_offsets",
+    "ohm_embed_sw_affine_forward",
+    "ohm_embed_sw_affine_forward_cpu",
     "ohm_sw_affine_workspace",
     "ohm_sw_workspace",
     "ohm_sw_affine_forward",
@@ -412,6 +415,13 @@ _emb_scores = _sig("ohm_embed_scores", ctypes.c_int, _vp, _vp, _vp, _i32, _i32, 
                    _vp, _vp)
 _emb_scores_cpu = _sig("ohm_embed_scores_cpu", ctypes.c_int, _vp, _vp, _vp, _i32, _i32, _i32, _i32,
                        _vp, _vp)
+_emb_offsets = _sig("ohm_embed_offsets", ctypes.c_int, _vp, _i32, _i32, _i32, _vp, _vp)
+_emb_sw = _sig("ohm_embed_sw_affine_forward", ctypes.c_int, _vp, _vp, _vp, _i32, _i32, _i32, _i32,
+               ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+               _vp, _sz, _vp)
+_emb_sw_cpu = _sig("ohm_embed_sw_affine_forward_cpu", ctypes.c_int, _vp, _vp, _vp, _i32, _i32, _i32,
+                   _i32, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp, _vp, _vp,
+                   _vp, _vp, _i32)
 _emb_ws = _sig("ohm_embed_grads_workspace", _sz, _i32, _i32, _i32, _i32)
 _emb_grads = _sig("ohm_embed_grads", ctypes.c_int, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _vp,
                   _vp, _vp, _vp, _vp, _sz, _vp)
@@ -431,6 +441,29 @@ def embed_scores(Q: torch.Tensor, K: torch.Tensor, lengths: torch.Tensor, offset
         _check(_emb_scores(*args, stream), "ohm_embed_scores")
     else:
         _check(_emb_scores_cpu(*args), "ohm_embed_scores_cpu")
+
+
+def embed_sw_affine(Q: torch.Tensor, K: torch.Tensor, lengths: torch.Tensor, gap_open: float,
+                    gap_ext: float, temp: float, *, offsets: torch.Tensor, scores: torch.Tensor,
+                    value: torch.Tensor, map=None, grad_open=None, grad_ext=None, grad_temp=None,
+                    workspace: torch.Tensor | None = None) -> None:
+    """Affine soft-SW of every pair scored by ``Q_b K_b^T``, the scores
+    generated inside the sweeps (include/ohm_embed.h,
+    ohm_embed_sw_affine_forward).  ``scores`` (capacity B * Lq * Lk) is
+    written only for pairs the log-domain sweeps redo; ``offsets`` is
+    written."""
+    B, Lq, D = Q.shape
+    Lk = K.shape[1]
+    args = (_ptr(Q), _ptr(K), _ptr(lengths), B, Lq, Lk, D, float(gap_open), float(gap_ext),
+            float(temp), _ptr(offsets), _ptr(scores), _ptr(value), _ptr(map), _ptr(grad_open),
+            _ptr(grad_ext), _ptr(grad_temp))
+    if Q.is_cuda:
+        stream = torch.cuda.current_stream(Q.device).cuda_stream
+        need = workspace_bytes(True, OP_FORWARD, B, Lq, Lk, B * Lq * Lk)
+        ws = workspace if workspace is not None else _workspace(Q.device, need, stream)
+        _check(_emb_sw(*args, _ptr(ws), ws.numel(), stream), "ohm_embed_sw_affine_forward")
+    else:
+        _check(_emb_sw_cpu(*args, _cpu_threads()), "ohm_embed_sw_affine_forward_cpu")
 
 
 def embed_grads(Q: torch.Tensor, K: torch.Tensor, lengths: torch.Tensor, offsets: torch.Tensor,

@@ -124,3 +124,17 @@ extern "C" ohm_status ohm_embed_grads_cpu(const void* Q, const void* K, const in
     });
     return OHM_OK;
 }
+
+extern "C" ohm_status ohm_embed_sw_affine_forward_cpu(const void* Q, const void* K,
+                                                      const int32_t* lengths, int32_t B, int32_t Lq,
+                                                      int32_t Lk, int32_t D, float gap_open,
+                                                      float gap_ext, float temp, int64_t* offsets,
+                                                      float* scores, float* value, float* map,
+                                                      float* grad_open, float* grad_ext,
+                                                      float* grad_temp, int32_t threads) {
+    if (D != 32 && D != 64 && D != 128) return fail(OHM_EINVAL, "the fused embedding sweeps take D = 32, 64 or 128");
+    ohm_status st = ohm_embed_scores_cpu(Q, K, lengths, B, Lq, Lk, D, offsets, scores);
+    if (st != OHM_OK || B == 0) return st;
+    return ohm_sw_affine_forward_cpu(scores, lengths, offsets, B, Lq, Lk, gap_open, gap_ext, temp, value,
+                                     map, grad_open, grad_ext, grad_temp, threads);
+}

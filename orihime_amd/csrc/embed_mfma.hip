@@ -408,6 +408,16 @@ size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace
 
+extern "C" ohm_status ohm_embed_offsets(const int32_t* lengths, int32_t B, int32_t Lq, int32_t Lk,
+                                        int64_t* offsets, void* stream) {
+    if (B < 0 || Lq < 0 || Lk < 0) return fail(OHM_EINVAL, "negative batch size or length");
+    if (B == 0) return OHM_OK;
+    if (!lengths || !offsets) return fail(OHM_EINVAL, "lengths and offsets must be non-null");
+    hipLaunchKernelGGL(embed_offsets_kernel, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream),
+                       lengths, B, Lq, Lk, offsets);
+    return launched("embed offsets kernel");
+}
+
 extern "C" ohm_status ohm_embed_scores(const void* Q, const void* K, const int32_t* lengths,
                                        int32_t B, int32_t Lq, int32_t Lk, int32_t D,
                                        int64_t* offsets, float* scores, void* stream) {

@@ -495,8 +495,8 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
 // Zero the padded-layout cells of pair b outside its lengths, with the
 // `nthr` threads of the caller (a wave or a workgroup): the tail of each of
 // the rows [0, L1) row by row, threads across its columns, then the
-// contiguous rows [L1, L1max) in 16-byte stores.  (No per-element index
-// division: a padded pair can have millions of such cells.)
+// contiguous rows [L1, L1max).  (No per-element index division: a padded
+// pair can have millions of such cells.)
 DEV void zero_padding(float* out, const PairDesc& d, int L1max, int L2max, int tid, int nthr) {
     if (!out) return;
     float* base = out + d.s_off;
@@ -508,16 +508,8 @@ DEV void zero_padding(float* out, const PairDesc& d, int L1max, int L2max, int t
             for (int c = tid; c < w; c += nthr) row[c] = 0.0f;
         }
     float* p = base + (int64_t)L1 * L2max;
-    int64_t n = (int64_t)(L1max - L1) * L2max;
-    // head up to a 16-byte boundary, then float4 stores, then the tail
-    const int head = (int)min<int64_t>(n, ((16 - ((uintptr_t)p & 15)) & 15) / 4);
-    if (tid < head) p[tid] = 0.0f;
-    p += head;
-    n -= head;
-    const int64_t n4 = n / 4;
-    float4* p4 = reinterpret_cast<float4*>(p);
-    for (int64_t q = tid; q < n4; q += nthr) p4[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (tid < (int)(n - 4 * n4)) p[4 * n4 + tid] = 0.0f;
+    const int64_t n = (int64_t)(L1max - L1) * L2max;
+    for (int64_t q = tid; q < n; q += nthr) p[q] = 0.0f;
 }
 
 // Outputs of one call, for lost_pairs_kernel.
@@ -1024,6 +1016,7 @@ __global__ __launch_bounds__(W * 64) void sw_affine_backward_kernel(SwArgs a) {
 #include "sw_grid.hiph"
 #include "sw_affine_scaled.hiph"
 #include "sw_linear_scaled.hiph"
+#include "embed_fused.hiph"
 
 // ==========================================================================
 // Linear gap: forward.  Table planes: A (cell value), X = s + LSE(A', 0).
@@ -1441,8 +1434,8 @@ ohm_status setup(int op, bool affine, const float* scores, const int32_t* length
 // Team kernels (wave-level strip scheduling, one workgroup of kTeamWaves
 // waves per CU) pay off for batches of few large pairs (measured: 4096-long
 // pairs, 16 per batch, 10% faster); for batches with at least a quarter as
-// many pairs as CUs the per-pair workgroup kernels are faster (C2/C3-like
-// batches: 15-20%).  They hold per-slot progress counters and partials in
+// many pairs as CUs the per-pair workgroup kernels are faster (256 x 512^2
+// and ragged batches: 15-20%).  They hold per-slot progress counters and partials in
 // LDS, and the reverse sweep its waves' map rings; past that size (very tall
 // pairs) the workgroup kernels run.  OHM_TEAMS=0/1 forces the choice.
 constexpr size_t kLdsLimit = 160 * 1024;
@@ -2061,6 +2054,73 @@ extern "C" ohm_status ohm_sw_tangent(const float* scores, const int32_t* lengths
                         workspace, workspace_bytes, stream);
     if (st != OHM_OK) return st;
     return finish_lost(lengths, offsets, B, L1max, L2max, workspace, CallOut{value, {nullptr, nullptr, nullptr}, contract, 2, map, map_tangent}, stream);
+}
+
+// --------------------------------------------------------------------------
+// Embedding-scored affine sweeps, scores generated inside the sweep
+// (include/ohm_embed.h; kernels in embed_fused.hiph)
+// --------------------------------------------------------------------------
+
+extern "C" ohm_status ohm_embed_offsets(const int32_t* lengths, int32_t B, int32_t Lq, int32_t Lk,
+                                        int64_t* offsets, void* stream);
+
+extern "C" ohm_status ohm_embed_sw_affine_forward(const void* Q, const void* K, const int32_t* lengths,
+                                                  int32_t B, int32_t Lq, int32_t Lk, int32_t D,
+                                                  float gap_open, float gap_ext, float temp,
+                                                  int64_t* offsets, float* scores, float* value,
+                                                  float* map, float* grad_open, float* grad_ext,
+                                                  float* grad_temp, void* workspace,
+                                                  size_t workspace_bytes, void* stream) {
+    if (B < 0 || Lq < 0 || Lk < 0) return fail(OHM_EINVAL, "negative batch size or length");
+    if (!(D == 32 || D == 64 || D == 128))
+        return fail(OHM_EINVAL, "the fused embedding sweeps take D = 32, 64 or 128 "
+                                "(D = 256: ohm_embed_scores, then ohm_sw_affine_forward)");
+    if (B == 0) return OHM_OK;
+    if (!lengths || !offsets || !value) return fail(OHM_EINVAL, "lengths, offsets and value must be non-null");
+    if (Lq > 0 && Lk > 0 && (!Q || !K || !scores)) return fail(OHM_EINVAL, "Q, K and scores must be non-null");
+    ohm_status st = ohm_embed_offsets(lengths, B, Lq, Lk, offsets, stream);
+    if (st != OHM_OK) return st;
+    Launch L;
+    st = setup(kOpForward, true, scores, lengths, offsets, B, Lq, Lk, gap_open, gap_ext, temp, workspace,
+               workspace_bytes, stream, L);
+    if (st != OHM_OK) return st;
+    if (!use_scaled(L.a))
+        return fail(OHM_EINVAL, "gap factors outside the exponent-tracked sweeps' range "
+                                "(ohm_embed_scores, then ohm_sw_affine_forward)");
+    L.a.value = value;
+    L.a.map = map;
+    L.a.g0 = grad_open;
+    L.a.g1 = grad_ext;
+    L.a.g2 = grad_temp;
+    const EmbArgs e{static_cast<const __bf16*>(Q), static_cast<const __bf16*>(K), scores, Lq, Lk};
+    constexpr int kW = 4;
+    const size_t lds = embed_fused_lds_bytes(kW, L.a.max_strips);
+    // two workgroups per CU fit the LDS (rings and map rings ~67 KB each)
+    const unsigned groups = (unsigned)std::max(1, std::min(B, 2 * device_cus()));
+    auto launch = [&](auto kernel) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+            return fail(OHM_ELAUNCH, "cannot reserve LDS for the embedding sweep kernel");
+        hipLaunchKernelGGL(kernel, dim3(groups), dim3(64 * kW), lds, L.stream, L.a, e);
+        return check_launch("embedding sweep kernel");
+    };
+    if (D == 32) st = launch(embed_sw_fused_kernel<kW, 32>);
+    else if (D == 64) st = launch(embed_sw_fused_kernel<kW, 64>);
+    else st = launch(embed_sw_fused_kernel<kW, 128>);
+    if (st != OHM_OK) return st;
+    stage_mark(1, L.stream);
+    stage_mark(2, L.stream);
+    // pairs the scaled sweep flagged: the log-domain sweeps, on the scores
+    // the fused kernel wrote for them
+    L.a.sel = 1;
+    OHM_DISPATCH_W(L, (sw_affine_forward_kernel<true, W>));
+    if ((st = check_launch("embedding fallback forward")) != OHM_OK) return st;
+    stage_mark(3, L.stream);
+    OHM_DISPATCH_W(L, (sw_affine_backward_kernel<W>));
+    stage_mark(4, L.stream);
+    if ((st = check_launch("embedding fallback backward")) != OHM_OK) return st;
+    return finish_lost(lengths, offsets, B, Lq, Lk, workspace,
+                       CallOut{value, {grad_open, grad_ext, grad_temp}, nullptr, 0, map, nullptr}, stream);
 }
 
 // --------------------------------------------------------------------------

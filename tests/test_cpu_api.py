@@ -476,6 +476,9 @@ def test_embedding_scored_alignment_cpu_matches_fp64_oracle():
     assert float(Q.grad[1, 31:].abs().max()) == 0.0 and float(K.grad[1, 22:].abs().max()) == 0.0
     with pytest.raises(TypeError, match="bfloat16"):
         soft_align_embeddings(Q.detach().float(), K.detach().float(), ln, go, ge, T)
+    # the in-sweep score generation is a GPU path of its own
+    with pytest.raises(ValueError, match="fused sweeps take GPU tensors"):
+        soft_align_embeddings(Q, K, ln, go, ge, T, fused=True)
 
 
 @pytest.mark.parametrize("affine", [True, False], ids=["affine", "linear"])

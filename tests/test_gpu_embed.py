@@ -106,3 +106,122 @@ def test_grads_and_scores_are_bitwise_repeatable():
         outs.append((S, dQ, dK))
     for a, b in zip(*outs):
         assert torch.equal(a, b)
+
+
+# --------------------------------------------------------------------------
+# Scores generated inside the sweeps (ohm_embed_sw_affine_forward)
+# --------------------------------------------------------------------------
+
+from orihime_amd import embed  # noqa: E402
+
+FUSED_CASES = [
+    # B, Lq, Lk, D, lengths
+    (5, 300, 257, 128, [[300, 257], [1, 200], [299, 3], [150, 150], [64, 64]]),
+    (4, 70, 90, 64, [[70, 90], [1, 1], [33, 65], [0, 7]]),
+    (3, 200, 130, 32, [[200, 130], [129, 1], [65, 128]]),
+]
+
+
+def _unfused(Qd, Kd, ln, B, Lq, Lk, params):
+    """Score kernel, then the per-pair fused forward + reverse sweep kernel
+    with four waves: the launch the embedding sweep kernel mirrors."""
+    off = torch.empty(B, dtype=torch.int64, device=DEV)
+    S = torch.zeros(B * Lq * Lk, device=DEV)
+    _lib.embed_scores(Qd, Kd, ln, off, S)
+    out = [torch.empty(B, device=DEV), torch.full_like(S, float("nan"))] + \
+        [torch.empty(B, device=DEV) for _ in range(3)]
+    try:
+        _lib.set_launch_config(4, 0, 4)
+        _lib.affine_forward(S, ln, off, *params, value=out[0], map=out[1], grad_open=out[2],
+                            grad_ext=out[3], grad_temp=out[4], L1max=Lq, L2max=Lk)
+    finally:
+        _lib.set_launch_config()
+    return off, S, out
+
+
+def _fused(Qd, Kd, ln, B, Lq, Lk, params):
+    off = torch.empty(B, dtype=torch.int64, device=DEV)
+    S = torch.full((B * Lq * Lk,), float("nan"), device=DEV)
+    out = [torch.empty(B, device=DEV), torch.full_like(S, float("nan"))] + \
+        [torch.empty(B, device=DEV) for _ in range(3)]
+    _lib.embed_sw_affine(Qd, Kd, ln, *params, offsets=off, scores=S, value=out[0], map=out[1],
+                         grad_open=out[2], grad_ext=out[3], grad_temp=out[4])
+    return off, S, out
+
+
+@pytest.mark.parametrize("case", FUSED_CASES, ids=lambda c: f"B{c[0]}_{c[1]}x{c[2]}_D{c[3]}")
+def test_fused_sweeps_equal_score_kernel_plus_sweeps_bitwise(case):
+    """The sweeps that generate their scores on the matrix cores give the
+    bits of the score kernel followed by the same sweep launch, and never
+    write the score buffer for pairs they keep."""
+    B, Lq, Lk, D, lens = case
+    print(f"[gpu case] fused embedding sweeps {case[:4]} lengths {lens}", flush=True)
+    Q, K = _inputs(B, Lq, Lk, D, seed=B * 13 + D)
+    Qd, Kd = Q.to(DEV), K.to(DEV)
+    ln = torch.tensor(lens, dtype=torch.int32).to(DEV)
+    params = (-4.0, -0.5, 0.25)
+    off_u, _, ref = _unfused(Qd, Kd, ln, B, Lq, Lk, params)
+    off_f, S_f, got = _fused(Qd, Kd, ln, B, Lq, Lk, params)
+    torch.cuda.synchronize()
+    assert torch.equal(off_u, off_f)
+    total = sum(int(m) * int(n) for m, n in lens)
+    for k, (a, b) in enumerate(zip(ref, got)):
+        if k == 1:
+            a, b = a[:total], b[:total]
+        assert torch.equal(a, b), k
+    assert torch.isnan(S_f).all()  # no pair was handed to the log-domain sweeps
+
+
+def test_fused_flagged_pair_scores_written_for_the_log_domain_redo():
+    """A pair whose scores jump from -80 T to +80 T between adjacent columns
+    leaves the exponent-tracked range: the fused kernel writes its scores to
+    the packed buffer and the log-domain sweeps redo it, as in the score
+    kernel + sweep path; the other pair stays in the fused kernel."""
+    B, Lq, Lk, D = 2, 40, 50, 32
+    print("[gpu case] fused embedding sweeps, flagged pair", flush=True)
+    Q, K = _inputs(B, Lq, Lk, D, seed=77)
+    Q[0] = 0.0
+    K[0] = 0.0
+    Q[0, :, 0] = 4.0
+    K[0, :, 0] = -5.0
+    K[0, 25, 0] = 5.0  # column 25: +20 = 80 T, every other column -20 = -80 T
+    Qd, Kd = Q.to(DEV), K.to(DEV)
+    lens = [[40, 50], [40, 50]]
+    ln = torch.tensor(lens, dtype=torch.int32).to(DEV)
+    params = (-4.0, -0.5, 0.25)
+    off_u, S_u, ref = _unfused(Qd, Kd, ln, B, Lq, Lk, params)
+    off_f, S_f, got = _fused(Qd, Kd, ln, B, Lq, Lk, params)
+    torch.cuda.synchronize()
+    flagged = torch.isfinite(S_f[:2000])
+    assert flagged.all() and torch.isnan(S_f[2000:4000]).all()  # pair 0 written, pair 1 not
+    assert torch.equal(S_f[:2000], S_u[:2000])
+    for k, (a, b) in enumerate(zip(ref, got)):
+        if k == 1:
+            a, b = a[:4000], b[:4000]
+        assert torch.equal(a, b), k
+    s64 = (Q.double() @ K.double().transpose(1, 2)).numpy()
+    from oracle import oracle as O
+    f64 = O.sw_affine_forward(s64, *params, np.array(lens, np.int32), prec="f64")
+    assert np.max(np.abs(got[0].cpu().numpy() - f64[0]) / np.maximum(1e-3, np.abs(f64[0]))) < 1e-5
+
+
+def test_soft_align_embeddings_fused_and_unfused_agree():
+    """Value and embedding gradients through autograd, fused against the
+    score kernel path under the same sweep launch."""
+    B, Lq, Lk, D = 4, 160, 140, 128
+    print("[gpu case] soft_align_embeddings fused vs unfused", flush=True)
+    Q, K = _inputs(B, Lq, Lk, D, seed=5)
+    ln = torch.tensor([[160, 140], [17, 100], [159, 1], [80, 140]], dtype=torch.int32).to(DEV)
+    outs = []
+    for fused in (True, False):
+        q = Q.to(DEV).requires_grad_(True)
+        k = K.to(DEV).requires_grad_(True)
+        try:
+            _lib.set_launch_config(4, 0, 4)
+            v = embed.soft_align_embeddings(q, k, ln, fused=fused)
+            v.sum().backward()
+        finally:
+            _lib.set_launch_config()
+        outs.append((v.detach(), q.grad, k.grad))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
