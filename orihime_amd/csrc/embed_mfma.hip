@@ -11,13 +11,13 @@
 // reduction slots of A and B are paired by the instruction, so any fixed
 // assignment of reduction indices to (q, slot) is a valid product.
 //
-// Each wave loads its operands straight from global memory: a reduction
-// slot run is 16 contiguous bytes (one dwordx4) when the reduction index is
-// the fast axis of the array.  For dQ and dK that axis is the embedding
-// dimension of Q and K, so the kernels read transposed copies (Q^T, K^T,
-// zero past each pair's length) written by embed_transpose_kernel.  The
-// map P is FP32; it enters the BF16 products as two terms, P = hi + lo with
-// hi = bf16(P) and lo = bf16(P - hi), which keeps ~16 bits of it.
+// A reduction slot run is 16 contiguous bytes (one dwordx4) when the
+// reduction index is the fast axis of the array.  For dQ and dK the
+// reduction runs over the pair's columns or rows, the slow axis of K and Q,
+// so those kernels read transposed copies (Q^T, K^T, zero past each pair's
+// length) written by embed_transpose_kernel.  The map P is FP32; it enters
+// the BF16 products as two terms, P = hi + lo with hi = bf16(P) and
+// lo = bf16(P - hi), which keeps ~16 bits of it.
 //
 // Work is dealt to the 8 XCDs in contiguous runs (work item w runs on XCD
 // w / (grid / 8)), so the tiles of one pair share an L2 and its Q and K rows
@@ -27,6 +27,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "ohm_embed.h"
 
@@ -42,8 +43,9 @@ typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
-constexpr int kRows = 128;     // output rows per workgroup (32 per wave)
-constexpr int kThreads = 256;  // four waves
+constexpr int kRows = 128;     // output rows per workgroup
+constexpr int kThreads = 256;  // four waves (scores: 32 rows each)
+constexpr int kGradThreads = 512;  // eight waves (dQ, dK: 16 rows each)
 constexpr int kXcds = 8;
 
 #define DEV static __device__ __forceinline__
@@ -59,24 +61,15 @@ DEV uint32_t bf16_bits(float x) {
     uint32_t u = __builtin_bit_cast(uint32_t, x);
     return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
 }
-DEV float bf16_value(uint32_t h) { return __builtin_bit_cast(float, h << 16); }
 
-// p = hi + lo as two BF16 vectors
+// p = hi + lo as two BF16 vectors (v_cvt_pk_bf16_f32: round to nearest even)
 DEV void split8(const float (&p)[8], bf16x8& hi, bf16x8& lo) {
-    uint32_t h[8], l[8];
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
-        h[v] = bf16_bits(p[v]);
-        l[v] = bf16_bits(p[v] - bf16_value(h[v]));
+        const __bf16 h = (__bf16)p[v];
+        hi[v] = h;
+        lo[v] = (__bf16)(p[v] - (float)h);
     }
-    u32x4 hv, lv;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        hv[v] = h[2 * v] | (h[2 * v + 1] << 16);
-        lv[v] = l[2 * v] | (l[2 * v + 1] << 16);
-    }
-    hi = __builtin_bit_cast(bf16x8, hv);
-    lo = __builtin_bit_cast(bf16x8, lv);
 }
 
 // four FP32 values as four BF16 (8 bytes)
@@ -193,9 +186,10 @@ __global__ __launch_bounds__(kThreads) void embed_scores_kernel(
 }
 
 // --------------------------------------------------------------------------
-// X [B, L, D] -> XT [B, D, L32] (L32 = L rounded up to 32, so a wave's
-// 32-wide reduction run never leaves a row), zero for rows past each pair's
-// length (lengths[2 b + side]) and in the padding
+// X [B, L, D] -> XT [B, D, L32] (L32 = L rounded up to 32, so a 32-wide
+// reduction run never leaves a row).  Pair b's columns are written up to
+// its length rounded up to 32 (zero past the length), the only ones the
+// reductions read.
 // --------------------------------------------------------------------------
 template <int D>
 __global__ __launch_bounds__(kThreads) void embed_transpose_kernel(const __bf16* __restrict__ X,
@@ -205,7 +199,7 @@ __global__ __launch_bounds__(kThreads) void embed_transpose_kernel(const __bf16*
     const int b = blockIdx.y;
     const int len = clampi(lengths[2 * b + side], 0, L);
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    if (j >= L32) return;
+    if (j >= ((len + 31) & ~31)) return;
     const __bf16* src = X + ((size_t)b * L + min(j, L - 1)) * D;
     __bf16* dst = XT + (size_t)b * D * L32 + j;
     for (int c = threadIdx.x >> 6; c < D / 8; c += kThreads / 64) {
@@ -220,29 +214,62 @@ __global__ __launch_bounds__(kThreads) void embed_transpose_kernel(const __bf16*
 }
 
 // --------------------------------------------------------------------------
-// dQ and dK: one workgroup per 128 output rows of a pair, a wave per 32 rows
-// (two 16-row blocks) x all D.
-//   dQ^T[d, i] = sum_j K^T[d, j] P[i, j]   (map rows: 8 contiguous floats)
-//   dK^T[d, j] = sum_i Q^T[d, i] P[i, j]   (map columns: 8 rows, 64-byte
-//                                           segments across 16 lanes)
-// Per reduction step of 32, the D x 32 tile of the transposed copy is the
-// same for the four waves: the workgroup loads it once (32 bytes a thread)
-// into one of two LDS buffers (rows padded to 80 bytes, so the 16 rows a
-// wave reads at once fall in different bank groups), one step ahead; each
-// wave keeps its own map values in registers, also one step ahead.  One
-// barrier per step separates the buffer's last readers from its next writer.
+// dQ and dK: one workgroup per 128 output rows of a pair, eight waves of 16
+// rows x all D.
+//   dQ^T[d, i] = sum_j K^T[d, j] P[i, j]   (map rows)
+//   dK^T[d, j] = sum_i Q^T[d, i] P[i, j]   (map columns)
+// Per reduction step of 32 the D x 32 tile of the transposed copy is shared
+// by the eight waves: the workgroup requests it two steps ahead (16 bytes a
+// thread) and stores it one step ahead into a ring of three LDS slots (rows
+// padded to 80 bytes, so the 16 rows a wave reads at once fall in different
+// bank groups); one barrier per step separates a slot's last readers from
+// its next writer.  Each wave requests its map values two steps ahead into
+// three register sets that trade roles as the loop body is unrolled three
+// times (no copies between them).
+// Reduction slots: slot v of lane q (= lane / 16) holds reduction index
+// 4q + v for v < 4 and 16 + 4q + v - 4 for v >= 4, so the dQ lanes of one
+// map row read 64 contiguous bytes per request (two requests per step); the
+// LDS tile is stored in the same order.  The order of the products inside
+// one MFMA is not the order of the indices; the sums are FP32 either way.
 // --------------------------------------------------------------------------
 constexpr int kTileRowBytes = 80;  // 64 bytes of a 32-wide BF16 run + padding
+constexpr int kTileSlots = 3;
+
+DEV f4u ld16f(const float* base, uint32_t byte_off) {
+    return *reinterpret_cast<const f4u*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+DEV float ld4f(const float* base, uint32_t byte_off) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+DEV u32x4 ld16x(const __bf16* base, uint32_t byte_off) {
+    return *reinterpret_cast<const u32x4*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// reduction index of slot v of lane group q (see above)
+DEV int red_index(int q, int v) { return v < 4 ? 4 * q + v : 12 + 4 * q + v; }
+
+// LDS position (8-byte units within a tile row) of 8-byte group g of the
+// row: group g < 4 (indices 4g .. 4g + 3) at 2g, group g >= 4 at 2(g - 4) + 1
+DEV int tile_pos(int g) { return g < 4 ? 2 * g : 2 * (g - 4) + 1; }
+
+template <int N, typename F>
+DEV void static_for(F&& f) {
+    if constexpr (N > 0) {
+        static_for<N - 1>(f);
+        f(std::integral_constant<int, N - 1>{});
+    }
+}
 
 template <int D, bool ROWS>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D <= 128 ? 3 : 2))) void embed_grad_kernel(
+__global__ __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(D <= 128 ? 4 : 2))) void embed_grad_kernel(
     const __bf16* __restrict__ XT, const float* __restrict__ P, const int* __restrict__ lengths,
     const int64_t* __restrict__ offsets, const float* __restrict__ gv, int Lq, int Lk, int L32,
     int tiles, int items, __bf16* __restrict__ out) {
     constexpr int NB = D / 16;
-    constexpr int kChunks = D * 4;  // 16-byte pieces of one D x 32 tile
-    constexpr int kPer = (kChunks + kThreads - 1) / kThreads;
-    __shared__ __align__(16) unsigned char tile[2][D * kTileRowBytes];
+    constexpr int kPieces = D * 4;  // 16-byte pieces of one D x 32 tile
+    constexpr int kPer = (kPieces + kGradThreads - 1) / kGradThreads;
+    constexpr int PD = 2;  // map values requested this many steps ahead
+    __shared__ __align__(16) unsigned char tile[kTileSlots][D * kTileRowBytes];
     const int w = work_item();
     if (w >= items) return;  // uniform over the workgroup
     const int b = w / tiles, t = w - b * tiles;
@@ -250,118 +277,121 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D <= 1
     // ROWS: the output rows are the map's rows (dQ), else its columns (dK)
     const int Lout = ROWS ? Lq : Lk, nout = ROWS ? M : N, nred = ROWS ? N : M;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-    const int o0 = t * (4 * 32) + 32 * wave;
-    f32x4 acc[2][NB];
+    const int o0 = t * kRows + 16 * wave;  // this wave's first output row
+    const int o = o0 + r;
+    f32x4 acc[NB];
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int d = 0; d < NB; ++d) acc[x][d] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int d = 0; d < NB; ++d) acc[d] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     // the workgroup runs the reduction when any of its rows is inside the pair
-    if (t * (4 * 32) < nout && nred > 0) {
+    if (t * kRows < nout && nred > 0) {
         const float* Pb = P + offsets[b];
         const bool mine = o0 < nout;  // this wave has rows inside the pair
-        bool live[2];
-        const float* pp[2];
+        const bool live = o < nout;
+        // map values of the step at s0 (wave-uniform bases, 32-bit lane offsets)
+        auto load_p = [&](int s0, float (&pv)[8]) {
+            if constexpr (ROWS) {
+                // row o, columns s0 + red_index(q, v)
+                const float* base = Pb + (size_t)o0 * N + s0;
+                const uint32_t off = (uint32_t)(r * N + 4 * q) * 4u;
+                if (live && s0 + 32 <= nred) {
+                    const f4u u0 = ld16f(base, off), u1 = ld16f(base, off + 64);
 #pragma unroll
-        for (int x = 0; x < 2; ++x) {
-            const int o = o0 + 16 * x + r;
-            live[x] = o < nout;
-            // dQ: row o of the map; dK: column o
-            pp[x] = ROWS ? Pb + (size_t)min(o, nout - 1) * N : Pb + min(o, nout - 1);
-        }
-        auto load_p = [&](int s0, float (&pv)[2][8]) {
-            const int ss = s0 + 8 * q;
-#pragma unroll
-            for (int x = 0; x < 2; ++x) {
-                if constexpr (ROWS) {
-                    if (live[x] && ss + 8 <= nred) {
-                        const f4u u0 = *reinterpret_cast<const f4u*>(pp[x] + ss);
-                        const f4u u1 = *reinterpret_cast<const f4u*>(pp[x] + ss + 4);
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) { pv[x][v] = u0[v]; pv[x][4 + v] = u1[v]; }
-                    } else {
-#pragma unroll
-                        for (int v = 0; v < 8; ++v) pv[x][v] = (live[x] && ss + v < nred) ? pp[x][ss + v] : 0.0f;
-                    }
+                    for (int v = 0; v < 4; ++v) { pv[v] = u0[v]; pv[4 + v] = u1[v]; }
                 } else {
 #pragma unroll
-                    for (int v = 0; v < 8; ++v)
-                        pv[x][v] = (live[x] && ss + v < nred) ? pp[x][(size_t)(ss + v) * N] : 0.0f;
-                }
-            }
-        };
-        // the workgroup's share of one D x 32 tile: piece c = thread + 256 k is
-        // row c / 4, bytes 16 (c % 4) of the row's 64
-        const __bf16* xt = XT + (size_t)b * D * L32;
-        auto load_tile = [&](int s0, u32x4 (&st)[kPer]) {
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-                const int c = threadIdx.x + kThreads * k;
-                if (c < kChunks)
-                    st[k] = *reinterpret_cast<const u32x4*>(xt + (size_t)(c >> 2) * L32 + s0 + 8 * (c & 3));
-            }
-        };
-        auto store_tile = [&](int buf, const u32x4 (&st)[kPer]) {
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-                const int c = threadIdx.x + kThreads * k;
-                if (c < kChunks)
-                    *reinterpret_cast<u32x4*>(&tile[buf][(c >> 2) * kTileRowBytes + 16 * (c & 3)]) = st[k];
-            }
-        };
-        u32x4 stage[kPer];
-        float pc[2][8], pn[2][8];
-        load_tile(0, stage);
-        store_tile(0, stage);
-        if (mine) load_p(0, pc);
-        __syncthreads();
-        const int nsteps = (nred + 31) / 32;
-        for (int st = 0; st < nsteps; ++st) {
-            const int buf = st & 1;
-            const bool more = st + 1 < nsteps;
-            if (more) {
-                load_tile(32 * (st + 1), stage);
-                if (mine) load_p(32 * (st + 1), pn);
-            }
-            if (mine) {
-                const unsigned char* tb = &tile[buf][r * kTileRowBytes + 16 * q];
-#pragma unroll
-                for (int x = 0; x < 2; ++x) {
-                    bf16x8 hi, lo;
-                    split8(pc[x], hi, lo);
-#pragma unroll
-                    for (int d = 0; d < NB; ++d) {
-                        const bf16x8 a = *reinterpret_cast<const bf16x8*>(tb + 16 * d * kTileRowBytes);
-                        acc[x][d] = mma(a, hi, acc[x][d]);
-                        acc[x][d] = mma(a, lo, acc[x][d]);
+                    for (int v = 0; v < 8; ++v) {
+                        const int j = red_index(q, v);
+                        pv[v] = (live && s0 + j < nred) ? ld4f(base, off + 4u * (uint32_t)(j - 4 * q)) : 0.0f;
                     }
                 }
+            } else {
+                // column o, rows s0 + red_index(q, v)
+                const float* base = Pb + (size_t)s0 * N + o0;
+                const uint32_t off = (uint32_t)(4 * q * N + r) * 4u;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const int j = red_index(q, v);
+                    const float* bv = base + (size_t)(j - 4 * q) * N;
+                    pv[v] = (live && s0 + j < nred) ? ld4f(bv, off) : 0.0f;
+                }
             }
-            if (more) {
-                store_tile(buf ^ 1, stage);
-                if (mine) {
+        };
+        // the workgroup's share of one D x 32 tile: piece c is row c / 4,
+        // bytes 16 (c % 4) of the row's 64, i.e. 8-byte groups 2 (c % 4) and
+        // 2 (c % 4) + 1
+        const __bf16* xt = XT + (size_t)b * D * L32;
+        auto load_tile = [&](int s0, u32x4 (&stg)[kPer]) {
+            const __bf16* base = xt + s0;
 #pragma unroll
-                    for (int x = 0; x < 2; ++x)
+            for (int k = 0; k < kPer; ++k) {
+                const int c = threadIdx.x + kGradThreads * k;
+                if (c < kPieces) stg[k] = ld16x(base, (uint32_t)((c >> 2) * L32 + 8 * (c & 3)) * 2u);
+            }
+        };
+        auto store_tile = [&](int slot, const u32x4 (&stg)[kPer]) {
 #pragma unroll
-                        for (int v = 0; v < 8; ++v) pc[x][v] = pn[x][v];
+            for (int k = 0; k < kPer; ++k) {
+                const int c = threadIdx.x + kGradThreads * k;
+                if (c < kPieces) {
+                    unsigned char* row = &tile[slot][(c >> 2) * kTileRowBytes];
+                    const int g = 2 * (c & 3);
+                    *reinterpret_cast<u32x2*>(row + 8 * tile_pos(g)) = u32x2{stg[k][0], stg[k][1]};
+                    *reinterpret_cast<u32x2*>(row + 8 * tile_pos(g + 1)) = u32x2{stg[k][2], stg[k][3]};
+                }
+            }
+        };
+        const int nsteps = (nred + 31) / 32;
+        u32x4 stage[kPer];
+        float ps[PD + 1][8];
+        load_tile(0, stage);
+        store_tile(0, stage);
+        if (nsteps > 1) {
+            load_tile(32, stage);
+            store_tile(1, stage);
+        }
+        if (nsteps > 2) load_tile(64, stage);
+        if (mine) {
+            static_for<PD>([&](auto J) {
+                if (J < nsteps) load_p(32 * J, ps[J]);
+            });
+        }
+        __syncthreads();
+        // step st: products from slot st % 3 and `cur`; the tile of step
+        // st + 2 goes into its slot (whose last readers passed the barrier
+        // of step st - 1), the tile of step st + 3 is requested, and the map
+        // values of step st + 2 go into `nxt`
+        auto body = [&](int st, const float (&cur)[8], float (&nxt)[8]) {
+            if (st + 2 < nsteps) store_tile((st + 2) % kTileSlots, stage);
+            if (st + 3 < nsteps) load_tile(32 * (st + 3), stage);
+            if (mine) {
+                if (st + PD < nsteps) load_p(32 * (st + PD), nxt);
+                bf16x8 hi, lo;
+                split8(cur, hi, lo);
+                const unsigned char* tb = &tile[st % kTileSlots][r * kTileRowBytes + 16 * q];
+#pragma unroll
+                for (int d = 0; d < NB; ++d) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(tb + 16 * d * kTileRowBytes);
+                    acc[d] = mma(a, hi, acc[d]);
+                    acc[d] = mma(a, lo, acc[d]);
                 }
             }
             __syncthreads();
+        };
+        for (int st = 0; st < nsteps; st += PD + 1) {
+            static_for<PD + 1>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                if (st + j < nsteps) body(st + j, ps[j], ps[(j + PD) % (PD + 1)]);
+            });
         }
     }
-    if (o0 >= Lout) return;
-    // acc[x][d][v] = grad[row o0 + 16 x + r][16 d + 4 q + v]; zero past the length
+    if (o >= Lout) return;
+    // acc[d][v] = grad[row o][16 d + 4 q + v]; zero past the length
+    const float g = o < nout ? gv[b] : 0.0f;
+    __bf16* dst = out + ((size_t)b * Lout + o) * D + 4 * q;
 #pragma unroll
-    for (int x = 0; x < 2; ++x) {
-        const int o = o0 + 16 * x + r;
-        if (o >= Lout) continue;
-        const float g = o < nout ? gv[b] : 0.0f;
-        __bf16* dst = out + ((size_t)b * Lout + o) * D + 4 * q;
-#pragma unroll
-        for (int d = 0; d < NB; ++d)
-            *reinterpret_cast<u32x2*>(dst + 16 * d) =
-                pack4(g * acc[x][d][0], g * acc[x][d][1], g * acc[x][d][2], g * acc[x][d][3]);
-    }
+    for (int d = 0; d < NB; ++d)
+        *reinterpret_cast<u32x2*>(dst + 16 * d) =
+            pack4(g * acc[d][0], g * acc[d][1], g * acc[d][2], g * acc[d][3]);
 }
 
 // --------------------------------------------------------------------------
@@ -484,7 +514,7 @@ extern "C" ohm_status ohm_embed_grads(const void* Q, const void* K, const int32_
         if (Lq > 0) {
             const int tiles = (Lq + kRows - 1) / kRows;
             const int64_t items = (int64_t)B * tiles;
-            hipLaunchKernelGGL((embed_grad_kernel<DD, true>), dim3(round_grid(items)), dim3(kThreads), 0, s,
+            hipLaunchKernelGGL((embed_grad_kernel<DD, true>), dim3(round_grid(items)), dim3(kGradThreads), 0, s,
                                kt, map, lengths, offsets, grad_value, Lq, Lk, lk32, tiles, (int)items,
                                static_cast<__bf16*>(dQ));
             if ((r = launched("embed dQ kernel")) != OHM_OK) return r;
@@ -492,7 +522,7 @@ extern "C" ohm_status ohm_embed_grads(const void* Q, const void* K, const int32_
         if (Lk > 0) {
             const int tiles = (Lk + kRows - 1) / kRows;
             const int64_t items = (int64_t)B * tiles;
-            hipLaunchKernelGGL((embed_grad_kernel<DD, false>), dim3(round_grid(items)), dim3(kThreads), 0, s,
+            hipLaunchKernelGGL((embed_grad_kernel<DD, false>), dim3(round_grid(items)), dim3(kGradThreads), 0, s,
                                qt, map, lengths, offsets, grad_value, Lq, Lk, lq32, tiles, (int)items,
                                static_cast<__bf16*>(dK));
             if ((r = launched("embed dK kernel")) != OHM_OK) return r;
