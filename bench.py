@@ -292,37 +292,38 @@ class Launch:
         torch.cuda.synchronize()
 
 
-def profiled_traffic(workload: str, kernel: str):
+def _profiled(name: str, tag: str, kernel: str, shape=None):
+    """Per-launch figure of `kernel` from the `tag:` line of a committed
+    counter profile: the entry of the launch shape (waves, checkpoint stride)
+    when given and present, else the first entry of that kernel."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith(tag + ":"):
+            table = json.loads(line.split(":", 1)[1])
+            if shape is not None:
+                exact = KERNEL_NAMES[kernel].replace("CK", str(shape[1])).replace("W", str(shape[0]))
+                if exact in table:
+                    return int(table[exact]), os.path.relpath(path, ROOT)
+            key = KERNEL_NAMES[kernel].split("<")[0] + "<" + ("true" if kernel == "forward" else "")
+            for k, v in table.items():
+                if k.startswith(key):
+                    return int(v), os.path.relpath(path, ROOT)
+    return None, None
+
+
+def profiled_traffic(workload: str, kernel: str, shape=None):
     """HBM bytes per launch of `kernel` from the committed byte-counter profile
     of this workload (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, calibrated with
     a device copy of known size), or None when there is none."""
-    path = os.path.join(ROOT, "profiles", f"sw_affine_{workload}_hbm_traffic.md")
-    if not os.path.exists(path):
-        return None, None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            table = json.loads(line.split(":", 1)[1])
-            key = KERNEL_NAMES[kernel].split("<")[0] + "<" + ("true" if kernel == "forward" else "")
-            for k, v in table.items():
-                if k.startswith(key):
-                    return int(v), os.path.relpath(path, ROOT)
-    return None, None
+    return _profiled(f"sw_affine_{workload}_hbm_traffic.md", "traffic_bytes_per_launch", kernel, shape)
 
 
-def profiled_valu(workload: str, kernel: str):
+def profiled_valu(workload: str, kernel: str, shape=None):
     """Wave-level VALU instructions per launch of `kernel` from the committed
     SQ-counter profile of this workload (rocprofv3 SQ_INSTS_VALU), or None."""
-    path = os.path.join(ROOT, "profiles", f"sw_affine_{workload}_instructions.md")
-    if not os.path.exists(path):
-        return None, None
-    for line in open(path):
-        if line.startswith("valu_instructions_per_launch:"):
-            table = json.loads(line.split(":", 1)[1])
-            key = KERNEL_NAMES[kernel].split("<")[0] + "<" + ("true" if kernel == "forward" else "")
-            for k, v in table.items():
-                if k.startswith(key):
-                    return int(v), os.path.relpath(path, ROOT)
-    return None, None
+    return _profiled(f"sw_affine_{workload}_instructions.md", "valu_instructions_per_launch", kernel, shape)
 
 
 def timed(torch, fn, steps):
@@ -1052,8 +1053,9 @@ def main():
     step_bytes = sum(BYTES_PER_CELL[k] for k in sweeps)
     step_roof = min(hbm_peak * 1e9 / step_bytes, trans_peak / step_trans) / 1e9
     wl_name = "ragged2048" if args.workload == "ragged" else args.workload
-    traffic, traffic_src = profiled_traffic(wl_name, dom)
-    valu, valu_src = profiled_valu(wl_name, dom)
+    shape = (tuned["waves"], tuned["checkpoint_stride"]) if isinstance(tuned, dict) else None
+    traffic, traffic_src = profiled_traffic(wl_name, dom, shape)
+    valu, valu_src = profiled_valu(wl_name, dom, shape)
     issue_peak = max(rates["fma"], rates["dpp"]) * 1e12 / 64 if rates else VALU_LANE_OPS / 64
     step_achieved = step_trans * cells / (ms * 1e-3)
     dom_hbm = BYTES_PER_CELL[dom] * cells / t_dom / 1e9
