@@ -261,6 +261,43 @@ def test_affine_fused_forward_reverse_kernel_vs_oracle(shape, monkeypatch):
     assert rel_err(h(out[2]), ref[2], 1.0) < 1e-4 and rel_err(h(out[3]), ref[3], 1.0) < 1e-4
 
 
+# widths on both sides of the 16- and 32-column boundaries: the fused
+# launch fetches two chunks' scores per request (forward: chunks k+1, k+2 at
+# even k; reverse: chunks k, k-1 from the last chunk down), so the chunk
+# count's parity and the last, partial chunk are the cases to cover
+PAIRED_WIDTHS = [1, 15, 16, 17, 31, 32, 33, 47, 48, 49, 80, 111]
+
+
+@pytest.mark.parametrize("L2", PAIRED_WIDTHS)
+def test_fused_paired_score_loads_vs_single_windows(L2, monkeypatch):
+    """The fused launch (paired score loads in both sweeps) against the
+    separate launches (single windows in the forward: maps bitwise equal)
+    and the team kernels (single windows in both sweeps; their partition is
+    reduced in another order, so maps agree to a few FP32 ulps); rows 1, 63,
+    64, 65 and 130 cover partial strips."""
+    go, ge, T = PARAMS[1]
+    rng = np.random.default_rng(stable_seed("paired", L2))
+    L1 = 130
+    s = (rng.normal(size=(5, L1, L2)) * 2 - 0.5).astype(np.float32)
+    ln = np.array([[130, L2], [1, L2], [63, L2], [64, max(1, L2 - 1)], [65, L2]], np.int32)
+    st, lt = g(s), g(ln)
+    case(f"fused paired loads vs single windows, L2={L2}, lengths={ln.tolist()}")
+    monkeypatch.setenv("OHM_TEAMS", "1")
+    team = [t.clone() for t in torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)]
+    monkeypatch.setenv("OHM_TEAMS", "0")
+    sep = [t.clone() for t in torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)]
+    monkeypatch.delenv("OHM_TEAMS")
+    monkeypatch.setenv("OHM_FUSED", "1")
+    out = torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)
+    assert torch.equal(out[1], sep[1])
+    assert abs_err(h(out[1]), h(team[1])) < 2e-6
+    for k in (0, 2, 3, 4):
+        assert rel_err(h(out[k]), h(team[k]), 1.0) < 1e-5
+    ref = O.sw_affine_forward(s, go, ge, T, ln)
+    assert rel_err(h(out[0]), ref[0], 1e-3) < 1e-5
+    assert abs_err(h(out[1]), ref[1]) < 1e-4
+
+
 def test_affine_fused_kernel_leaves_flagged_pairs_to_the_log_domain(monkeypatch):
     """Under the fused launch a pair that leaves the scaled range skips the
     fused reverse and is redone by the log-domain sweeps, as with separate
