@@ -1038,8 +1038,11 @@ def main():
     log(f"measured HBM copy bandwidth: {hbm_peak:.0f} GB/s")
     kd = kernel_durations(torch, _lib, hip, main_launch, args.steps)
     log("kernel durations (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in kd.items()))
+    # untuned: the library's automatic choice, the fused launch for the
+    # per-pair batches (not the long pairs' grid kernels)
     fused = isinstance(tuned, dict) and tuned["teams"] == "fused" or (
-        not isinstance(tuned, dict) and os.environ.get("OHM_FUSED") == "1")
+        not isinstance(tuned, dict) and os.environ.get("OHM_FUSED", "1") != "0"
+        and args.workload != "long")
     if fused:
         # the fused launch runs between the plan and forward marks
         kd["fused"] = kd["forward"]

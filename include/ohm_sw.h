@@ -89,7 +89,8 @@ ohm_status ohm_set_stage_events(void* const* events, int32_t count);
  * kernels, 2 per-pair workgroup kernels, 3 grid kernels (the strips of every
  * pair spread over all CUs; batches of at most 64 pairs), 4 per-pair
  * workgroup kernels that run each pair's forward and reverse sweep in one
- * launch (affine sweeps).  Results do not
+ * launch (affine sweeps; the automatic choice where the per-pair kernels
+ * run, 2 keeps separate forward and reverse launches).  Results do not
  * depend on the configuration, only the speed.  Process-wide and
  * unsynchronised like ohm_set_stage_events; orihime_amd/tune.py chooses it
  * by timing the caller's batch. */

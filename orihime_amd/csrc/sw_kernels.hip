@@ -1494,13 +1494,18 @@ ohm_status launch_grid(K kernel, size_t lds, const Launch& L) {
     return check_launch("grid kernel");
 }
 
-// Fused forward + reverse per pair (sw_affine_scaled_fused_kernel) for the
-// per-pair workgroup path: teams = 4 in ohm_set_launch_config or OHM_FUSED=1.
-bool use_fused() {
+// Fused forward + reverse per pair (sw_affine_scaled_fused_kernel): the
+// automatic choice wherever the per-pair workgroup kernels run (measured
+// faster than separate launches on 2048 ragged pairs up to 1024^2, 256
+// pairs of 512^2 and 512 pairs up to 768^2); teams = 4 / 2 in
+// ohm_set_launch_config or OHM_FUSED=1 / 0 force it on / off.
+bool use_fused(int B, int max_strips) {
     const int cfg = g_cfg_teams.load();
     if (cfg == 4) return true;
     if (cfg != 0) return false;
-    return env_int("OHM_FUSED", 0) == 1;
+    const int env = env_int("OHM_FUSED", -1);
+    if (env >= 0) return env == 1;
+    return !use_teams(B, max_strips);
 }
 
 // The scaled affine sweeps need representable gap factors 2^(g log2(e) / T)
@@ -1533,7 +1538,7 @@ ohm_status forward_impl(bool affine, const float* scores, const int32_t* lengths
     const bool lin_scaled = !affine && use_scaled(L.a) && env_int("OHM_LINEAR_SCALED", 1) != 0;
     const bool lin_grid = lin_scaled && use_grid(L.a);
     const bool grid = scaled && use_grid(L.a);
-    const bool fused = scaled && !grid && need_bwd && use_fused();
+    const bool fused = scaled && !grid && need_bwd && use_fused(B, L.a.max_strips);
     const bool team = scaled && !grid && !fused && use_teams(B, L.a.max_strips);
     if (scaled) {
         // scaled sweep for every pair, then the log-domain sweep for the pairs

@@ -215,8 +215,10 @@ def test_affine_log_domain_sweeps_vs_oracle(shape, params, monkeypatch):
 @pytest.mark.parametrize("shape", [(3, 63, 65), (2, 257, 31), (2, 191, 193)], ids=lambda s: "x".join(map(str, s)))
 def test_affine_per_pair_workgroup_kernels_vs_oracle(shape, monkeypatch):
     """The scaled sweeps' per-pair workgroup kernels (used for pairs too tall
-    for the wave-scheduled team kernels) against the oracle."""
+    for the wave-scheduled team kernels), as separate forward and reverse
+    launches, against the oracle (the fused launch: below)."""
     monkeypatch.setenv("OHM_TEAMS", "0")
+    monkeypatch.setenv("OHM_FUSED", "0")
     go, ge, T = PARAMS[1]
     s, _, _, ln = _inputs(*shape, seed=stable_seed(shape, "wg"))
     case(f"affine per-pair workgroups shape={shape}")
@@ -251,6 +253,7 @@ def test_affine_fused_forward_reverse_kernel_vs_oracle(shape, monkeypatch):
     st, lt = g(s), g(ln)
     case(f"affine fused forward+reverse shape={shape} lengths={ln.tolist()}")
     monkeypatch.setenv("OHM_TEAMS", "0")
+    monkeypatch.setenv("OHM_FUSED", "0")
     sep = [t.clone() for t in torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)]
     monkeypatch.setenv("OHM_FUSED", "1")
     out = torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)
@@ -285,6 +288,7 @@ def test_fused_paired_score_loads_vs_single_windows(L2, monkeypatch):
     monkeypatch.setenv("OHM_TEAMS", "1")
     team = [t.clone() for t in torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)]
     monkeypatch.setenv("OHM_TEAMS", "0")
+    monkeypatch.setenv("OHM_FUSED", "0")
     sep = [t.clone() for t in torch.ops.orihime.soft_sw_affine_with_grads(st, go, ge, T, lt)]
     monkeypatch.delenv("OHM_TEAMS")
     monkeypatch.setenv("OHM_FUSED", "1")
