@@ -1500,6 +1500,10 @@ ohm_status launch_grid(K kernel, size_t lds, const Launch& L) {
 // pairs of 512^2 and 512 pairs up to 768^2); teams = 4 / 2 in
 // ohm_set_launch_config or OHM_FUSED=1 / 0 force it on / off.
 bool use_fused(int B, int max_strips) {
+    // its progress words (forward and reverse, 2 x max_strips ints) stay
+    // within the default dynamic LDS of a launch; taller batches run
+    // separate launches (one set of words each)
+    if (2 * (size_t)max_strips * sizeof(int) > 48 * 1024) return false;
     const int cfg = g_cfg_teams.load();
     if (cfg == 4) return true;
     if (cfg != 0) return false;
