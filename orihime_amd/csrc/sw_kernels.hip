@@ -396,11 +396,16 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
     __shared__ int64_t s_carry[1];
     __shared__ float s_key[kSortMax];
     __shared__ int s_idx[kSortMax];
+    __shared__ int s_kmin, s_kmax;  // key range (bit patterns of non-negative floats)
     const int tid = tid_x();
     if (tid < 64) claims[tid] = 0;
     if (grid)
         for (int q = tid; q < grid_ints; q += kPlanThreads) grid[q] = 0;
-    if (tid == 0) s_carry[0] = 0;
+    if (tid == 0) {
+        s_carry[0] = 0;
+        s_kmin = 0x7fffffff;
+        s_kmax = 0;
+    }
     __syncthreads();
     const bool sort = B <= kSortMax;
     for (int base = 0; base < B; base += kPlanThreads) {
@@ -452,8 +457,11 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
             }
             desc[b] = pd;
             if (sort) {
-                s_key[b] = (float)ns * (float)(L2 + 63);
+                const float key = (float)ns * (float)(L2 + 63);
+                s_key[b] = key;
                 s_idx[b] = b;
+                atomicMin(&s_kmin, __builtin_bit_cast(int, key));
+                atomicMax(&s_kmax, __builtin_bit_cast(int, key));
             } else {
                 order[b] = b;
             }
@@ -463,6 +471,11 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
         __syncthreads();
     }
     if (!sort) return;
+    if (s_kmin == s_kmax) {
+        // every pair costs the same: the order (ties by index) is the identity
+        for (int q = tid; q < B; q += kPlanThreads) order[q] = q;
+        return;
+    }
     int n = 1;
     while (n < B) n <<= 1;
     for (int q = B + tid; q < n; q += kPlanThreads) {
@@ -470,9 +483,16 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
         s_idx[q] = q;
     }
     __syncthreads();
-    // Bitonic sort, descending by cost, ties by index.
+    // Bitonic sort, descending by cost, ties by index.  Stages whose partners
+    // are 64 or more positions apart exchange through LDS (a barrier each);
+    // the rest of a level runs in registers, partners exchanged by lane
+    // shuffles (position q = tid + 1024 m lives in lane tid % 64 of wave
+    // tid / 64, its partner q ^ j, j < 64, in the same wave).  Every stage
+    // swaps partner positions, so the result is a permutation.
+    constexpr int kPer = kSortMax / kPlanThreads;
     for (int k = 2; k <= n; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
+        int j = k >> 1;
+        for (; j >= kWave; j >>= 1) {
             for (int q = tid; q < n; q += kPlanThreads) {
                 int p = q ^ j;
                 if (p > q) {
@@ -488,6 +508,41 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
             }
             __syncthreads();
         }
+        float rk[kPer];
+        int ri[kPer];
+#pragma unroll
+        for (int m = 0; m < kPer; ++m) {
+            const int q = tid + kPlanThreads * m;
+            rk[m] = q < n ? s_key[q] : -1.0f;
+            ri[m] = q < n ? s_idx[q] : q;
+        }
+        for (; j > 0; j >>= 1) {
+#pragma unroll
+            for (int m = 0; m < kPer; ++m) {
+                const int q = tid + kPlanThreads * m;
+                const float kp = __shfl_xor(rk[m], j, kWave);
+                const int ip = __shfl_xor(ri[m], j, kWave);
+                const bool desc_block = ((q & ~j) & k) == 0;  // block of the pair's lower position
+                const bool mine_first = rk[m] > kp || (rk[m] == kp && ri[m] < ip);
+                const bool lower = (q & j) == 0;
+                // the lower position takes the first element of a descending
+                // block and the second of an ascending one
+                const bool keep = lower ? (mine_first == desc_block) : (mine_first != desc_block);
+                if (!keep) {
+                    rk[m] = kp;
+                    ri[m] = ip;
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < kPer; ++m) {
+            const int q = tid + kPlanThreads * m;
+            if (q < n) {
+                s_key[q] = rk[m];
+                s_idx[q] = ri[m];
+            }
+        }
+        __syncthreads();
     }
     for (int q = tid; q < B; q += kPlanThreads) order[q] = s_idx[q];
 }
