@@ -396,16 +396,12 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
     __shared__ int64_t s_carry[1];
     __shared__ float s_key[kSortMax];
     __shared__ int s_idx[kSortMax];
-    __shared__ int s_kmin, s_kmax;  // key range (bit patterns of non-negative floats)
+    __shared__ int s_differs[kPlanThreads / kWave];  // per wave: a key differs from pair 0's
     const int tid = tid_x();
     if (tid < 64) claims[tid] = 0;
     if (grid)
         for (int q = tid; q < grid_ints; q += kPlanThreads) grid[q] = 0;
-    if (tid == 0) {
-        s_carry[0] = 0;
-        s_kmin = 0x7fffffff;
-        s_kmax = 0;
-    }
+    if (tid == 0) s_carry[0] = 0;
     __syncthreads();
     const bool sort = B <= kSortMax;
     for (int base = 0; base < B; base += kPlanThreads) {
@@ -457,11 +453,8 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
             }
             desc[b] = pd;
             if (sort) {
-                const float key = (float)ns * (float)(L2 + 63);
-                s_key[b] = key;
+                s_key[b] = (float)ns * (float)(L2 + 63);
                 s_idx[b] = b;
-                atomicMin(&s_kmin, __builtin_bit_cast(int, key));
-                atomicMax(&s_kmax, __builtin_bit_cast(int, key));
             } else {
                 order[b] = b;
             }
@@ -471,10 +464,19 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
         __syncthreads();
     }
     if (!sort) return;
-    if (s_kmin == s_kmax) {
-        // every pair costs the same: the order (ties by index) is the identity
-        for (int q = tid; q < B; q += kPlanThreads) order[q] = q;
-        return;
+    {
+        bool differs = false;
+        for (int q = tid; q < B; q += kPlanThreads) differs |= s_key[q] != s_key[0];
+        const uint64_t m = __builtin_amdgcn_ballot_w64(differs);
+        if ((tid & (kWave - 1)) == 0) s_differs[tid / kWave] = m != 0;
+        __syncthreads();
+        bool any = false;
+        for (int w = 0; w < kPlanThreads / kWave; ++w) any |= s_differs[w] != 0;
+        if (!any) {
+            // every pair costs the same: the order (ties by index) is the identity
+            for (int q = tid; q < B; q += kPlanThreads) order[q] = q;
+            return;
+        }
     }
     int n = 1;
     while (n < B) n <<= 1;
@@ -516,9 +518,12 @@ __global__ __launch_bounds__(kPlanThreads) void sw_plan_kernel(
             rk[m] = q < n ? s_key[q] : -1.0f;
             ri[m] = q < n ? s_idx[q] : q;
         }
+        const int wbase = tid & ~(kWave - 1);  // the wave's first position (m = 0)
         for (; j > 0; j >>= 1) {
 #pragma unroll
             for (int m = 0; m < kPer; ++m) {
+                // a wave whose positions are all padding skips them (wave-uniform)
+                if (wbase + kPlanThreads * m >= n) continue;
                 const int q = tid + kPlanThreads * m;
                 const float kp = __shfl_xor(rk[m], j, kWave);
                 const int ip = __shfl_xor(ri[m], j, kWave);
