@@ -7,8 +7,8 @@ stride: anti-diagonal steps per recomputed block, 4, 8 or 16) — only their
 speed does, and the best choice depends on the batch: many mid-size pairs
 favour 4 waves per pair, few long pairs the team or grid kernels; a shorter
 stride stores more checkpoints and holds fewer registers per block.
-``autotune`` times the candidate shapes on the caller's own batch (one
-forward+backward call each, after a warm-up), then the strides under the
+``autotune`` times the candidate shapes on the caller's own batch (the best
+of five forward+backward calls each, after a warm-up), then the strides under the
 fastest shape (affine sweeps only: the linear sweeps keep full tables),
 applies the winner through ``ohm_set_launch_config`` and
 ``ohm_set_checkpoint_stride`` and caches it per batch signature (pair count
@@ -86,7 +86,7 @@ def _time(fn, reps):
 
 
 def autotune(scores, gap_open, gap_ext=None, temperature=1.0, *, lengths=None, offsets=None,
-             reps: int = 3, candidates=CANDIDATES, strides=STRIDES,
+             reps: int = 5, candidates=CANDIDATES, strides=STRIDES,
              apply: bool = True) -> LaunchConfig:
     """Choose the fastest launch shape for this batch (affine when ``gap_ext``
     is given, linear otherwise) and apply it.  ``scores`` is a GPU tensor in
